@@ -1,0 +1,133 @@
+/*
+ * rz.h -- C ABI of the MI355X (gfx950) rasterizer: clip-space vertices to the
+ * resolved framebuffer, 4x RGSS MSAA, strict-LESS depth, perspective-correct
+ * attributes, bilinear RGBA8 textures, box-filter resolve.
+ *
+ * It is the drop-in for rusterizer's draw API. Each entry point names the
+ * reference item it replaces (paths relative to the rusterizer crate).
+ *
+ * Conventions
+ *  - Every call returns 0 on success or a negative RZ_ERR_* code; nothing
+ *    aborts. rz_last_error() gives the message of the last failure.
+ *  - One context per host thread; contexts are independent. All work of a
+ *    context runs in order on its own HIP stream (rz_stream()).
+ *  - Host arrays passed in are copied during the call; meshes and textures
+ *    are device-resident and owned by the context until destroyed.
+ *  - Draws accumulate into the frame until rz_resolve*, which resolves the
+ *    4x MSAA buffers to ARGB and clears them, as Rasterizer::framebuffer does.
+ */
+#ifndef RZ_H
+#define RZ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rz_ctx rz_ctx;
+typedef struct rz_mesh rz_mesh;
+
+/* Vertex shaders (src/main.rs:147-152: projection * view * world * v). */
+enum rz_vertex_shader { RZ_VS_MVP = 0 };
+
+/* Fragment shaders (src/main.rs:67-77). */
+enum rz_fragment_shader {
+    RZ_FS_TEXTURE = 0, /* uniforms.get_texture(0).sample(uv)        */
+    RZ_FS_COLOR = 1,   /* interpolated vertex colour                 */
+    RZ_FS_DEBUG = 2    /* Color::grayscale(frag_coords.depths[0])    */
+};
+
+enum rz_status {
+    RZ_OK = 0,
+    RZ_ERR_ARG = -1,      /* bad argument (null, size, enum, index)     */
+    RZ_ERR_HIP = -2,      /* HIP runtime error                           */
+    RZ_ERR_STATE = -3,    /* call not valid now (e.g. missing texture)   */
+    RZ_ERR_NOMEM = -4,    /* device allocation failed                    */
+    RZ_ERR_LIMIT = -5     /* a documented limit was exceeded             */
+};
+
+/* Bits reported by rz_frame_flags() for the last resolved frame: inputs on
+ * which the reference would panic; the device applies a defined behaviour. */
+#define RZ_FLAG_TEXEL_OOB 1u   /* texel index past the texture: last texel used (texture.rs:51-61) */
+#define RZ_FLAG_CLIP_FEWER 4u  /* clipper returned 1-2 vertices: no triangle (clipping.rs:183)   */
+
+/* Pipeline stages timed by rz_profile_read(). */
+enum rz_stage { RZ_STAGE_GEOMETRY = 0, RZ_STAGE_BINNING = 1, RZ_STAGE_RASTER = 2, RZ_NSTAGES = 3 };
+
+/* ------------------------------------------------------------------ context */
+
+/* Renderer::new / Rasterizer::new (src/render.rs:48-69, src/rasterizer/mod.rs:273-281),
+ * headless: no window. width, height in [1, 16384]. Uses the current HIP device. */
+int rz_create(uint32_t width, uint32_t height, rz_ctx **out);
+/* Drop for Renderer. Accepts NULL. */
+int rz_destroy(rz_ctx *ctx);
+/* Message of the last failed call on ctx ("" if none); static text for NULL ctx. */
+const char *rz_last_error(const rz_ctx *ctx);
+/* The HIP stream (hipStream_t) all work of ctx runs on. */
+void *rz_stream(const rz_ctx *ctx);
+
+/* Depth the MSAA buffers are cleared to (CLEAR_DEPTH, src/rasterizer/buffers.rs:6;
+ * default FLT_MAX). Takes effect from the next frame. */
+int rz_set_clear_depth(rz_ctx *ctx, float depth);
+
+/* Uniforms::write_block (src/uniform.rs:43-45): world, view, projection as
+ * row-major f32[16] (Mat4.array, src/math/matrix.rs:8-14). Applies to later draws. */
+int rz_set_uniforms(rz_ctx *ctx, const float world[16], const float view[16], const float proj[16]);
+
+/* Uniforms::bind_texture (src/uniform.rs:29-33): index must equal the number
+ * of textures bound so far. rgba8 is th rows of tw texels, (0,0) top-left
+ * (src/texture.rs:47-63); copied to the device. */
+int rz_bind_texture(rz_ctx *ctx, uint32_t index, const uint8_t *rgba8, uint32_t tw, uint32_t th);
+
+/* --------------------------------------------------------------------- mesh */
+
+/* Mesh<WorldSpace> (src/mesh.rs:5-12) uploaded once and kept resident:
+ * pos nv*3 f32, color nv*4 f32 (or NULL: zeros), uv nv*2 f32 (or NULL: zeros),
+ * idx ni u32 (ni a multiple of 3, every index < nv). Host arrays are copied. */
+int rz_mesh_create(rz_ctx *ctx, const float *pos, const float *color, const float *uv, uint32_t nv,
+                   const uint32_t *idx, uint64_t ni, rz_mesh **out);
+/* Same, from device pointers already in HBM (copied device-to-device). */
+int rz_mesh_create_device(rz_ctx *ctx, const float *pos, const float *color, const float *uv,
+                          uint32_t nv, const uint32_t *idx, uint64_t ni, rz_mesh **out);
+/* Fails with RZ_ERR_STATE while a draw of the mesh waits for rz_resolve*. */
+int rz_mesh_destroy(rz_ctx *ctx, rz_mesh *mesh);
+
+/* -------------------------------------------------------------------- draws */
+
+/* Renderer::render(mesh, vertex_shader, fragment_shader) (src/render.rs:98-114):
+ * queues vertex shading, clipping and triangle setup of every triangle of the
+ * mesh with the current uniforms, in index order. Asynchronous. */
+int rz_draw(rz_ctx *ctx, const rz_mesh *mesh, int vertex_shader, int fragment_shader);
+
+/* Rasterizer::framebuffer (src/rasterizer/mod.rs:478-522): rasterizes all
+ * queued draws in order, resolves the 4x MSAA buffers into width*height
+ * 0xAARRGGBB words, clears them for the next frame. Blocks until done. */
+int rz_resolve(rz_ctx *ctx, uint32_t *argb_out);
+/* Same, writing to device memory (width*height u32). Returns after the frame
+ * finished on the device. */
+int rz_resolve_device(rz_ctx *ctx, uint32_t *argb_dev);
+/* Debug variant: also copies the pre-resolve MSAA buffers to host memory
+ * (width*height*4 colour words / depths, sample-major per pixel;
+ * ColorBuffer/DepthBuffer of src/rasterizer/buffers.rs:82-157). Any pointer may be NULL. */
+int rz_resolve_msaa(rz_ctx *ctx, uint32_t *argb_out, uint32_t *color4_out, float *depth4_out);
+/* RZ_FLAG_* bits raised while rendering the last resolved frame. */
+uint32_t rz_frame_flags(const rz_ctx *ctx);
+/* Counts of the last resolved frame: [input triangles, triangles clipped,
+ * rasterizer triangles, coarse bin entries, frame replays]. */
+int rz_frame_stats(const rz_ctx *ctx, uint64_t out[5]);
+
+/* ------------------------------------------------------------ tuning/timing */
+
+/* Coarse bin side in pixels (multiple of 16, 0 = automatic). */
+int rz_set_coarse_bin(rz_ctx *ctx, uint32_t pixels);
+/* Enable (1) / disable (0) per-stage HIP event timing; enabling resets sums. */
+int rz_profile(rz_ctx *ctx, int enable);
+/* Summed milliseconds per stage (RZ_NSTAGES floats) and the number of frames. */
+int rz_profile_read(const rz_ctx *ctx, float ms[RZ_NSTAGES], uint32_t *frames);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,67 @@
+"""ctypes binding of librz.so (the HIP rasterizer's C ABI, include/rz.h).
+
+Loading fails loudly when the library has not been built: there is no
+CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "librz.so")
+
+RZ_VS_MVP = 0
+RZ_FS_TEXTURE, RZ_FS_COLOR, RZ_FS_DEBUG = 0, 1, 2
+RZ_FLAG_TEXEL_OOB, RZ_FLAG_CLIP_FEWER = 1, 4
+RZ_STAGE_GEOMETRY, RZ_STAGE_BINNING, RZ_STAGE_RASTER, RZ_NSTAGES = 0, 1, 2, 3
+
+# every entry point of include/rz.h: name -> (restype, argtypes)
+_vp = C.c_void_p
+_f = C.POINTER(C.c_float)
+_u32 = C.POINTER(C.c_uint32)
+_u8 = C.POINTER(C.c_uint8)
+SIGNATURES = {
+    "rz_create": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
+    "rz_destroy": (C.c_int, [_vp]),
+    "rz_last_error": (C.c_char_p, [_vp]),
+    "rz_stream": (_vp, [_vp]),
+    "rz_set_clear_depth": (C.c_int, [_vp, C.c_float]),
+    "rz_set_uniforms": (C.c_int, [_vp, _f, _f, _f]),
+    "rz_bind_texture": (C.c_int, [_vp, C.c_uint32, _u8, C.c_uint32, C.c_uint32]),
+    "rz_mesh_create": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp, C.c_uint64, C.POINTER(_vp)]),
+    "rz_mesh_create_device": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp, C.c_uint64, C.POINTER(_vp)]),
+    "rz_mesh_destroy": (C.c_int, [_vp, _vp]),
+    "rz_draw": (C.c_int, [_vp, _vp, C.c_int, C.c_int]),
+    "rz_resolve": (C.c_int, [_vp, _vp]),
+    "rz_resolve_device": (C.c_int, [_vp, _vp]),
+    "rz_resolve_msaa": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "rz_frame_flags": (C.c_uint32, [_vp]),
+    "rz_frame_stats": (C.c_int, [_vp, C.POINTER(C.c_uint64)]),
+    "rz_set_coarse_bin": (C.c_int, [_vp, C.c_uint32]),
+    "rz_profile": (C.c_int, [_vp, C.c_int]),
+    "rz_profile_read": (C.c_int, [_vp, _f, _u32]),
+}
+
+_lib = None
+
+
+def lib_path() -> str:
+    return LIB_PATH
+
+
+def load():
+    """Load librz.so; raises if it is missing (build it with __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L

@@ -1,0 +1,287 @@
+"""Host-side mirror of rusterizer's draw API over the HIP library.
+
+`Renderer`, `Uniforms`, `UniformBlock`, `Mesh`, `Texture` keep the names,
+argument meaning and call order of src/render.rs, src/uniform.rs,
+src/mesh.rs and src/texture.rs; shaders, which are Rust fn pointers there,
+are enum values here (`VertexShader.MVP`, `FragmentShader.{TEXTURE,COLOR,DEBUG}`,
+the closures of src/main.rs:67-77 and :147-152) because the code runs on
+the GPU. `Context` is the thin wrapper of the C ABI that Renderer uses.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import weakref
+
+import numpy as np
+
+from . import _lib as L
+
+
+class VertexShader(enum.IntEnum):
+    MVP = L.RZ_VS_MVP  # main.rs:147-152
+
+
+class FragmentShader(enum.IntEnum):
+    TEXTURE = L.RZ_FS_TEXTURE  # main.rs:69-71
+    COLOR = L.RZ_FS_COLOR      # main.rs:72
+    DEBUG = L.RZ_FS_DEBUG      # main.rs:73-75
+
+
+class RzError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"rz error {code}: {msg}")
+        self.code = code
+
+
+def _f32(a, shape_last=None) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+    if shape_last is not None:
+        a = a.reshape(-1, shape_last)
+    return a
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class DeviceMesh:
+    """A mesh resident on the device (rz_mesh)."""
+
+    def __init__(self, ctx: "Context", handle, ntri: int):
+        self._ctx = ctx
+        self.handle = handle
+        self.ntri = ntri
+
+    def destroy(self):
+        if self.handle:
+            self._ctx._check(L.load().rz_mesh_destroy(self._ctx.handle, self.handle))
+            self.handle = None
+
+
+class Context:
+    """One rz_ctx: a headless renderer with its own HIP stream."""
+
+    def __init__(self, width: int, height: int):
+        self._L = L.load()
+        h = C.c_void_p()
+        rc = self._L.rz_create(width, height, C.byref(h))
+        if rc != 0:
+            raise RzError(rc, "rz_create failed")
+        self.handle = h
+        self.width, self.height = width, height
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._L.rz_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != 0:
+            msg = self._L.rz_last_error(self.handle)
+            raise RzError(rc, msg.decode() if msg else "")
+
+    # ---- state
+    @property
+    def stream(self) -> int:
+        return int(self._L.rz_stream(self.handle) or 0)
+
+    def set_clear_depth(self, depth: float):
+        self._check(self._L.rz_set_clear_depth(self.handle, float(depth)))
+
+    def set_uniforms(self, world, view, proj):
+        w, v, p = (_f32(m).reshape(16) for m in (world, view, proj))
+        fp = C.POINTER(C.c_float)
+        self._check(self._L.rz_set_uniforms(self.handle, w.ctypes.data_as(fp), v.ctypes.data_as(fp),
+                                            p.ctypes.data_as(fp)))
+
+    def bind_texture(self, index: int, rgba8: np.ndarray):
+        t = np.ascontiguousarray(rgba8, dtype=np.uint8)
+        if t.ndim != 3 or t.shape[2] != 4:
+            raise ValueError("texture must be (height, width, 4) RGBA8")
+        th, tw = t.shape[:2]
+        self._check(self._L.rz_bind_texture(self.handle, index, t.ctypes.data_as(C.POINTER(C.c_uint8)), tw, th))
+
+    def set_coarse_bin(self, pixels: int):
+        self._check(self._L.rz_set_coarse_bin(self.handle, pixels))
+
+    # ---- meshes
+    def mesh(self, pos, color=None, uv=None, idx=None) -> DeviceMesh:
+        pos = _f32(pos, 3)
+        nv = pos.shape[0]
+        color = None if color is None else _f32(color, 4)
+        uv = None if uv is None else _f32(uv, 2)
+        if idx is None:
+            idx = np.arange(nv, dtype=np.uint32)
+        idx = np.ascontiguousarray(np.asarray(idx, dtype=np.uint32).reshape(-1))
+        h = C.c_void_p()
+        self._check(self._L.rz_mesh_create(self.handle, _ptr(pos), _ptr(color), _ptr(uv), nv, _ptr(idx),
+                                           idx.size, C.byref(h)))
+        return DeviceMesh(self, h, idx.size // 3)
+
+    def mesh_device(self, pos, color, uv, idx) -> DeviceMesh:
+        """From torch device tensors (pos (nv,3) f32, color (nv,4) or None, uv (nv,2) or None, idx u32/int32)."""
+        nv = int(pos.shape[0])
+        dp = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+        h = C.c_void_p()
+        self._check(self._L.rz_mesh_create_device(self.handle, dp(pos), dp(color), dp(uv), nv, dp(idx),
+                                                  int(idx.numel()), C.byref(h)))
+        return DeviceMesh(self, h, int(idx.numel()) // 3)
+
+    # ---- frames
+    def draw(self, mesh: DeviceMesh, fs: int, vs: int = L.RZ_VS_MVP):
+        self._check(self._L.rz_draw(self.handle, mesh.handle, int(vs), int(fs)))
+
+    def resolve(self) -> np.ndarray:
+        out = np.empty((self.height, self.width), np.uint32)
+        self._check(self._L.rz_resolve(self.handle, _ptr(out)))
+        return out
+
+    def resolve_device(self, dev_ptr: int):
+        self._check(self._L.rz_resolve_device(self.handle, C.c_void_p(dev_ptr)))
+
+    def resolve_msaa(self):
+        out = np.empty((self.height, self.width), np.uint32)
+        col = np.empty((self.height, self.width, 4), np.uint32)
+        dep = np.empty((self.height, self.width, 4), np.float32)
+        self._check(self._L.rz_resolve_msaa(self.handle, _ptr(out), _ptr(col), _ptr(dep)))
+        return out, col, dep
+
+    def frame_flags(self) -> int:
+        return int(self._L.rz_frame_flags(self.handle))
+
+    def frame_stats(self) -> dict:
+        a = (C.c_uint64 * 5)()
+        self._check(self._L.rz_frame_stats(self.handle, a))
+        return dict(zip(["tris", "clipped", "prims", "bin_entries", "replays"], (int(x) for x in a)))
+
+    def profile(self, enable: bool):
+        self._check(self._L.rz_profile(self.handle, 1 if enable else 0))
+
+    def profile_read(self):
+        ms = (C.c_float * L.RZ_NSTAGES)()
+        n = C.c_uint32()
+        self._check(self._L.rz_profile_read(self.handle, ms, C.byref(n)))
+        return dict(geometry=ms[0], binning=ms[1], raster=ms[2], frames=n.value)
+
+
+# ---------------------------------------------------------------- mirror API
+class Texture:
+    """src/texture.rs: RGBA8, (0, 0) is the upper-left texel."""
+
+    def __init__(self, rgba8: np.ndarray):
+        self.buf = np.ascontiguousarray(rgba8, dtype=np.uint8)
+        if self.buf.ndim != 3 or self.buf.shape[2] != 4:
+            raise ValueError("texture must be (height, width, 4) RGBA8")
+        self.height, self.width = self.buf.shape[:2]
+
+    @staticmethod
+    def from_rgba8(rgba8) -> "Texture":
+        return Texture(rgba8)
+
+
+class Mesh:
+    """src/mesh.rs:5-12. vertices (n,3) f32, indices (3k,) int, attributes
+    as colors (n,4) f32 and uvs (n,2) f32 (VertexAttribute, graphics_primitives.rs:9-13).
+    Treat as immutable once rendered, or call invalidate()."""
+
+    def __init__(self, vertices, indices, colors=None, uvs=None):
+        self.vertices = _f32(vertices, 3)
+        n = self.vertices.shape[0]
+        self.indices = np.ascontiguousarray(np.asarray(indices, dtype=np.uint32).reshape(-1))
+        self.colors = _f32(colors if colors is not None else np.zeros((n, 4)), 4)
+        self.uvs = _f32(uvs if uvs is not None else np.zeros((n, 2)), 2)
+        self.version = 0
+
+    def invalidate(self):
+        self.version += 1
+
+
+class UniformBlock:
+    """src/uniform.rs:5-9 (row-major 4x4 f32 matrices)."""
+
+    def __init__(self):
+        self.world = np.eye(4, dtype=np.float32)
+        self.view = np.eye(4, dtype=np.float32)
+        self.projection = np.eye(4, dtype=np.float32)
+
+
+class Uniforms:
+    """src/uniform.rs:12-46."""
+
+    def __init__(self, ctx: Context):
+        self._ctx = ctx
+        self.textures: list[Texture] = []
+        self.uniform_block = UniformBlock()
+
+    def bind_texture(self, index: int, tex: Texture):
+        # uniform.rs:29-33 asserts index == textures.len()
+        if index != len(self.textures):
+            raise ValueError("texture index must equal the number of bound textures")
+        self._ctx.bind_texture(index, tex.buf)
+        self.textures.append(tex)
+
+    def get_texture(self, index: int) -> Texture:
+        return self.textures[index]
+
+    def read_block(self) -> UniformBlock:
+        return self.uniform_block
+
+    def write_block(self) -> UniformBlock:
+        return self.uniform_block
+
+
+class Renderer:
+    """src/render.rs:38-145 without the window: `framebuffer()` returns the
+    resolved ARGB image that `display()` would hand to minifb."""
+
+    def __init__(self, width: int, height: int):
+        self.width, self.height = width, height
+        self._ctx = Context(width, height)
+        self._uniforms = Uniforms(self._ctx)
+        self._meshes: dict[int, tuple] = {}
+
+    @property
+    def context(self) -> Context:
+        return self._ctx
+
+    def uniforms(self) -> Uniforms:
+        return self._uniforms
+
+    def _device_mesh(self, mesh: Mesh) -> DeviceMesh:
+        key = id(mesh)
+        hit = self._meshes.get(key)
+        if hit is not None:
+            ref, ver, dm = hit
+            if ref() is mesh and ver == mesh.version:
+                return dm
+        dm = self._ctx.mesh(mesh.vertices, mesh.colors, mesh.uvs, mesh.indices)
+        self._meshes[key] = (weakref.ref(mesh), mesh.version, dm)
+        return dm
+
+    def render(self, mesh: Mesh, vertex_shader: VertexShader = VertexShader.MVP,
+               fragment_shader: FragmentShader = FragmentShader.TEXTURE):
+        """render.rs:98-114: shade vertices with the current uniforms, assemble
+        and rasterize; accumulates into the frame until framebuffer()."""
+        b = self._uniforms.uniform_block
+        self._ctx.set_uniforms(b.world, b.view, b.projection)
+        self._ctx.draw(self._device_mesh(mesh), int(fragment_shader), int(vertex_shader))
+
+    def framebuffer(self) -> np.ndarray:
+        """Rasterizer::framebuffer (mod.rs:520-522): resolve + clear; (H, W) u32 ARGB."""
+        return self._ctx.resolve()
+
+    def close(self):
+        for _, _, dm in self._meshes.values():
+            try:
+                dm.destroy()
+            except RzError:
+                pass
+        self._meshes.clear()
+        self._ctx.close()

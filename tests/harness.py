@@ -1,0 +1,93 @@
+"""Shared helpers: run a Scene through the HIP library and through the oracle,
+and compare the results at the reference's bar (coverage and depth bit-exact,
+resolved colour within 1 LSB per channel)."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+CLEAR_COLOR = 0xFF191919
+CPU_THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+def run_gpu(scenes, coarse_bin: int | None = None, ctx=None, msaa: bool = True):
+    """Render one frame made of the given scenes' draws (same size) on the GPU."""
+    from rusterizer_amd.render import Context
+
+    if not isinstance(scenes, (list, tuple)):
+        scenes = [scenes]
+    s0 = scenes[0]
+    own = ctx is None
+    if own:
+        ctx = Context(s0.width, s0.height)
+    try:
+        ctx.set_clear_depth(s0.clear_depth)
+        if coarse_bin is not None:
+            ctx.set_coarse_bin(coarse_bin)
+        tex_bound = False
+        meshes = []
+        for sc in scenes:
+            if sc.texture is not None and not tex_bound:
+                ctx.bind_texture(0, sc.texture)
+                tex_bound = True
+            ctx.set_uniforms(sc.world, sc.view, sc.proj)
+            m = ctx.mesh(sc.pos, sc.color, sc.uv, sc.idx)
+            meshes.append(m)
+            ctx.draw(m, sc.fs)
+        if msaa:
+            img, col, dep = ctx.resolve_msaa()
+        else:
+            img, col, dep = ctx.resolve(), None, None
+        res = dict(img=img, col=col, dep=dep, flags=ctx.frame_flags(), stats=ctx.frame_stats())
+        for m in meshes:
+            m.destroy()
+        return res
+    finally:
+        if own:
+            ctx.close()
+
+
+def run_oracle(scenes, threads: int = CPU_THREADS):
+    from oracle.oracle import OracleRenderer
+
+    if not isinstance(scenes, (list, tuple)):
+        scenes = [scenes]
+    s0 = scenes[0]
+    r = OracleRenderer(s0.width, s0.height, clear_depth=s0.clear_depth)
+    tex_bound = False
+    for sc in scenes:
+        if sc.texture is not None and not tex_bound:
+            r.bind_texture(0, sc.texture)
+            tex_bound = True
+        r.set_uniforms(sc.world, sc.view, sc.proj)
+        r.render(sc.pos, sc.color, sc.uv, sc.idx, sc.fs, threads=threads)
+    col, dep = r.read_msaa()
+    img = r.framebuffer()
+    return dict(img=img, col=col, dep=dep, flags=r.flags, stats=r.stats())
+
+
+def channel_diff(a: np.ndarray, b: np.ndarray) -> int:
+    """Largest per-channel difference between two ARGB u32 arrays."""
+    a = a.astype(np.uint32).view(np.uint8).astype(np.int16)
+    b = b.astype(np.uint32).view(np.uint8).astype(np.int16)
+    return int(np.abs(a - b).max()) if a.size else 0
+
+
+def assert_parity(gpu: dict, cpu: dict, where: str = ""):
+    # depth: bit-exact per sample (this also pins coverage: every covered,
+    # depth-passing sample carries its interpolated depth)
+    dg = gpu["dep"].view(np.uint32)
+    dc = cpu["dep"].view(np.uint32)
+    bad = np.argwhere(dg != dc)
+    assert bad.size == 0, f"{where}: {len(bad)} depth samples differ, first {bad[:5].tolist()}"
+    # coverage: which samples hold a written colour
+    wg = gpu["col"] != CLEAR_COLOR
+    wc = cpu["col"] != CLEAR_COLOR
+    assert np.array_equal(wg, wc), f"{where}: coverage differs at {np.argwhere(wg != wc)[:5].tolist()}"
+    # colour: within 1 LSB per channel (MSAA samples and the resolved image)
+    d_msaa = channel_diff(gpu["col"], cpu["col"])
+    d_img = channel_diff(gpu["img"], cpu["img"])
+    assert d_msaa <= 1 and d_img <= 1, f"{where}: colour differs by {d_msaa} / {d_img} LSB"
+    assert gpu["flags"] == cpu["flags"] & 5, f"{where}: flags gpu={gpu['flags']} cpu={cpu['flags']}"
+    return dict(exact_colour=bool(np.array_equal(gpu["col"], cpu["col"]) and np.array_equal(gpu["img"], cpu["img"])))
