@@ -79,11 +79,10 @@ struct rz_ctx {
     // rasterizer triangles
     DBuf<PrimGeom> geom;
     DBuf<float> attrs;  // ATTR_STRIDE per prim
-    DBuf<uint2> pbbox;
+    DBuf<uint4> pbin;
     // coarse bins
     DBuf<uint32_t> bin_counts, bin_offsets;
-    DBuf<uint32_t> list_prim;
-    DBuf<uint2> list_bbox;
+    DBuf<uint4> list;
     // outputs
     DBuf<uint32_t> out;
     DBuf<uint32_t> msaa_color;
@@ -171,12 +170,11 @@ int ensure_frame_buffers(rz_ctx* c, uint64_t prims_need, uint64_t clip_need, uin
     bool g = false;
     if ((rc = grow(c, c->geom, prims_need, &g))) return rc;
     if ((rc = grow(c, c->attrs, c->geom.cap * ATTR_STRIDE))) return rc;
-    if ((rc = grow(c, c->pbbox, c->geom.cap))) return rc;
+    if ((rc = grow(c, c->pbin, c->geom.cap))) return rc;
     if ((rc = grow(c, c->clip_tri, clip_need, &g))) return rc;
     if ((rc = grow(c, c->clip_n, c->clip_tri.cap))) return rc;
     if ((rc = grow(c, c->clip_out, c->clip_tri.cap * MAXV * 10))) return rc;
-    if ((rc = grow(c, c->list_prim, pairs_need))) return rc;
-    if ((rc = grow(c, c->list_bbox, c->list_prim.cap))) return rc;
+    if ((rc = grow(c, c->list, pairs_need))) return rc;
     size_t bm = bin_matrix_len(c);
     if ((rc = grow(c, c->bin_counts, bm))) return rc;
     if ((rc = grow(c, c->bin_offsets, bm))) return rc;
@@ -196,7 +194,7 @@ void enqueue_draw_geometry(rz_ctx* c, uint32_t d) {
                 c->clip_n.p);
     launch_scan(s, c->nsub.p, c->suboff.p, h.ntri, nullptr, 0, c->parts.p, &c->d_cnt->nprims, nullptr);
     launch_setup(s, c->d_draws, d, h.ntri, c->d_cnt, c->cls.p, c->nsub.p, c->suboff.p, c->clip_out.p,
-                 (uint32_t)c->clip_tri.cap, c->geom.p, c->attrs.p, c->pbbox.p, (uint32_t)c->geom.cap, c->W, c->H,
+                 (uint32_t)c->clip_tri.cap, c->geom.p, c->attrs.p, c->pbin.p, (uint32_t)c->geom.cap, c->W, c->H,
                  c->bg);
 }
 
@@ -259,14 +257,14 @@ int resolve_impl(rz_ctx* c, uint32_t* host_out, uint32_t* dev_out, uint32_t* hos
         const uint32_t nb = c->bg.nbx * c->bg.nby;
         const uint32_t nc_max = (uint32_t)((c->geom.cap + CHUNK - 1) / CHUNK);
         const uint32_t bin_grid = std::min<uint32_t>((nc_max + 3) / 4, 4096);
-        launch_bin(s, false, bin_grid, c->d_cnt, c->pbbox.p, c->bg, c->bin_counts.p, nullptr, nullptr, nullptr, 0);
+        launch_bin(s, false, bin_grid, c->d_cnt, c->pbin.p, c->bg, c->bin_counts.p, nullptr, nullptr, 0);
         launch_scan(s, c->bin_counts.p, c->bin_offsets.p, nb * nc_max, &c->d_cnt->nprims, nb, c->parts.p, nullptr,
                     &c->d_cnt->pairs_binned);
-        launch_bin(s, true, bin_grid, c->d_cnt, c->pbbox.p, c->bg, nullptr, c->bin_offsets.p, c->list_prim.p,
-                   c->list_bbox.p, (uint32_t)c->list_prim.cap);
+        launch_bin(s, true, bin_grid, c->d_cnt, c->pbin.p, c->bg, nullptr, c->bin_offsets.p, c->list.p,
+                   (uint32_t)c->list.cap);
         if (c->prof) HIP_TRY(c, hipEventRecord(c->ev[2], s));
-        launch_raster(s, c->d_cnt, c->geom.p, c->attrs.p, c->d_draws, c->bin_offsets.p, c->list_prim.p,
-                      c->list_bbox.p, c->bg, c->W, c->H, c->clear_depth, target,
+        launch_raster(s, c->d_cnt, c->geom.p, c->attrs.p, c->d_draws, c->bin_offsets.p, c->list.p, c->bg, c->W,
+                      c->H, c->clear_depth, target,
                       want_msaa ? c->msaa_color.p : nullptr, want_msaa ? c->msaa_depth.p : nullptr);
         if (c->prof) HIP_TRY(c, hipEventRecord(c->ev[3], s));
         HIP_TRY(c, hipGetLastError());
@@ -284,10 +282,10 @@ int resolve_impl(rz_ctx* c, uint32_t* host_out, uint32_t* dev_out, uint32_t* hos
         }
         uint64_t prims_need = std::max<uint64_t>(fc.nprims, c->geom.cap);
         uint64_t clip_need = std::max<uint64_t>(fc.clip_need, c->clip_tri.cap);
-        uint64_t pairs_need = std::max<uint64_t>(fc.npairs, c->list_prim.cap);
+        uint64_t pairs_need = std::max<uint64_t>(fc.npairs, c->list.cap);
         if (fc.overflow & OVF_PRIMS) prims_need = std::max<uint64_t>(fc.nprims, c->geom.cap * 2);
         if (fc.overflow & OVF_CLIP) clip_need = std::max<uint64_t>(fc.clip_need, c->clip_tri.cap * 2);
-        if (fc.overflow & OVF_PAIRS) pairs_need = std::max<uint64_t>(fc.npairs, c->list_prim.cap * 2);
+        if (fc.overflow & OVF_PAIRS) pairs_need = std::max<uint64_t>(fc.npairs, c->list.cap * 2);
         if ((rc = ensure_frame_buffers(c, prims_need, clip_need, pairs_need, std::max(max_draw_tris(c), 1024u),
                                        nullptr))) {
             end_frame(c);
@@ -439,11 +437,10 @@ int rz_destroy(rz_ctx* c) {
     release(c->clip_out);
     release(c->geom);
     release(c->attrs);
-    release(c->pbbox);
+    release(c->pbin);
     release(c->bin_counts);
     release(c->bin_offsets);
-    release(c->list_prim);
-    release(c->list_bbox);
+    release(c->list);
     release(c->out);
     release(c->msaa_color);
     release(c->msaa_depth);
@@ -544,7 +541,7 @@ int rz_draw(rz_ctx* c, const rz_mesh* mesh_c, int vs, int fs) {
     uint64_t tris = c->frame_tris + mesh->ntri;
     uint64_t prims_est = tris + tris / 4 + 4096;
     uint64_t clip_est = std::max<uint64_t>(c->clip_tri.cap, mesh->ntri / 8 + 1024);
-    uint64_t pairs_est = std::max<uint64_t>(c->list_prim.cap, prims_est * 2);
+    uint64_t pairs_est = std::max<uint64_t>(c->list.cap, prims_est * 2);
     bool grew_prims = false;
     if ((rc = ensure_frame_buffers(c, std::max<uint64_t>(prims_est, c->geom.cap), clip_est, pairs_est,
                                    std::max(mesh->ntri, max_draw_tris(c)), &grew_prims)))

@@ -56,9 +56,13 @@ struct PrimGeom {
     float w0, w1, w2;                    // clip-space w
     uint32_t draw;
     uint32_t bbx, bby;                   // clamped pixel bbox: lo | hi << 16
-    uint32_t pad0, pad1;
+    uint32_t flags;                      // tie-rule bit per edge (0..2) | fs << 8
+    float zlo;                           // lower bound of every interpolated depth
 };
 static_assert(sizeof(PrimGeom) == 96, "PrimGeom layout");
+
+// Per-prim binning record and coarse-bin list entry: {prim, bbx, bby, zlo bits}.
+// (uint4; for the per-prim record .x is unused.)
 
 struct BinGeom {
     uint32_t cb;        // coarse bin side in pixels (multiple of 16)
@@ -84,14 +88,12 @@ void launch_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uint32_t n_ho
                  uint32_t bins, uint32_t* parts, uint32_t* base_total, uint32_t* total2);
 void launch_setup(hipStream_t s, const DrawDev* draws, uint32_t d, uint32_t ntri, FrameCounters* cnt,
                   const uint32_t* cls, const uint32_t* nsub, const uint32_t* sub_off, const float* clip_out,
-                  uint32_t clip_cap, PrimGeom* geom, float* attrs, uint2* pbbox, uint32_t prim_cap, uint32_t W,
+                  uint32_t clip_cap, PrimGeom* geom, float* attrs, uint4* pbin, uint32_t prim_cap, uint32_t W,
                   uint32_t H, BinGeom bg);
-void launch_bin(hipStream_t s, bool scatter, uint32_t grid, FrameCounters* cnt, const uint2* pbbox, BinGeom bg,
-                uint32_t* counts, const uint32_t* offsets, uint32_t* list_prim, uint2* list_bbox,
-                uint32_t pair_cap);
+void launch_bin(hipStream_t s, bool scatter, uint32_t grid, FrameCounters* cnt, const uint4* pbin, BinGeom bg,
+                uint32_t* counts, const uint32_t* offsets, uint4* list, uint32_t pair_cap);
 void launch_raster(hipStream_t s, FrameCounters* cnt, const PrimGeom* geom, const float* attrs,
-                   const DrawDev* draws, const uint32_t* offsets, const uint32_t* list_prim,
-                   const uint2* list_bbox, BinGeom bg, uint32_t W, uint32_t H, float clear_depth, uint32_t* out,
-                   uint32_t* msaa_color, float* msaa_depth);
+                   const DrawDev* draws, const uint32_t* offsets, const uint4* list, BinGeom bg, uint32_t W,
+                   uint32_t H, float clear_depth, uint32_t* out, uint32_t* msaa_color, float* msaa_depth);
 
 }  // namespace rzk

@@ -163,20 +163,23 @@ __global__ __launch_bounds__(256) void k_classify(const DrawDev* __restrict__ dr
             c = out_any ? 0u : (in_all ? 1u : 2u);
         }
     }
-    bool need = live && c == 2;
-    uint64_t m = __ballot(need);
-    uint32_t slot = 0;
-    if (m) {
-        uint32_t leader = __ffsll((unsigned long long)m) - 1;
-        uint32_t base = 0;
-        if (lane_id() == leader) {
-            base = atomicAdd(&cnt->nclip, (uint32_t)__popcll(m));
-            atomicMax(&cnt->clip_need, base + (uint32_t)__popcll(m));
-            atomicAdd(&cnt->nclipped, (uint32_t)__popcll(m));
+    // one clip-list reservation per workgroup: wave counts -> LDS -> one atomic
+    __shared__ uint32_t wcount[4], wbase[4];
+    const bool need = live && c == 2;
+    const uint64_t m = __ballot(need);
+    const uint32_t wave = threadIdx.x >> 6;
+    if (lane_id() == 0) wcount[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        uint32_t b = tot ? atomicAdd(&cnt->nclip, tot) : 0u;
+        for (int w = 0; w < 4; ++w) {
+            wbase[w] = b;
+            b += wcount[w];
         }
-        base = __shfl(base, leader);
-        slot = base + mbcnt(m);
     }
+    __syncthreads();
+    const uint32_t slot = wbase[wave] + mbcnt(m);
     if (live) {
         if (need) {
             cls[t] = 2u + slot;
@@ -214,6 +217,11 @@ __global__ __launch_bounds__(64) void k_clip(const DrawDev* __restrict__ draws, 
     const DrawDev& dd = draws[d];
     const uint32_t lane = threadIdx.x;
     uint32_t nslots = min(uniform(cnt->nclip), clip_cap);
+    if (blockIdx.x == 0 && lane == 0) {  // per-draw bookkeeping of the clip list
+        uint32_t used = cnt->nclip;
+        cnt->clip_need = max(cnt->clip_need, used);
+        cnt->nclipped += used;
+    }
     if (uniform(cnt->overflow) != 0) return;
     for (uint32_t s = blockIdx.x; s < nslots; s += gridDim.x) {
         uint32_t t = clip_tri[s];
@@ -413,10 +421,17 @@ __global__ __launch_bounds__(256) void k_scan_down(const uint32_t* __restrict__ 
 // at sub_off[t] + k in fan order.
 // mod.rs:284-313 (divide), mod.rs:315-345 (viewport), mod.rs:187-222 (edge
 // setup), bounding_box.rs:13-42 + mod.rs:347-361 (clamped pixel bbox).
+// Besides the reference's values each prim carries derived, exactly
+// computed helpers for the raster: its tie-rule bits and a rigorous lower
+// bound on any depth it can interpolate.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ bool tie_rule(float nx, float ny) {
+    return (nx > 0.0f) || (!(nx < 0.0f) && (ny < 0.0f));  // mod.rs:159-168
+}
+
 __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t H, uint32_t draw,
                                            uint32_t fs, PrimGeom* __restrict__ g, float* __restrict__ attr,
-                                           uint2* __restrict__ pbb, BinGeom bg, uint32_t& pairs) {
+                                           uint4* __restrict__ pbin, BinGeom bg, uint32_t& pairs) {
     const float zmin = 0.0f, zmax = 1.0f;
     float sx[3], sy[3], sz[3], w[3];
     for (int k = 0; k < 3; ++k) {
@@ -454,10 +469,15 @@ __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t
     uint32_t y0 = min(sat_u32(floorf(a2)), H), y1 = min(sat_u32(ceilf(a3)), H);
     o.bbx = x0 | (x1 << 16);
     o.bby = y0 | (y1 << 16);
-    o.pad0 = 0;
-    o.pad1 = 0;
+    o.flags = (tie_rule(o.nx0, o.ny0) ? 1u : 0u) | (tie_rule(o.nx1, o.ny1) ? 2u : 0u) |
+              (tie_rule(o.nx2, o.ny2) ? 4u : 0u) | (fs << 8);
+    // Depth of a covered sample is ((b0*z0 + b1*z1) + b2*z2) with clamped
+    // b_i in [0, 1] and 1 - 2^-23 <= b0+b1+b2 <= 2; three roundings later it
+    // is >= zlo below, so a sample can pass LESS only where zbuf > zlo.
+    float zm = fminf(fminf(sz[0], sz[1]), sz[2]);
+    o.zlo = zm >= 0.0f ? zm * (1.0f - 0x1p-18f) - 0x1p-126f : zm * (2.0f + 0x1p-17f) - 0x1p-126f;
     *g = o;
-    *pbb = make_uint2(o.bbx, o.bby);
+    *pbin = make_uint4(0u, o.bbx, o.bby, __float_as_uint(o.zlo));
     if (x0 < x1 && y0 < y1) {
         uint32_t cx0 = x0 / bg.cb, cx1 = (x1 - 1) / bg.cb, cy0 = y0 / bg.cb, cy1 = (y1 - 1) / bg.cb;
         pairs += (cx1 - cx0 + 1) * (cy1 - cy0 + 1);
@@ -479,7 +499,7 @@ __global__ __launch_bounds__(256) void k_setup(const DrawDev* __restrict__ draws
                                                const uint32_t* __restrict__ sub_off,
                                                const float* __restrict__ clip_out, uint32_t clip_cap,
                                                PrimGeom* __restrict__ geom, float* __restrict__ attrs,
-                                               uint2* __restrict__ pbbox, uint32_t prim_cap,
+                                               uint4* __restrict__ pbin, uint32_t prim_cap,
                                                uint32_t W, uint32_t H, BinGeom bg) {
     const DrawDev& dd = draws[d];
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -498,7 +518,7 @@ __global__ __launch_bounds__(256) void k_setup(const DrawDev* __restrict__ draws
                 tv[k].x = p.x; tv[k].y = p.y; tv[k].z = p.z; tv[k].w = p.w;
             }
             if (base < prim_cap)
-                setup_prim(tv, W, H, d, fs, geom + base, attrs + (size_t)base * ATTR_STRIDE, pbbox + base, bg, pairs);
+                setup_prim(tv, W, H, d, fs, geom + base, attrs + (size_t)base * ATTR_STRIDE, pbin + base, bg, pairs);
             else
                 ovf = true;
         } else if (c >= 2) {
@@ -517,7 +537,7 @@ __global__ __launch_bounds__(256) void k_setup(const DrawDev* __restrict__ draws
                     }
                     if (base + k < prim_cap)
                         setup_prim(tv, W, H, d, fs, geom + base + k, attrs + (size_t)(base + k) * ATTR_STRIDE,
-                                   pbbox + base + k, bg, pairs);
+                                   pbin + base + k, bg, pairs);
                     else
                         ovf = true;
                 }
@@ -534,17 +554,17 @@ __global__ __launch_bounds__(256) void k_setup(const DrawDev* __restrict__ draws
 // ---------------------------------------------------------------------------
 // k_bin: ordered coarse binning, one wave per chunk of CHUNK prims.
 // COUNT:   counts[b * NC + chunk] = prims of the chunk overlapping bin b.
-// SCATTER: writes (prim, bbox) to list position offsets[b * NC + chunk] + rank,
-//          rank = number of earlier prims of the chunk in bin b.
+// SCATTER: writes {prim, bbox, zlo} to list position offsets[b * NC + chunk]
+//          + rank, rank = number of earlier prims of the chunk in bin b.
 // Both loop bins in the same order and rank lanes with ballots, so each bin's
 // list holds its prims in submission order.
 // ---------------------------------------------------------------------------
 template <bool SCATTER>
 __global__ __launch_bounds__(256) void k_bin(const FrameCounters* __restrict__ cnt,
-                                             const uint2* __restrict__ pbbox, BinGeom bg,
+                                             const uint4* __restrict__ pbin, BinGeom bg,
                                              uint32_t* __restrict__ counts, const uint32_t* __restrict__ offsets,
-                                             uint32_t* __restrict__ list_prim, uint2* __restrict__ list_bbox,
-                                             uint32_t pair_cap, FrameCounters* __restrict__ cnt_w) {
+                                             uint4* __restrict__ list, uint32_t pair_cap,
+                                             FrameCounters* __restrict__ cnt_w) {
     if (uniform(cnt->overflow) != 0) return;
     const uint32_t S = uniform(cnt->nprims);
     const uint32_t NC = (S + CHUNK - 1) / CHUNK;
@@ -564,9 +584,9 @@ __global__ __launch_bounds__(256) void k_bin(const FrameCounters* __restrict__ c
         }
         for (uint32_t sub = 0; sub < CHUNK / 64; ++sub) {
             uint32_t p = chunk * CHUNK + sub * 64 + lane;
-            uint2 bb = make_uint2(0u, 0u);
-            if (p < S) bb = pbbox[p];
-            uint32_t x0 = bb.x & 0xFFFFu, x1 = bb.x >> 16, y0 = bb.y & 0xFFFFu, y1 = bb.y >> 16;
+            uint4 pb = make_uint4(0u, 0u, 0u, 0u);
+            if (p < S) pb = pbin[p];
+            uint32_t x0 = pb.y & 0xFFFFu, x1 = pb.y >> 16, y0 = pb.z & 0xFFFFu, y1 = pb.z >> 16;
             bool ne = (x0 < x1) && (y0 < y1);
             uint32_t cx0 = x0 / bg.cb, cx1 = ne ? (x1 - 1) / bg.cb : 0u;
             uint32_t cy0 = y0 / bg.cb, cy1 = ne ? (y1 - 1) / bg.cb : 0u;
@@ -585,12 +605,10 @@ __global__ __launch_bounds__(256) void k_bin(const FrameCounters* __restrict__ c
                         uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)rv, (int)bl);
                         if (hit) {
                             uint32_t pos = r0 + mbcnt(m);
-                            if (pos < pair_cap) {
-                                list_prim[pos] = p;
-                                list_bbox[pos] = bb;
-                            } else {
+                            if (pos < pair_cap)
+                                list[pos] = make_uint4(p, pb.y, pb.z, pb.w);
+                            else
                                 ovf = true;
-                            }
                         }
                     }
                     if (lane == bl) {
@@ -616,14 +634,20 @@ __global__ __launch_bounds__(256) void k_bin(const FrameCounters* __restrict__ c
 // k_raster: one workgroup per 16x16 screen tile, one wave per 8x8 quadrant,
 // one lane per pixel. The pixel's 4 MSAA colours and depths live in the
 // lane's registers for the whole frame; the wave walks its coarse bin's list
-// in submission order (ballot filter against the quadrant), and applies
-// mod.rs:443-473 to each overlapping rasterizer triangle, then resolves
-// (buffers.rs:111-125) and writes the ARGB word once.
+// in submission order and applies mod.rs:443-473 to each rasterizer
+// triangle that can touch the quadrant, then resolves (buffers.rs:111-125)
+// and writes the ARGB word once.
+//
+// Work that provably cannot change a sample is skipped (exact culls):
+//  - list entries whose bbox misses the quadrant;
+//  - hierarchical depth: the quadrant keeps qzmax = max of its stored
+//    depths; a prim whose depth lower bound zlo is >= qzmax fails the strict
+//    LESS test on every sample of the quadrant;
+//  - edge reject: if an edge function is negative, with a margin that covers
+//    the rounding of the reference's evaluation, at the quadrant corner where
+//    it is largest, no sample of the quadrant is inside that edge.
 // ---------------------------------------------------------------------------
-__constant__ float RGSS_X[4] = {5.0f / 8.0f, 7.0f / 8.0f, 3.0f / 8.0f, 1.0f / 8.0f};  // mod.rs:108-114
-__constant__ float RGSS_Y[4] = {1.0f / 8.0f, 5.0f / 8.0f, 7.0f / 8.0f, 3.0f / 8.0f};
-
-__device__ __forceinline__ float sample_x(int s) {
+__device__ __forceinline__ float sample_x(int s) {  // mod.rs:108-114
     return s == 0 ? 0.625f : (s == 1 ? 0.875f : (s == 2 ? 0.375f : 0.125f));
 }
 __device__ __forceinline__ float sample_y(int s) {
@@ -689,19 +713,28 @@ __device__ __forceinline__ float edge(float nx, float ny, float ax, float ay, fl
     return (0.0f + nx * (x - ax)) + ny * (y - ay);  // vector.rs:17-23 on (n, p - a)
 }
 
-// mod.rs:148-170 for one edge: e > 0, or e not < 0 and the tie rule holds
-__device__ __forceinline__ bool edge_in(float e, bool tie) { return (e > 0.0f) || (!(e < 0.0f) && tie); }
+// false only if every sample position in [xl, xh] x [yl, yh] evaluates the
+// edge (reference formula and rounding) strictly below zero
+__device__ __forceinline__ bool edge_may_cover(float nx, float ny, float ax, float ay, float xl, float xh,
+                                               float yl, float yh) {
+    float xs = nx > 0.0f ? xh : xl;
+    float ys = ny > 0.0f ? yh : yl;
+    float dx = xs - ax, dy = ys - ay;
+    float e = nx * dx + ny * dy;
+    float mag = fabsf(nx) * (fabsf(dx) + 16.0f) + fabsf(ny) * (fabsf(dy) + 16.0f);
+    return !(e < -(mag * 0x1p-16f + 0x1p-126f));
+}
 
-__device__ __forceinline__ bool tie_rule(float nx, float ny) {
-    return (nx > 0.0f) || (!(nx < 0.0f) && (ny < 0.0f));
+__device__ __forceinline__ float wave_max(float v) {
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
 }
 
 __global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict__ cnt,
                                                 const PrimGeom* __restrict__ geom, const float* __restrict__ attrs,
                                                 const DrawDev* __restrict__ draws,
                                                 const uint32_t* __restrict__ offsets,
-                                                const uint32_t* __restrict__ list_prim,
-                                                const uint2* __restrict__ list_bbox, BinGeom bg,
+                                                const uint4* __restrict__ list, BinGeom bg,
                                                 uint32_t W, uint32_t H, float clear_depth,
                                                 uint32_t* __restrict__ out, uint32_t* __restrict__ msaa_color,
                                                 float* __restrict__ msaa_depth, FrameCounters* __restrict__ cnt_w) {
@@ -721,10 +754,16 @@ __global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict_
     const uint32_t qx0 = tx * 16 + (wave & 1) * 8, qy0 = ty * 16 + (wave >> 1) * 8;
     const uint32_t px = qx0 + (lane & 7), py = qy0 + (lane >> 3);
     const float fx = (float)px, fy = (float)py;
+    const bool onscreen = px < W && py < H;
+    const float qxl = (float)qx0 + 0.125f, qxh = (float)qx0 + 7.875f;
+    const float qyl = (float)qy0 + 0.125f, qyh = (float)qy0 + 7.875f;
 
     float zb0 = clear_depth, zb1 = clear_depth, zb2 = clear_depth, zb3 = clear_depth;
     uint32_t cb0 = CLEAR_COLOR, cb1 = CLEAR_COLOR, cb2 = CLEAR_COLOR, cb3 = CLEAR_COLOR;
     bool oob = false;
+    // qzmax: largest depth stored in the quadrant's on-screen samples
+    float qzmax = __int_as_float(
+        __builtin_amdgcn_readfirstlane(__float_as_int(wave_max(onscreen ? clear_depth : -FLT_MAX_F))));
 
     const uint32_t S = uniform(cnt->nprims);
     const uint32_t NC = (S + CHUNK - 1) / CHUNK;
@@ -737,100 +776,123 @@ __global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict_
     }
 
     for (uint32_t base = lbeg; base < lend; base += 64) {
-        uint32_t i = base + lane;
+        const uint32_t i = base + lane;
         uint32_t myprim = 0;
-        bool hit = false;
+        bool cand = false;
         if (i < lend) {
-            uint2 bb = list_bbox[i];
-            myprim = list_prim[i];
-            uint32_t x0 = bb.x & 0xFFFFu, x1 = bb.x >> 16, y0 = bb.y & 0xFFFFu, y1 = bb.y >> 16;
-            hit = (x0 < qx0 + 8) && (x1 > qx0) && (y0 < qy0 + 8) && (y1 > qy0);
+            const uint4 en = list[i];
+            myprim = en.x;
+            const uint32_t x0 = en.y & 0xFFFFu, x1 = en.y >> 16, y0 = en.z & 0xFFFFu, y1 = en.z >> 16;
+            cand = (x0 < qx0 + 8) & (x1 > qx0) & (y0 < qy0 + 8) & (y1 > qy0) & (__uint_as_float(en.w) < qzmax);
         }
-        uint64_t m = __ballot(hit);
-        while (m) {
-            const uint32_t j = (uint32_t)(__ffsll((unsigned long long)m) - 1);
+        if (cand) {
+            const float4* gp = reinterpret_cast<const float4*>(geom + myprim);
+            const float4 g0 = gp[0], g1 = gp[1], g2 = gp[2];  // a0 a1 | a2 n0 | n1 n2
+            cand = edge_may_cover(g1.z, g1.w, g0.x, g0.y, qxl, qxh, qyl, qyh) &&
+                   edge_may_cover(g2.x, g2.y, g0.z, g0.w, qxl, qxh, qyl, qyh) &&
+                   edge_may_cover(g2.z, g2.w, g1.x, g1.y, qxl, qxh, qyl, qyh);
+        }
+        uint64_t m = __ballot(cand);
+        if (m == 0) continue;
+        uint32_t nprim = (uint32_t)__builtin_amdgcn_readlane((int)myprim, (int)(__ffsll((unsigned long long)m) - 1));
+        PrimGeom nxt = geom[nprim];
+        for (;;) {
             m &= m - 1;
-            const uint32_t prim = (uint32_t)__builtin_amdgcn_readlane((int)myprim, (int)j);
-            const PrimGeom& g = geom[prim];
-            const uint32_t bx0 = g.bbx & 0xFFFFu, bx1 = g.bbx >> 16, by0 = g.bby & 0xFFFFu, by1 = g.bby >> 16;
-            if (!(px >= bx0 && px < bx1 && py >= by0 && py < by1)) continue;
-            const bool t0 = tie_rule(g.nx0, g.ny0), t1 = tie_rule(g.nx1, g.ny1), t2 = tie_rule(g.nx2, g.ny2);
-            // mod.rs:134-146: coverage of the 4 samples
-            float e1s[4], e2s[4];
-            uint32_t cov = 0;
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                float sx = fx + sample_x(s), sy = fy + sample_y(s);
-                float e0 = edge(g.nx0, g.ny0, g.ax0, g.ay0, sx, sy);
-                float e1 = edge(g.nx1, g.ny1, g.ax1, g.ay1, sx, sy);
-                float e2 = edge(g.nx2, g.ny2, g.ax2, g.ay2, sx, sy);
-                bool in = edge_in(e0, t0) && edge_in(e1, t1) && edge_in(e2, t2);
-                cov |= (in ? 1u : 0u) << s;
-                e1s[s] = e1;
-                e2s[s] = e2;
+            const PrimGeom g = nxt;
+            const uint32_t prim = nprim;
+            if (m) {  // start loading the next hit's record now
+                nprim = (uint32_t)__builtin_amdgcn_readlane((int)myprim, (int)(__ffsll((unsigned long long)m) - 1));
+                nxt = geom[nprim];
             }
-            if (cov == 0) continue;
-            // mod.rs:225-253: linear depth at covered samples; mod.rs:363-378: strict LESS
-            float sd[4];
-            uint32_t pass = 0;
-            const float zbuf[4] = {zb0, zb1, zb2, zb3};
+            bool wrote = false;
+            if (g.zlo < qzmax) {
+                const uint32_t bx0 = g.bbx & 0xFFFFu, bx1 = g.bbx >> 16, by0 = g.bby & 0xFFFFu, by1 = g.bby >> 16;
+                const bool inb = (px >= bx0) & (px < bx1) & (py >= by0) & (py < by1);
+                // mod.rs:134-146 + 148-170: coverage of the 4 samples
+                const bool t0 = g.flags & 1u, t1 = g.flags & 2u, t2 = g.flags & 4u;
+                float e1s[4], e2s[4];
+                uint32_t cov = 0;
 #pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                sd[s] = 0.0f;
-                if (cov & (1u << s)) {
-                    float b0 = clamp_bary(e1s[s] * g.inv_area);
-                    float b1 = clamp_bary(e2s[s] * g.inv_area);
-                    float b2 = clamp_bary(1.0f - b0 - b1);
-                    sd[s] = b0 * g.z0 + b1 * g.z1 + b2 * g.z2;
-                    if (sd[s] < zbuf[s]) pass |= 1u << s;
+                for (int s = 0; s < 4; ++s) {
+                    const float sx = fx + sample_x(s), sy = fy + sample_y(s);
+                    const float e0 = edge(g.nx0, g.ny0, g.ax0, g.ay0, sx, sy);
+                    const float e1 = edge(g.nx1, g.ny1, g.ax1, g.ay1, sx, sy);
+                    const float e2 = edge(g.nx2, g.ny2, g.ax2, g.ay2, sx, sy);
+                    // bitwise on purpose: straight-line compares, no branches
+                    const bool in = ((e0 > 0.0f) | (t0 & !(e0 < 0.0f))) & ((e1 > 0.0f) | (t1 & !(e1 < 0.0f))) &
+                                    ((e2 > 0.0f) | (t2 & !(e2 < 0.0f)));
+                    cov |= (in ? 1u : 0u) << s;
+                    e1s[s] = e1;
+                    e2s[s] = e2;
+                }
+                if (!inb) cov = 0;
+                // mod.rs:225-253: linear depth at covered samples; mod.rs:363-378: strict LESS
+                float sd[4];
+                uint32_t pass = 0;
+                const float zbuf[4] = {zb0, zb1, zb2, zb3};
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const float b0 = clamp_bary(e1s[s] * g.inv_area);
+                    const float b1 = clamp_bary(e2s[s] * g.inv_area);
+                    const float b2 = clamp_bary(1.0f - b0 - b1);
+                    const float z = b0 * g.z0 + b1 * g.z1 + b2 * g.z2;
+                    const bool c = (cov >> s) & 1u;
+                    sd[s] = c ? z : 0.0f;
+                    pass |= (c & (z < zbuf[s])) ? (1u << s) : 0u;
+                }
+                if (pass != 0) {
+                    // mod.rs:70-99: perspective-correct barycentrics at the pixel
+                    // centre or at the first depth-passing sample
+                    float ix = fx + 0.5f, iy = fy + 0.5f;
+                    if (pass != 0xFu) {
+                        const int s = __builtin_ctz(pass);
+                        ix = fx + sample_x(s);
+                        iy = fy + sample_y(s);
+                    }
+                    const float ef0 = edge(g.nx0, g.ny0, g.ax0, g.ay0, ix, iy);
+                    const float ef1 = edge(g.nx1, g.ny1, g.ax1, g.ay1, ix, iy);
+                    const float ef2 = edge(g.nx2, g.ny2, g.ax2, g.ay2, ix, iy);
+                    const float f_u = ef1 / g.w0;
+                    const float f_v = ef2 / g.w1;
+                    const float f_w = ef0 / g.w2;
+                    const float sum = f_u + f_v + f_w;
+                    const float u = clamp_bary(f_u / sum);
+                    const float v = clamp_bary(f_v / sum);
+                    const float w = clamp_bary(1.0f - u - v);
+                    // main.rs:67-77 fragment shaders; Debug is Color::grayscale(depths[0])
+                    const uint32_t fs = g.flags >> 8;
+                    const float* at = attrs + (size_t)prim * ATTR_STRIDE;
+                    float cr = sd[0], cg = sd[0], cbl = sd[0], ca = 1.0f;
+                    if (fs == RZ_FS_COLOR) {
+                        cr = at[0] * u + at[4] * v + at[8] * w;
+                        cg = at[1] * u + at[5] * v + at[9] * w;
+                        cbl = at[2] * u + at[6] * v + at[10] * w;
+                        ca = at[3] * u + at[7] * v + at[11] * w;
+                    } else if (fs == RZ_FS_TEXTURE) {
+                        const DrawDev& dd = draws[g.draw];
+                        const float tu = at[0] * u + at[2] * v + at[4] * w;
+                        const float tv = at[1] * u + at[3] * v + at[5] * w;
+                        const RGBA c = texture_sample(dd.tex, dd.tw, dd.th, tu, tv, lut, oob);
+                        cr = c.r; cg = c.g; cbl = c.b; ca = c.a;
+                    }
+                    // mod.rs:380-397
+                    const uint32_t argb = to_argb(cr, cg, cbl, ca);
+                    if (pass & 1u) { cb0 = argb; zb0 = sd[0]; }
+                    if (pass & 2u) { cb1 = argb; zb1 = sd[1]; }
+                    if (pass & 4u) { cb2 = argb; zb2 = sd[2]; }
+                    if (pass & 8u) { cb3 = argb; zb3 = sd[3]; }
+                    wrote = true;
                 }
             }
-            if (pass == 0) continue;
-            // mod.rs:70-99: perspective-correct barycentrics at the pixel centre
-            // or at the first depth-passing sample
-            float ix = fx + 0.5f, iy = fy + 0.5f;
-            if (pass != 0xFu) {
-                int s = __builtin_ctz(pass);
-                ix = fx + sample_x(s);
-                iy = fy + sample_y(s);
+            if (__ballot(wrote)) {
+                const float lm = onscreen ? fmaxf(fmaxf(zb0, zb1), fmaxf(zb2, zb3)) : -FLT_MAX_F;
+                qzmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(wave_max(lm))));
             }
-            float ef0 = edge(g.nx0, g.ny0, g.ax0, g.ay0, ix, iy);
-            float ef1 = edge(g.nx1, g.ny1, g.ax1, g.ay1, ix, iy);
-            float ef2 = edge(g.nx2, g.ny2, g.ax2, g.ay2, ix, iy);
-            float f_u = ef1 / g.w0;
-            float f_v = ef2 / g.w1;
-            float f_w = ef0 / g.w2;
-            float sum = f_u + f_v + f_w;
-            float u = clamp_bary(f_u / sum);
-            float v = clamp_bary(f_v / sum);
-            float w = clamp_bary(1.0f - u - v);
-            // main.rs:67-77 fragment shaders
-            const DrawDev& dd = draws[g.draw];
-            const uint32_t fs = dd.fs;
-            const float* at = attrs + (size_t)prim * ATTR_STRIDE;
-            // Debug: Color::grayscale(frag_coords.depths[0]) (color.rs:65-72)
-            float cr = sd[0], cg = sd[0], cbl = sd[0], ca = 1.0f;
-            if (fs == RZ_FS_COLOR) {
-                cr = at[0] * u + at[4] * v + at[8] * w;
-                cg = at[1] * u + at[5] * v + at[9] * w;
-                cbl = at[2] * u + at[6] * v + at[10] * w;
-                ca = at[3] * u + at[7] * v + at[11] * w;
-            } else if (fs == RZ_FS_TEXTURE) {
-                float tu = at[0] * u + at[2] * v + at[4] * w;
-                float tv = at[1] * u + at[3] * v + at[5] * w;
-                RGBA c = texture_sample(dd.tex, dd.tw, dd.th, tu, tv, lut, oob);
-                cr = c.r; cg = c.g; cbl = c.b; ca = c.a;
-            }
-            // mod.rs:380-397
-            const uint32_t argb = to_argb(cr, cg, cbl, ca);
-            if (pass & 1u) { cb0 = argb; zb0 = sd[0]; }
-            if (pass & 2u) { cb1 = argb; zb1 = sd[1]; }
-            if (pass & 4u) { cb2 = argb; zb2 = sd[2]; }
-            if (pass & 8u) { cb3 = argb; zb3 = sd[3]; }
+            if (m == 0) break;
         }
     }
     if (__ballot(oob) && lane == 0) atomicOr(&cnt_w->flags, RZ_FLAG_TEXEL_OOB);
-    if (px < W && py < H) {
+    if (onscreen) {
         // buffers.rs:111-125
         uint32_t rs = ((cb0 & 0x00FF0000u) >> 16) + ((cb1 & 0x00FF0000u) >> 16) + ((cb2 & 0x00FF0000u) >> 16) +
                       ((cb3 & 0x00FF0000u) >> 16);
@@ -849,7 +911,6 @@ __global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict_
         }
     }
 }
-
 
 // largest index of a device index buffer (device-resident mesh upload check)
 __global__ __launch_bounds__(256) void k_max_index(const uint32_t* __restrict__ idx, uint64_t n,
@@ -896,30 +957,28 @@ void launch_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uint32_t n_ho
 
 void launch_setup(hipStream_t s, const DrawDev* draws, uint32_t d, uint32_t ntri, FrameCounters* cnt,
                   const uint32_t* cls, const uint32_t* nsub, const uint32_t* sub_off, const float* clip_out,
-                  uint32_t clip_cap, PrimGeom* geom, float* attrs, uint2* pbbox, uint32_t prim_cap, uint32_t W,
+                  uint32_t clip_cap, PrimGeom* geom, float* attrs, uint4* pbin, uint32_t prim_cap, uint32_t W,
                   uint32_t H, BinGeom bg) {
     if (ntri == 0) return;
     k_setup<<<(ntri + 255) / 256, 256, 0, s>>>(draws, d, cnt, cls, nsub, sub_off, clip_out, clip_cap, geom, attrs,
-                                               pbbox, prim_cap, W, H, bg);
+                                               pbin, prim_cap, W, H, bg);
 }
 
-void launch_bin(hipStream_t s, bool scatter, uint32_t grid, FrameCounters* cnt, const uint2* pbbox, BinGeom bg,
-                uint32_t* counts, const uint32_t* offsets, uint32_t* list_prim, uint2* list_bbox,
-                uint32_t pair_cap) {
+void launch_bin(hipStream_t s, bool scatter, uint32_t grid, FrameCounters* cnt, const uint4* pbin, BinGeom bg,
+                uint32_t* counts, const uint32_t* offsets, uint4* list, uint32_t pair_cap) {
     if (grid == 0) return;
     if (scatter)
-        k_bin<true><<<grid, 256, 0, s>>>(cnt, pbbox, bg, counts, offsets, list_prim, list_bbox, pair_cap, cnt);
+        k_bin<true><<<grid, 256, 0, s>>>(cnt, pbin, bg, counts, offsets, list, pair_cap, cnt);
     else
-        k_bin<false><<<grid, 256, 0, s>>>(cnt, pbbox, bg, counts, offsets, list_prim, list_bbox, pair_cap, cnt);
+        k_bin<false><<<grid, 256, 0, s>>>(cnt, pbin, bg, counts, offsets, list, pair_cap, cnt);
 }
 
 void launch_raster(hipStream_t s, FrameCounters* cnt, const PrimGeom* geom, const float* attrs,
-                   const DrawDev* draws, const uint32_t* offsets, const uint32_t* list_prim,
-                   const uint2* list_bbox, BinGeom bg, uint32_t W, uint32_t H, float clear_depth, uint32_t* out,
-                   uint32_t* msaa_color, float* msaa_depth) {
+                   const DrawDev* draws, const uint32_t* offsets, const uint4* list, BinGeom bg, uint32_t W,
+                   uint32_t H, float clear_depth, uint32_t* out, uint32_t* msaa_color, float* msaa_depth) {
     uint32_t nt = ((W + 15) / 16) * ((H + 15) / 16);
-    k_raster<<<nt, 256, 0, s>>>(cnt, geom, attrs, draws, offsets, list_prim, list_bbox, bg, W, H, clear_depth, out,
-                                msaa_color, msaa_depth, cnt);
+    k_raster<<<nt, 256, 0, s>>>(cnt, geom, attrs, draws, offsets, list, bg, W, H, clear_depth, out, msaa_color,
+                                msaa_depth, cnt);
 }
 
 }  // namespace rzk
