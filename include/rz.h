@@ -59,7 +59,9 @@ enum rz_stage { RZ_STAGE_GEOMETRY = 0, RZ_STAGE_BINNING = 1, RZ_STAGE_RASTER = 2
 /* ------------------------------------------------------------------ context */
 
 /* Renderer::new / Rasterizer::new (src/render.rs:48-69, src/rasterizer/mod.rs:273-281),
- * headless: no window. width, height in [1, 16384]. Uses the current HIP device. */
+ * headless: no window. width, height in [1, 16384] and the screen must fit 256
+ * bins of 256x256 pixels (e.g. up to 4096x4096 or 3840x2160), else RZ_ERR_LIMIT.
+ * Uses the current HIP device. */
 int rz_create(uint32_t width, uint32_t height, rz_ctx **out);
 /* Drop for Renderer. Accepts NULL. */
 int rz_destroy(rz_ctx *ctx);
@@ -115,12 +117,16 @@ int rz_resolve_msaa(rz_ctx *ctx, uint32_t *argb_out, uint32_t *color4_out, float
 /* RZ_FLAG_* bits raised while rendering the last resolved frame. */
 uint32_t rz_frame_flags(const rz_ctx *ctx);
 /* Counts of the last resolved frame: [input triangles, triangles clipped,
- * rasterizer triangles, coarse bin entries, frame replays]. */
-int rz_frame_stats(const rz_ctx *ctx, uint64_t out[5]);
+ * rasterizer triangles, coarse bin entries, frame replays, then -- only when
+ * the context was created with RZ_RASTER_STATS=1 in the environment, else 0 --
+ * raster list entries scanned, candidates after the exact culls, hits
+ * processed, hits with coverage, hits that wrote samples]. */
+int rz_frame_stats(const rz_ctx *ctx, uint64_t out[10]);
 
 /* ------------------------------------------------------------ tuning/timing */
 
-/* Coarse bin side in pixels (multiple of 16, 0 = automatic). */
+/* Coarse bin side in pixels: a multiple of 16 up to 256 with at most 256 bins
+ * on the screen, or 0 = automatic. Not inside a frame. */
 int rz_set_coarse_bin(rz_ctx *ctx, uint32_t pixels);
 /* Enable (1) / disable (0) per-stage HIP event timing; enabling resets sums. */
 int rz_profile(rz_ctx *ctx, int enable);

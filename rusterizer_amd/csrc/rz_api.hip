@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -24,6 +25,9 @@ using namespace rzk;
 namespace {
 
 constexpr uint32_t MAX_DIM = 16384;
+// a coarse bin holds at most 256 16x16 tiles (fine binning keeps one running
+// count per tile in 4 registers per lane)
+constexpr uint32_t MAX_COARSE_PX = 256;
 constexpr uint32_t MAX_DRAWS = 65536;
 
 const char* kNullCtxMsg = "null context";
@@ -83,13 +87,17 @@ struct rz_ctx {
     // coarse bins
     DBuf<uint32_t> bin_counts, bin_offsets;
     DBuf<uint4> list;
+    // fine bins (16x16 tiles)
+    DBuf<uint32_t> chunk_base, fine_counts, fine_off;
+    DBuf<uint4> flist;
     // outputs
     DBuf<uint32_t> out;
     DBuf<uint32_t> msaa_color;
     DBuf<float> msaa_depth;
 
     uint32_t frame_flags = 0;
-    uint64_t frame_stats[5] = {0, 0, 0, 0, 0};
+    uint64_t frame_stats[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    bool raster_stats = false;  // RZ_RASTER_STATS=1 at rz_create
     uint64_t frame_tris = 0;
 
     bool prof = false;
@@ -160,9 +168,18 @@ uint32_t bin_matrix_len(const rz_ctx* c) {
     return (uint32_t)(nb * nc);
 }
 
-// capacities for a frame whose draws total `tris` input triangles
+// upper bound of coarse-list chunks: every bin adds at most one partial chunk
+uint32_t nch_max(const rz_ctx* c) { return (uint32_t)(c->list.cap / CHUNK + MAX_BINS + 1); }
+
+uint32_t fine_matrix_len(const rz_ctx* c) {
+    uint32_t T = (c->bg.cb / 16) * (c->bg.cb / 16);
+    return T * nch_max(c);
+}
+
+// capacities for the frame's rasterizer triangles, clip slots and bin entries
 int ensure_frame_buffers(rz_ctx* c, uint64_t prims_need, uint64_t clip_need, uint64_t pairs_need,
-                         uint64_t max_draw_tris, bool* grew_prims) {
+                         uint64_t max_draw_tris, bool* grew_prims, uint64_t fine_need = 0) {
+    if (fine_need == 0) fine_need = std::max<uint64_t>(c->flist.cap, pairs_need * 4);
     int rc;
     if ((rc = grow(c, c->cls, max_draw_tris))) return rc;
     if ((rc = grow(c, c->nsub, max_draw_tris))) return rc;
@@ -178,7 +195,12 @@ int ensure_frame_buffers(rz_ctx* c, uint64_t prims_need, uint64_t clip_need, uin
     size_t bm = bin_matrix_len(c);
     if ((rc = grow(c, c->bin_counts, bm))) return rc;
     if ((rc = grow(c, c->bin_offsets, bm))) return rc;
-    size_t scan_n = std::max<size_t>(c->cls.cap, bm);
+    if ((rc = grow(c, c->chunk_base, MAX_BINS + 1))) return rc;
+    if ((rc = grow(c, c->flist, fine_need))) return rc;
+    size_t fm = fine_matrix_len(c);
+    if ((rc = grow(c, c->fine_counts, fm))) return rc;
+    if ((rc = grow(c, c->fine_off, fm))) return rc;
+    size_t scan_n = std::max<size_t>(std::max<size_t>(c->cls.cap, bm), fm);
     if ((rc = grow(c, c->parts, (scan_n + SCAN_TILE - 1) / SCAN_TILE + 1))) return rc;
     if (grew_prims) *grew_prims = g;
     return RZ_OK;
@@ -192,7 +214,7 @@ void enqueue_draw_geometry(rz_ctx* c, uint32_t d) {
                     (uint32_t)c->clip_tri.cap);
     launch_clip(s, c->d_draws, d, c->d_cnt, c->clip_tri.p, (uint32_t)c->clip_tri.cap, c->nsub.p, c->clip_out.p,
                 c->clip_n.p);
-    launch_scan(s, c->nsub.p, c->suboff.p, h.ntri, nullptr, 0, c->parts.p, &c->d_cnt->nprims, nullptr);
+    launch_scan(s, c->nsub.p, c->suboff.p, h.ntri, nullptr, 0, 1, c->parts.p, &c->d_cnt->nprims, nullptr);
     launch_setup(s, c->d_draws, d, h.ntri, c->d_cnt, c->cls.p, c->nsub.p, c->suboff.p, c->clip_out.p,
                  (uint32_t)c->clip_tri.cap, c->geom.p, c->attrs.p, c->pbin.p, (uint32_t)c->geom.cap, c->W, c->H,
                  c->bg);
@@ -258,14 +280,17 @@ int resolve_impl(rz_ctx* c, uint32_t* host_out, uint32_t* dev_out, uint32_t* hos
         const uint32_t nc_max = (uint32_t)((c->geom.cap + CHUNK - 1) / CHUNK);
         const uint32_t bin_grid = std::min<uint32_t>((nc_max + 3) / 4, 4096);
         launch_bin(s, false, bin_grid, c->d_cnt, c->pbin.p, c->bg, c->bin_counts.p, nullptr, nullptr, 0);
-        launch_scan(s, c->bin_counts.p, c->bin_offsets.p, nb * nc_max, &c->d_cnt->nprims, nb, c->parts.p, nullptr,
-                    &c->d_cnt->pairs_binned);
+        launch_scan(s, c->bin_counts.p, c->bin_offsets.p, nb * nc_max, &c->d_cnt->nprims, nb, CHUNK, c->parts.p,
+                    nullptr, &c->d_cnt->pairs_binned);
         launch_bin(s, true, bin_grid, c->d_cnt, c->pbin.p, c->bg, nullptr, c->bin_offsets.p, c->list.p,
                    (uint32_t)c->list.cap);
+        launch_fine(s, c->d_cnt, c->bin_offsets.p, c->list.p, c->chunk_base.p, c->bg, c->fine_counts.p,
+                    c->fine_off.p, c->flist.p, (uint32_t)c->flist.cap, nch_max(c), c->parts.p);
         if (c->prof) HIP_TRY(c, hipEventRecord(c->ev[2], s));
-        launch_raster(s, c->d_cnt, c->geom.p, c->attrs.p, c->d_draws, c->bin_offsets.p, c->list.p, c->bg, c->W,
-                      c->H, c->clear_depth, target,
-                      want_msaa ? c->msaa_color.p : nullptr, want_msaa ? c->msaa_depth.p : nullptr);
+        launch_raster(s, c->d_cnt, c->geom.p, c->attrs.p, c->d_draws, c->chunk_base.p, c->fine_off.p, c->flist.p,
+                      (uint32_t)c->flist.cap, c->bg, c->W, c->H, c->clear_depth, target,
+                      want_msaa ? c->msaa_color.p : nullptr, want_msaa ? c->msaa_depth.p : nullptr,
+                      c->raster_stats);
         if (c->prof) HIP_TRY(c, hipEventRecord(c->ev[3], s));
         HIP_TRY(c, hipGetLastError());
         HIP_TRY(c, hipMemcpyAsync(c->h_cnt, c->d_cnt, sizeof(FrameCounters), hipMemcpyDeviceToHost, s));
@@ -286,8 +311,10 @@ int resolve_impl(rz_ctx* c, uint32_t* host_out, uint32_t* dev_out, uint32_t* hos
         if (fc.overflow & OVF_PRIMS) prims_need = std::max<uint64_t>(fc.nprims, c->geom.cap * 2);
         if (fc.overflow & OVF_CLIP) clip_need = std::max<uint64_t>(fc.clip_need, c->clip_tri.cap * 2);
         if (fc.overflow & OVF_PAIRS) pairs_need = std::max<uint64_t>(fc.npairs, c->list.cap * 2);
+        uint64_t fine_need = std::max<uint64_t>(fc.fine_total, c->flist.cap);
+        if (fc.overflow & OVF_FINE) fine_need = std::max<uint64_t>(fc.fine_total, c->flist.cap * 2);
         if ((rc = ensure_frame_buffers(c, prims_need, clip_need, pairs_need, std::max(max_draw_tris(c), 1024u),
-                                       nullptr))) {
+                                       nullptr, fine_need))) {
             end_frame(c);
             return rc;
         }
@@ -305,6 +332,7 @@ int resolve_impl(rz_ctx* c, uint32_t* host_out, uint32_t* dev_out, uint32_t* hos
     c->frame_stats[2] = fc.nprims;
     c->frame_stats[3] = fc.pairs_binned;
     c->frame_stats[4] = replays;
+    for (int i = 0; i < 5; ++i) c->frame_stats[5 + i] = fc.rstat[i];
     if (c->prof) {
         float a = 0, b = 0, d = 0;
         HIP_TRY(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
@@ -391,6 +419,9 @@ int rz_create(uint32_t width, uint32_t height, rz_ctx** out) {
     if (!out) return RZ_ERR_ARG;
     *out = nullptr;
     if (width == 0 || height == 0 || width > MAX_DIM || height > MAX_DIM) return RZ_ERR_ARG;
+    // the screen must fit MAX_BINS coarse bins of at most MAX_COARSE_PX pixels
+    if (((width + MAX_COARSE_PX - 1) / MAX_COARSE_PX) * ((height + MAX_COARSE_PX - 1) / MAX_COARSE_PX) > MAX_BINS)
+        return RZ_ERR_LIMIT;
     rz_ctx* c = new rz_ctx();
     c->W = width;
     c->H = height;
@@ -399,6 +430,8 @@ int rz_create(uint32_t width, uint32_t height, rz_ctx** out) {
         c->uni.world[i] = c->uni.view[i] = c->uni.proj[i] = v;
     }
     c->bg = choose_bins(width, height, 0);
+    const char* st = getenv("RZ_RASTER_STATS");
+    c->raster_stats = st && st[0] == '1';
     auto bad = [&](hipError_t e) {
         std::string m = hipGetErrorString(e);
         rz_destroy(c);
@@ -441,6 +474,10 @@ int rz_destroy(rz_ctx* c) {
     release(c->bin_counts);
     release(c->bin_offsets);
     release(c->list);
+    release(c->chunk_base);
+    release(c->fine_counts);
+    release(c->fine_off);
+    release(c->flist);
     release(c->out);
     release(c->msaa_color);
     release(c->msaa_depth);
@@ -593,7 +630,7 @@ int rz_resolve_msaa(rz_ctx* c, uint32_t* argb_out, uint32_t* color4_out, float* 
 
 uint32_t rz_frame_flags(const rz_ctx* c) { return c ? c->frame_flags : 0u; }
 
-int rz_frame_stats(const rz_ctx* c, uint64_t out[5]) {
+int rz_frame_stats(const rz_ctx* c, uint64_t out[10]) {
     if (!c || !out) return RZ_ERR_ARG;
     memcpy(out, c->frame_stats, sizeof(c->frame_stats));
     return RZ_OK;
@@ -601,7 +638,8 @@ int rz_frame_stats(const rz_ctx* c, uint64_t out[5]) {
 
 int rz_set_coarse_bin(rz_ctx* c, uint32_t pixels) {
     if (!c) return RZ_ERR_ARG;
-    if (pixels % 16 != 0 || pixels > MAX_DIM) return fail(c, RZ_ERR_ARG, "coarse bin must be a multiple of 16");
+    if (pixels % 16 != 0 || pixels > MAX_COARSE_PX)
+        return fail(c, RZ_ERR_ARG, "coarse bin must be a multiple of 16 and at most 256 pixels (0 = automatic)");
     if (!c->draw_mesh.empty()) return fail(c, RZ_ERR_STATE, "cannot change bins inside a frame");
     BinGeom g = choose_bins(c->W, c->H, pixels);
     if (g.nbx * g.nby > MAX_BINS) return fail(c, RZ_ERR_LIMIT, "more than 256 coarse bins");

@@ -19,7 +19,7 @@ constexpr int SCAN_ITEMS = 8;
 constexpr uint32_t SCAN_TILE = 256 * SCAN_ITEMS;
 
 // overflow bits: the frame must be replayed with larger buffers
-constexpr uint32_t OVF_PRIMS = 1, OVF_CLIP = 2, OVF_PAIRS = 4, OVF_POLY = 8;
+constexpr uint32_t OVF_PRIMS = 1, OVF_CLIP = 2, OVF_PAIRS = 4, OVF_POLY = 8, OVF_FINE = 16;
 
 struct FrameCounters {
     uint32_t nprims;        // rasterizer triangles of the frame so far
@@ -29,8 +29,14 @@ struct FrameCounters {
     uint32_t flags;         // RZ_FLAG_* bits
     uint32_t nclipped;      // input triangles sent to the clipper
     unsigned long long npairs;  // coarse-bin entries counted by k_setup
-    uint32_t pairs_binned;  // entries placed by the bin scan
+    uint32_t pairs_binned;  // entries placed by the coarse bin scan
+    uint32_t nch;           // chunks of the coarse lists (fine binning)
+    uint32_t fine_total;    // entries placed by the fine bin scan
     uint32_t pad;
+    // raster work counters (stats variant only): list entries scanned,
+    // candidates after the bbox / depth / edge culls, hits processed,
+    // hits with coverage, hits that shaded
+    unsigned long long rstat[5];
 };
 
 // Per-draw state on the device (uniforms + mesh streams + fragment shader).
@@ -82,18 +88,23 @@ void launch_classify(hipStream_t s, const DrawDev* draws, uint32_t d, uint32_t n
                      uint32_t* cls, uint32_t* nsub, uint32_t* clip_tri, uint32_t clip_cap);
 void launch_clip(hipStream_t s, const DrawDev* draws, uint32_t d, FrameCounters* cnt, const uint32_t* clip_tri,
                  uint32_t clip_cap, uint32_t* nsub, float* clip_out, uint32_t* clip_n);
-// exclusive scan; with n_dev the length is bins * ceil(*n_dev / CHUNK) and
-// n_host is an upper bound for the grid.
+// exclusive scan; with n_dev the length is mul * ceil(*n_dev / div), at most
+// n_host (which also sizes the grid).
 void launch_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uint32_t n_host, const uint32_t* n_dev,
-                 uint32_t bins, uint32_t* parts, uint32_t* base_total, uint32_t* total2);
+                 uint32_t mul, uint32_t div, uint32_t* parts, uint32_t* base_total, uint32_t* total2);
 void launch_setup(hipStream_t s, const DrawDev* draws, uint32_t d, uint32_t ntri, FrameCounters* cnt,
                   const uint32_t* cls, const uint32_t* nsub, const uint32_t* sub_off, const float* clip_out,
                   uint32_t clip_cap, PrimGeom* geom, float* attrs, uint4* pbin, uint32_t prim_cap, uint32_t W,
                   uint32_t H, BinGeom bg);
 void launch_bin(hipStream_t s, bool scatter, uint32_t grid, FrameCounters* cnt, const uint4* pbin, BinGeom bg,
                 uint32_t* counts, const uint32_t* offsets, uint4* list, uint32_t pair_cap);
+// fine binning of the coarse lists into 16x16-tile lists (nch_max bounds the chunk count)
+void launch_fine(hipStream_t s, FrameCounters* cnt, const uint32_t* coff, const uint4* clist, uint32_t* chunk_base,
+                 BinGeom bg, uint32_t* fcounts, uint32_t* foff, uint4* flist, uint32_t fcap, uint32_t nch_max,
+                 uint32_t* parts);
 void launch_raster(hipStream_t s, FrameCounters* cnt, const PrimGeom* geom, const float* attrs,
-                   const DrawDev* draws, const uint32_t* offsets, const uint4* list, BinGeom bg, uint32_t W,
-                   uint32_t H, float clear_depth, uint32_t* out, uint32_t* msaa_color, float* msaa_depth);
+                   const DrawDev* draws, const uint32_t* chunk_base, const uint32_t* foff, const uint4* list,
+                   uint32_t fcap, BinGeom bg, uint32_t W, uint32_t H, float clear_depth, uint32_t* out,
+                   uint32_t* msaa_color, float* msaa_depth, bool stats);
 
 }  // namespace rzk

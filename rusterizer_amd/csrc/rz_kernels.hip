@@ -111,6 +111,9 @@ __global__ void k_frame_reset(FrameCounters* __restrict__ cnt) {
         cnt->nclipped = 0;
         cnt->npairs = 0;
         cnt->pairs_binned = 0;
+        cnt->nch = 0;
+        cnt->fine_total = 0;
+        for (int i = 0; i < 5; ++i) cnt->rstat[i] = 0;
     }
 }
 __global__ void k_draw_reset(FrameCounters* __restrict__ cnt) {
@@ -324,10 +327,10 @@ __global__ __launch_bounds__(64) void k_clip(const DrawDev* __restrict__ draws, 
 // Exclusive scan of u32 (3 launches: tile sums, scan of sums, tile scan).
 // Element count is n_host, or (when n_dev is set) bins * ceil(*n_dev / CHUNK).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t scan_len(uint32_t n_host, const uint32_t* n_dev, uint32_t bins) {
+__device__ __forceinline__ uint32_t scan_len(uint32_t n_host, const uint32_t* n_dev, uint32_t mul, uint32_t div) {
     if (n_dev == nullptr) return n_host;
     uint32_t s = uniform(*n_dev);
-    uint64_t n = (uint64_t)bins * ((s + CHUNK - 1) / CHUNK);
+    uint64_t n = (uint64_t)mul * ((s + div - 1) / div);
     return n < n_host ? (uint32_t)n : n_host;  // never past the buffers the host sized
 }
 
@@ -352,10 +355,10 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* l
 }
 
 __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ in, uint32_t n_host,
-                                                     const uint32_t* __restrict__ n_dev, uint32_t bins,
+                                                     const uint32_t* __restrict__ n_dev, uint32_t mul, uint32_t div,
                                                      uint32_t* __restrict__ parts) {
     __shared__ uint32_t lds[8];
-    uint32_t n = scan_len(n_host, n_dev, bins);
+    uint32_t n = scan_len(n_host, n_dev, mul, div);
     uint32_t base = blockIdx.x * SCAN_TILE;
     if (base >= n) return;
     uint32_t s = 0;
@@ -371,11 +374,11 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
 // one block: exclusive scan of parts, plus base (*base_total, if given);
 // writes the grand total back to *base_total and to *total2 (if given).
 __global__ __launch_bounds__(1024) void k_scan_top(uint32_t* __restrict__ parts, uint32_t n_host,
-                                                   const uint32_t* __restrict__ n_dev, uint32_t bins,
+                                                   const uint32_t* __restrict__ n_dev, uint32_t mul, uint32_t div,
                                                    uint32_t* __restrict__ base_total,
                                                    uint32_t* __restrict__ total2) {
     __shared__ uint32_t lds[16];
-    uint32_t n = scan_len(n_host, n_dev, bins);
+    uint32_t n = scan_len(n_host, n_dev, mul, div);
     uint32_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     uint32_t run = base_total ? *base_total : 0u;
     for (uint32_t off = 0; off < nb; off += blockDim.x) {
@@ -393,11 +396,11 @@ __global__ __launch_bounds__(1024) void k_scan_top(uint32_t* __restrict__ parts,
 }
 
 __global__ __launch_bounds__(256) void k_scan_down(const uint32_t* __restrict__ in, uint32_t n_host,
-                                                   const uint32_t* __restrict__ n_dev, uint32_t bins,
+                                                   const uint32_t* __restrict__ n_dev, uint32_t mul, uint32_t div,
                                                    const uint32_t* __restrict__ parts,
                                                    uint32_t* __restrict__ out) {
     __shared__ uint32_t lds[8];
-    uint32_t n = scan_len(n_host, n_dev, bins);
+    uint32_t n = scan_len(n_host, n_dev, mul, div);
     uint32_t base = blockIdx.x * SCAN_TILE;
     if (base >= n) return;
     uint32_t v[SCAN_ITEMS];
@@ -631,6 +634,126 @@ __global__ __launch_bounds__(256) void k_bin(const FrameCounters* __restrict__ c
 }
 
 // ---------------------------------------------------------------------------
+// Fine binning: each coarse bin's list is split again into its 16x16 screen
+// tiles (the raster workgroups), keeping submission order.
+// k_fine_prep numbers the coarse lists' chunks of CHUNK entries:
+//   chunk_base[b] = chunks of all bins before b, chunk_base[nb] = total (nch).
+// The fine count matrix holds, for coarse bin b, local tile lt and chunk c,
+// the entry (T * chunk_base[b] + lt * nch_b + c); its exclusive scan gives
+// every tile one contiguous, ordered list.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(512) void k_fine_prep(FrameCounters* __restrict__ cnt,
+                                                   const uint32_t* __restrict__ coff, BinGeom bg,
+                                                   uint32_t* __restrict__ chunk_base) {
+    __shared__ uint32_t lds[8];
+    if (uniform(cnt->overflow) != 0) return;
+    const uint32_t S = uniform(cnt->nprims);
+    const uint32_t NC = (S + CHUNK - 1) / CHUNK;
+    const uint32_t nb = bg.nbx * bg.nby;
+    const uint32_t b = threadIdx.x;
+    uint32_t len = 0;
+    if (b < nb && NC > 0) {
+        uint32_t beg = coff[(size_t)b * NC];
+        uint32_t end = (b + 1 < nb) ? coff[(size_t)(b + 1) * NC] : cnt->pairs_binned;
+        len = end - beg;
+    }
+    uint32_t tot;
+    uint32_t ex = block_exclusive_scan((len + CHUNK - 1) / CHUNK, lds, &tot);
+    if (b <= nb) chunk_base[b] = ex;
+    if (threadIdx.x == 0) cnt->nch = tot;
+}
+
+template <bool SCATTER>
+__global__ __launch_bounds__(256) void k_fine_bin(const FrameCounters* __restrict__ cnt,
+                                                  const uint32_t* __restrict__ coff, const uint4* __restrict__ clist,
+                                                  const uint32_t* __restrict__ chunk_base, BinGeom bg,
+                                                  uint32_t* __restrict__ fcounts, const uint32_t* __restrict__ foff,
+                                                  uint4* __restrict__ flist, uint32_t fcap,
+                                                  FrameCounters* __restrict__ cnt_w) {
+    __shared__ uint32_t cbase[MAX_BINS + 1];
+    if (uniform(cnt->overflow) != 0) return;
+    const uint32_t nb = bg.nbx * bg.nby;
+    for (uint32_t i = threadIdx.x; i <= nb; i += blockDim.x) cbase[i] = chunk_base[i];
+    __syncthreads();
+    const uint32_t NCH = cbase[nb];
+    const uint32_t S = uniform(cnt->nprims);
+    const uint32_t NC = (S + CHUNK - 1) / CHUNK;
+    const uint32_t TS = bg.cb / 16, T = TS * TS;
+    if (T > 256) return;  // the host never configures more tiles per coarse bin
+    const uint32_t lane = lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    bool ovf = false;
+    for (uint32_t g = wave; g < NCH; g += nwaves) {
+        uint32_t lo = 0, hi = nb;  // largest b < nb with cbase[b] <= g
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (cbase[mid] <= g) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t b = lo;
+        const uint32_t c = g - cbase[b], nch_b = cbase[b + 1] - cbase[b];
+        const uint32_t cbeg = uniform(coff[(size_t)b * NC]);
+        const uint32_t cend = (b + 1 < nb) ? uniform(coff[(size_t)(b + 1) * NC]) : uniform(cnt->pairs_binned);
+        const uint32_t binx0 = (b % bg.nbx) * bg.cb, biny0 = (b / bg.nbx) * bg.cb;
+        const size_t mbase = (size_t)T * cbase[b] + c;
+        uint32_t run0 = 0, run1 = 0, run2 = 0, run3 = 0;
+        if (SCATTER) {
+            if (lane < T) run0 = foff[mbase + (size_t)lane * nch_b];
+            if (lane + 64 < T) run1 = foff[mbase + (size_t)(lane + 64) * nch_b];
+            if (lane + 128 < T) run2 = foff[mbase + (size_t)(lane + 128) * nch_b];
+            if (lane + 192 < T) run3 = foff[mbase + (size_t)(lane + 192) * nch_b];
+        }
+        for (uint32_t sub = 0; sub < CHUNK / 64; ++sub) {
+            const uint32_t e = cbeg + c * CHUNK + sub * 64 + lane;
+            const bool valid = e < cend;
+            uint4 en = make_uint4(0u, 0u, 0u, 0u);
+            if (valid) en = clist[e];
+            const uint32_t x0 = en.y & 0xFFFFu, x1 = en.y >> 16, y0 = en.z & 0xFFFFu, y1 = en.z >> 16;
+            // local tile range inside this coarse bin (the entry overlaps it)
+            const uint32_t lx0 = (max(x0, binx0) - binx0) >> 4;
+            const uint32_t lx1 = valid ? (min(x1, binx0 + bg.cb) - 1 - binx0) >> 4 : 0u;
+            const uint32_t ly0 = (max(y0, biny0) - biny0) >> 4;
+            const uint32_t ly1 = valid ? (min(y1, biny0 + bg.cb) - 1 - biny0) >> 4 : 0u;
+            for (uint32_t ty = 0; ty < TS; ++ty) {
+                const bool rowhit = valid & (ty >= ly0) & (ty <= ly1);
+                if (__ballot(rowhit) == 0) continue;
+                for (uint32_t tx = 0; tx < TS; ++tx) {
+                    const bool hit = rowhit & (tx >= lx0) & (tx <= lx1);
+                    const uint64_t m = __ballot(hit);
+                    if (m == 0) continue;
+                    const uint32_t lt = ty * TS + tx;
+                    const uint32_t j = lt >> 6, bl = lt & 63;
+                    const uint32_t cnt_m = (uint32_t)__popcll(m);
+                    if (SCATTER) {
+                        const uint32_t rv = j == 0 ? run0 : (j == 1 ? run1 : (j == 2 ? run2 : run3));
+                        const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)rv, (int)bl);
+                        if (hit) {
+                            const uint32_t pos = r0 + mbcnt(m);
+                            if (pos < fcap) flist[pos] = en;
+                            else ovf = true;
+                        }
+                    }
+                    if (lane == bl) {
+                        if (j == 0) run0 += cnt_m;
+                        else if (j == 1) run1 += cnt_m;
+                        else if (j == 2) run2 += cnt_m;
+                        else run3 += cnt_m;
+                    }
+                }
+            }
+        }
+        if (!SCATTER) {
+            if (lane < T) fcounts[mbase + (size_t)lane * nch_b] = run0;
+            if (lane + 64 < T) fcounts[mbase + (size_t)(lane + 64) * nch_b] = run1;
+            if (lane + 128 < T) fcounts[mbase + (size_t)(lane + 128) * nch_b] = run2;
+            if (lane + 192 < T) fcounts[mbase + (size_t)(lane + 192) * nch_b] = run3;
+        }
+    }
+    if (SCATTER && __ballot(ovf) && lane == 0) atomicOr(&cnt_w->overflow, OVF_FINE);
+}
+
+// ---------------------------------------------------------------------------
 // k_raster: one workgroup per 16x16 screen tile, one wave per 8x8 quadrant,
 // one lane per pixel. The pixel's 4 MSAA colours and depths live in the
 // lane's registers for the whole frame; the wave walks its coarse bin's list
@@ -709,8 +832,14 @@ __device__ __forceinline__ uint32_t to_argb(float r, float g, float b, float a) 
            sat_u32(b * 255.0f);
 }
 
+// vector.rs:17-23 on (n, p - a) computes (0.0 + nx*(x-ax)) + ny*(y-ay). The
+// leading 0.0 + only turns a -0.0 product into +0.0, which can flip the sign
+// of a zero edge value; no output depends on that sign: the inside test
+// treats +-0 alike, a zero barycentric of either sign gives the same depth
+// (a sum that ends in +0 or a positive term) and the same attributes, and
+// to_argb / texel addressing map +-0 to the same value. So it is left out.
 __device__ __forceinline__ float edge(float nx, float ny, float ax, float ay, float x, float y) {
-    return (0.0f + nx * (x - ax)) + ny * (y - ay);  // vector.rs:17-23 on (n, p - a)
+    return nx * (x - ax) + ny * (y - ay);
 }
 
 // false only if every sample position in [xl, xh] x [yl, yh] evaluates the
@@ -730,11 +859,13 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+template <bool STATS>
 __global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict__ cnt,
                                                 const PrimGeom* __restrict__ geom, const float* __restrict__ attrs,
                                                 const DrawDev* __restrict__ draws,
-                                                const uint32_t* __restrict__ offsets,
-                                                const uint4* __restrict__ list, BinGeom bg,
+                                                const uint32_t* __restrict__ chunk_base,
+                                                const uint32_t* __restrict__ foff,
+                                                const uint4* __restrict__ list, uint32_t fcap, BinGeom bg,
                                                 uint32_t W, uint32_t H, float clear_depth,
                                                 uint32_t* __restrict__ out, uint32_t* __restrict__ msaa_color,
                                                 float* __restrict__ msaa_depth, FrameCounters* __restrict__ cnt_w) {
@@ -765,20 +896,26 @@ __global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict_
     float qzmax = __int_as_float(
         __builtin_amdgcn_readfirstlane(__float_as_int(wave_max(onscreen ? clear_depth : -FLT_MAX_F))));
 
-    const uint32_t S = uniform(cnt->nprims);
-    const uint32_t NC = (S + CHUNK - 1) / CHUNK;
+    // this tile's fine list (see k_fine_bin for the layout)
     const uint32_t nb = bg.nbx * bg.nby;
-    const uint32_t cbin = (qy0 / bg.cb) * bg.nbx + (qx0 / bg.cb);
-    uint32_t lbeg = 0, lend = 0;
-    if (NC > 0) {
-        lbeg = uniform(offsets[(size_t)cbin * NC]);
-        lend = (cbin + 1 < nb) ? uniform(offsets[(size_t)(cbin + 1) * NC]) : uniform(cnt->pairs_binned);
-    }
+    const uint32_t TS = bg.cb / 16, T = TS * TS;
+    const uint32_t cbx = (tx * 16) / bg.cb, cby = (ty * 16) / bg.cb;
+    const uint32_t cbin = cby * bg.nbx + cbx;
+    const uint32_t lt = ((ty * 16 - cby * bg.cb) >> 4) * TS + ((tx * 16 - cbx * bg.cb) >> 4);
+    const uint32_t chb = uniform(chunk_base[cbin]), nch_b = uniform(chunk_base[cbin + 1]) - chb;
+    const size_t nfine = (size_t)T * uniform(chunk_base[nb]);
+    const size_t s_idx = (size_t)T * chb + (size_t)lt * nch_b, e_idx = s_idx + nch_b;
+    const uint32_t ftot = uniform(cnt->fine_total);
+    // (clamped to the list buffer: a bad range can never read past it)
+    const uint32_t lend = min(e_idx < nfine ? uniform(foff[e_idx]) : ftot, fcap);
+    const uint32_t lbeg = min(s_idx < nfine ? uniform(foff[s_idx]) : ftot, lend);
 
+    uint32_t st_entries = 0, st_cand = 0, st_hits = 0, st_cov = 0, st_shaded = 0;
     for (uint32_t base = lbeg; base < lend; base += 64) {
         const uint32_t i = base + lane;
         uint32_t myprim = 0;
         bool cand = false;
+        if (STATS) st_entries += min(64u, lend - base);
         if (i < lend) {
             const uint4 en = list[i];
             myprim = en.x;
@@ -793,6 +930,7 @@ __global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict_
                    edge_may_cover(g2.z, g2.w, g1.x, g1.y, qxl, qxh, qyl, qyh);
         }
         uint64_t m = __ballot(cand);
+        if (STATS) st_cand += (uint32_t)__popcll(m);
         if (m == 0) continue;
         uint32_t nprim = (uint32_t)__builtin_amdgcn_readlane((int)myprim, (int)(__ffsll((unsigned long long)m) - 1));
         PrimGeom nxt = geom[nprim];
@@ -826,6 +964,10 @@ __global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict_
                     e2s[s] = e2;
                 }
                 if (!inb) cov = 0;
+                if (STATS) {
+                    st_hits += 1;
+                    st_cov += __ballot(cov != 0) ? 1u : 0u;
+                }
                 // mod.rs:225-253: linear depth at covered samples; mod.rs:363-378: strict LESS
                 float sd[4];
                 uint32_t pass = 0;
@@ -884,6 +1026,7 @@ __global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict_
                     wrote = true;
                 }
             }
+            if (STATS) st_shaded += __ballot(wrote) ? 1u : 0u;
             if (__ballot(wrote)) {
                 const float lm = onscreen ? fmaxf(fmaxf(zb0, zb1), fmaxf(zb2, zb3)) : -FLT_MAX_F;
                 qzmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(wave_max(lm))));
@@ -892,6 +1035,13 @@ __global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict_
         }
     }
     if (__ballot(oob) && lane == 0) atomicOr(&cnt_w->flags, RZ_FLAG_TEXEL_OOB);
+    if (STATS && lane == 0) {
+        atomicAdd(&cnt_w->rstat[0], (unsigned long long)st_entries);
+        atomicAdd(&cnt_w->rstat[1], (unsigned long long)st_cand);
+        atomicAdd(&cnt_w->rstat[2], (unsigned long long)st_hits);
+        atomicAdd(&cnt_w->rstat[3], (unsigned long long)st_cov);
+        atomicAdd(&cnt_w->rstat[4], (unsigned long long)st_shaded);
+    }
     if (onscreen) {
         // buffers.rs:111-125
         uint32_t rs = ((cb0 & 0x00FF0000u) >> 16) + ((cb1 & 0x00FF0000u) >> 16) + ((cb2 & 0x00FF0000u) >> 16) +
@@ -948,11 +1098,11 @@ void launch_clip(hipStream_t s, const DrawDev* draws, uint32_t d, FrameCounters*
 }
 
 void launch_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uint32_t n_host, const uint32_t* n_dev,
-                 uint32_t bins, uint32_t* parts, uint32_t* base_total, uint32_t* total2) {
+                 uint32_t mul, uint32_t div, uint32_t* parts, uint32_t* base_total, uint32_t* total2) {
     uint32_t nb = (n_host + SCAN_TILE - 1) / SCAN_TILE;
-    if (nb > 0) k_scan_reduce<<<nb, 256, 0, s>>>(in, n_host, n_dev, bins, parts);
-    k_scan_top<<<1, 1024, 0, s>>>(parts, n_host, n_dev, bins, base_total, total2);
-    if (nb > 0) k_scan_down<<<nb, 256, 0, s>>>(in, n_host, n_dev, bins, parts, out);
+    if (nb > 0) k_scan_reduce<<<nb, 256, 0, s>>>(in, n_host, n_dev, mul, div, parts);
+    k_scan_top<<<1, 1024, 0, s>>>(parts, n_host, n_dev, mul, div, base_total, total2);
+    if (nb > 0) k_scan_down<<<nb, 256, 0, s>>>(in, n_host, n_dev, mul, div, parts, out);
 }
 
 void launch_setup(hipStream_t s, const DrawDev* draws, uint32_t d, uint32_t ntri, FrameCounters* cnt,
@@ -973,12 +1123,28 @@ void launch_bin(hipStream_t s, bool scatter, uint32_t grid, FrameCounters* cnt, 
         k_bin<false><<<grid, 256, 0, s>>>(cnt, pbin, bg, counts, offsets, list, pair_cap, cnt);
 }
 
+void launch_fine(hipStream_t s, FrameCounters* cnt, const uint32_t* coff, const uint4* clist, uint32_t* chunk_base,
+                 BinGeom bg, uint32_t* fcounts, uint32_t* foff, uint4* flist, uint32_t fcap, uint32_t nch_max,
+                 uint32_t* parts) {
+    k_fine_prep<<<1, 512, 0, s>>>(cnt, coff, bg, chunk_base);
+    const uint32_t grid = nch_max / 4 + 1 < 4096 ? nch_max / 4 + 1 : 4096;
+    k_fine_bin<false><<<grid, 256, 0, s>>>(cnt, coff, clist, chunk_base, bg, fcounts, nullptr, nullptr, 0, cnt);
+    const uint32_t T = (bg.cb / 16) * (bg.cb / 16);
+    launch_scan(s, fcounts, foff, T * nch_max, &cnt->nch, T, 1, parts, nullptr, &cnt->fine_total);
+    k_fine_bin<true><<<grid, 256, 0, s>>>(cnt, coff, clist, chunk_base, bg, nullptr, foff, flist, fcap, cnt);
+}
+
 void launch_raster(hipStream_t s, FrameCounters* cnt, const PrimGeom* geom, const float* attrs,
-                   const DrawDev* draws, const uint32_t* offsets, const uint4* list, BinGeom bg, uint32_t W,
-                   uint32_t H, float clear_depth, uint32_t* out, uint32_t* msaa_color, float* msaa_depth) {
+                   const DrawDev* draws, const uint32_t* chunk_base, const uint32_t* foff, const uint4* list,
+                   uint32_t fcap, BinGeom bg, uint32_t W, uint32_t H, float clear_depth, uint32_t* out,
+                   uint32_t* msaa_color, float* msaa_depth, bool stats) {
     uint32_t nt = ((W + 15) / 16) * ((H + 15) / 16);
-    k_raster<<<nt, 256, 0, s>>>(cnt, geom, attrs, draws, offsets, list, bg, W, H, clear_depth, out, msaa_color,
-                                msaa_depth, cnt);
+    if (stats)
+        k_raster<true><<<nt, 256, 0, s>>>(cnt, geom, attrs, draws, chunk_base, foff, list, fcap, bg, W, H,
+                                          clear_depth, out, msaa_color, msaa_depth, cnt);
+    else
+        k_raster<false><<<nt, 256, 0, s>>>(cnt, geom, attrs, draws, chunk_base, foff, list, fcap, bg, W, H,
+                                           clear_depth, out, msaa_color, msaa_depth, cnt);
 }
 
 }  // namespace rzk

@@ -157,9 +157,11 @@ class Context:
         return int(self._L.rz_frame_flags(self.handle))
 
     def frame_stats(self) -> dict:
-        a = (C.c_uint64 * 5)()
+        a = (C.c_uint64 * 10)()
         self._check(self._L.rz_frame_stats(self.handle, a))
-        return dict(zip(["tris", "clipped", "prims", "bin_entries", "replays"], (int(x) for x in a)))
+        keys = ["tris", "clipped", "prims", "bin_entries", "replays", "raster_entries", "raster_candidates",
+                "raster_hits", "raster_covered", "raster_shaded"]
+        return dict(zip(keys, (int(x) for x in a)))
 
     def profile(self, enable: bool):
         self._check(self._L.rz_profile(self.handle, 1 if enable else 0))

@@ -36,3 +36,14 @@ def test_null_context_errors():
     h = ctypes.c_void_p()
     assert L.rz_create(0, 10, ctypes.byref(h)) < 0
     assert L.rz_create(20000, 10, ctypes.byref(h)) < 0
+
+
+def test_create_rejects_unsupported_screens():
+    """Screens that need more than 256 coarse bins of 256 px are refused
+    before any device call."""
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    assert L.rz_create(16384, 16384, ctypes.byref(h)) == -5
+    assert L.rz_create(8192, 8192, ctypes.byref(h)) == -5
+    assert L.rz_create(4097, 4096, ctypes.byref(h)) == -5
+    assert not h.value

@@ -64,7 +64,7 @@ def test_clear_depth_one():
 
 def test_uneven_screen_and_bins():
     sc = scenes.soup_scene(5000, 333, 217, seed=12, textured=True)
-    for cb in (32, 48, 64, 128, 336):
+    for cb in (32, 48, 64, 128, 256):
         check(sc, coarse_bin=cb)
     small = scenes.soup_scene(2000, 200, 150, seed=14)
     check(small, coarse_bin=16)
@@ -79,6 +79,8 @@ def test_too_many_bins_rejected():
             ctx.set_coarse_bin(16)  # 21 x 14 bins > 256
         with pytest.raises(RzError):
             ctx.set_coarse_bin(24)  # not a multiple of 16
+        with pytest.raises(RzError):
+            ctx.set_coarse_bin(272)  # more than 256 tiles of 16x16 per bin
     finally:
         ctx.close()
 
