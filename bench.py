@@ -35,7 +35,24 @@ def parse():
     ap.add_argument("--seed", type=int, default=1001)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample-tris", type=int, default=120_000)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 512 MiB device-to-device copy (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_probe():
+    """A copy kernel of known size: reads and writes 512 MiB once. Profiling
+    it with FETCH_SIZE / WRITE_SIZE gives the scale of those counters."""
+    import torch
+
+    n = 512 * 1024 * 1024 // 4
+    src = torch.arange(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    ok = bool((dst[:: 1 << 20] == src[:: 1 << 20]).all().item())
+    print(json.dumps({"copy_probe_bytes": n * 4, "ok": ok}), flush=True)
 
 
 def load_traffic():
@@ -97,6 +114,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a GPU"
     torch.cuda.set_device(0)
+    if a.copy_probe:
+        copy_probe()
+        return
     from rusterizer_amd import scenes
     from rusterizer_amd.render import Context
 

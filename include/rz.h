@@ -119,8 +119,9 @@ uint32_t rz_frame_flags(const rz_ctx *ctx);
 /* Counts of the last resolved frame: [input triangles, triangles clipped,
  * rasterizer triangles, coarse bin entries, frame replays, then -- only when
  * the context was created with RZ_RASTER_STATS=1 in the environment, else 0 --
- * raster list entries scanned, candidates after the exact culls, hits
- * processed, hits with coverage, hits that wrote samples]. */
+ * raster list entries scanned, candidates after the bbox / depth / edge
+ * culls, hits past the depth-plane cull, hits that ran the depth test, hits
+ * that wrote samples]. */
 int rz_frame_stats(const rz_ctx *ctx, uint64_t out[10]);
 
 /* ------------------------------------------------------------ tuning/timing */

@@ -52,20 +52,21 @@ struct DrawDev {
     uint32_t fs;
 };
 
-// One rasterizer triangle (RasterizerTriangle, rasterizer/mod.rs:177-184),
-// 96 bytes, read with scalar loads by the raster kernel.
+// One rasterizer triangle (RasterizerTriangle, rasterizer/mod.rs:177-184)
+// plus exact-cull helpers, 112 bytes, read with scalar loads by the raster.
 struct PrimGeom {
     float ax0, ay0, ax1, ay1, ax2, ay2;  // screen points
     float nx0, ny0, nx1, ny1, nx2, ny2;  // edge normals
     float inv_area;                      // 1 / (2 * area)
     float z0, z1, z2;                    // screen-space depth
     float w0, w1, w2;                    // clip-space w
-    uint32_t draw;
+    uint32_t flags;                      // tie-rule bit per edge (0..2) | fs << 8 | draw << 16
     uint32_t bbx, bby;                   // clamped pixel bbox: lo | hi << 16
-    uint32_t flags;                      // tie-rule bit per edge (0..2) | fs << 8
     float zlo;                           // lower bound of every interpolated depth
+    float pz0, pzx, pzy;                 // depth lower-bound plane: pz0 + pzx*(x-bx0) + pzy*(y-by0)
+    uint32_t pad0, pad1;
 };
-static_assert(sizeof(PrimGeom) == 96, "PrimGeom layout");
+static_assert(sizeof(PrimGeom) == 112, "PrimGeom layout");
 
 // Per-prim binning record and coarse-bin list entry: {prim, bbx, bby, zlo bits}.
 // (uint4; for the per-prim record .x is unused.)

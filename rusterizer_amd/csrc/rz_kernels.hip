@@ -432,6 +432,42 @@ __device__ __forceinline__ bool tie_rule(float nx, float ny) {
     return (nx > 0.0f) || (!(nx < 0.0f) && (ny < 0.0f));  // mod.rs:159-168
 }
 
+// A plane that is <= the depth the reference interpolates (mod.rs:226-236)
+// at every sample the prim can cover (samples inside its pixel bbox):
+//   exact Z(p) = z2 + B0(p)(z0-z2) + B1(p)(z1-z2), B0 = inv*E1(p), B1 = inv*E2(p),
+// computed in f64, minus a slack that bounds the f32 rounding of the edge
+// functions (|dE| <= 4u*M), of e*inv, of the clamps (covered samples have
+// barycentrics within that error of [0,1]), of b2 = (1-b0)-b1 and of the
+// final three products and two sums, and the f32 evaluation of the plane
+// itself -- all with a further 2x margin. u = 2^-24.
+__device__ void depth_plane_bound(PrimGeom& o, uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1) {
+    const double u = 0x1p-24;
+    const double inv = o.inv_area;
+    const double z0 = o.z0, z1 = o.z1, z2 = o.z2;
+    const double dz0 = z0 - z2, dz1 = z1 - z2;
+    const double n1x = o.nx1, n1y = o.ny1, n2x = o.nx2, n2y = o.ny2;
+    const double a1x = o.ax1, a1y = o.ay1, a2x = o.ax2, a2y = o.ay2;
+    const double bx = x0, by = y0, bw = (double)x1 - x0, bh = (double)y1 - y0;
+    const double px = inv * (n1x * dz0 + n2x * dz1);
+    const double py = inv * (n1y * dz0 + n2y * dz1);
+    const double e1 = n1x * (bx - a1x) + n1y * (by - a1y);
+    const double e2 = n2x * (bx - a2x) + n2y * (by - a2y);
+    const double zb = z2 + inv * e1 * dz0 + inv * e2 * dz1;
+    const double m1 = fabs(n1x) * fmax(fabs(bx - a1x), fabs(bx + bw - a1x)) +
+                      fabs(n1y) * fmax(fabs(by - a1y), fabs(by + bh - a1y));
+    const double m2 = fabs(n2x) * fmax(fabs(bx - a2x), fabs(bx + bw - a2x)) +
+                      fabs(n2y) * fmax(fabs(by - a2y), fabs(by + bh - a2y));
+    const double d = 10.0 * u * fmax(m1, m2) * fabs(inv) + 2.0 * u;
+    const double slack = (3.0 * d + 16.0 * u) * (fabs(z0) + fabs(z1) + fabs(z2));
+    const double ev = 8.0 * u * (fabs(zb) + fabs(px) * bw + fabs(py) * bh) + 1e-30;
+    const double lo = zb - 2.0 * (slack + ev);
+    const bool ok = (x0 < x1) && (y0 < y1) && isfinite(lo) && isfinite(px) && isfinite(py) &&
+                    isfinite(zb) && fabs(px) < 1e30 && fabs(py) < 1e30;
+    o.pz0 = ok ? (float)lo : -FLT_MAX_F;
+    o.pzx = ok ? (float)px : 0.0f;
+    o.pzy = ok ? (float)py : 0.0f;
+}
+
 __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t H, uint32_t draw,
                                            uint32_t fs, PrimGeom* __restrict__ g, float* __restrict__ attr,
                                            uint4* __restrict__ pbin, BinGeom bg, uint32_t& pairs) {
@@ -457,7 +493,6 @@ __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t
     o.inv_area = 1.0f / (v10x * v20y - v20x * v10y);
     o.z0 = sz[0]; o.z1 = sz[1]; o.z2 = sz[2];
     o.w0 = w[0]; o.w1 = w[1]; o.w2 = w[2];
-    o.draw = draw;
     // bounding_box.rs:15-32: fold from (MAX, MIN); floor/ceil; saturating cast
     float a0 = FLT_MAX_F, a1 = -FLT_MAX_F, a2 = FLT_MAX_F, a3 = -FLT_MAX_F;
     for (int k = 0; k < 3; ++k) {
@@ -473,12 +508,15 @@ __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t
     o.bbx = x0 | (x1 << 16);
     o.bby = y0 | (y1 << 16);
     o.flags = (tie_rule(o.nx0, o.ny0) ? 1u : 0u) | (tie_rule(o.nx1, o.ny1) ? 2u : 0u) |
-              (tie_rule(o.nx2, o.ny2) ? 4u : 0u) | (fs << 8);
+              (tie_rule(o.nx2, o.ny2) ? 4u : 0u) | (fs << 8) | (draw << 16);
     // Depth of a covered sample is ((b0*z0 + b1*z1) + b2*z2) with clamped
     // b_i in [0, 1] and 1 - 2^-23 <= b0+b1+b2 <= 2; three roundings later it
     // is >= zlo below, so a sample can pass LESS only where zbuf > zlo.
     float zm = fminf(fminf(sz[0], sz[1]), sz[2]);
     o.zlo = zm >= 0.0f ? zm * (1.0f - 0x1p-18f) - 0x1p-126f : zm * (2.0f + 0x1p-17f) - 0x1p-126f;
+    depth_plane_bound(o, x0, x1, y0, y1);
+    o.pad0 = 0;
+    o.pad1 = 0;
     *g = o;
     *pbin = make_uint4(0u, o.bbx, o.bby, __float_as_uint(o.zlo));
     if (x0 < x1 && y0 < y1) {
@@ -895,6 +933,8 @@ __global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict_
     // qzmax: largest depth stored in the quadrant's on-screen samples
     float qzmax = __int_as_float(
         __builtin_amdgcn_readfirstlane(__float_as_int(wave_max(onscreen ? clear_depth : -FLT_MAX_F))));
+    // lane_zbmax: largest depth stored in this pixel's 4 samples
+    float lane_zbmax = onscreen ? clear_depth : -FLT_MAX_F;
 
     // this tile's fine list (see k_fine_bin for the layout)
     const uint32_t nb = bg.nbx * bg.nby;
@@ -943,8 +983,15 @@ __global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict_
                 nxt = geom[nprim];
             }
             bool wrote = false;
-            if (g.zlo < qzmax) {
-                const uint32_t bx0 = g.bbx & 0xFFFFu, bx1 = g.bbx >> 16, by0 = g.bby & 0xFFFFu, by1 = g.bby >> 16;
+            const uint32_t bx0 = g.bbx & 0xFFFFu, bx1 = g.bbx >> 16, by0 = g.bby & 0xFFFFu, by1 = g.bby >> 16;
+            // depth lower bound over the quadrant's samples inside the bbox
+            const float fbx0 = (float)bx0, fby0 = (float)by0;
+            const float xl = fmaxf(qxl, fbx0 + 0.125f), xh = fminf(qxh, (float)bx1 - 0.125f);
+            const float yl = fmaxf(qyl, fby0 + 0.125f), yh = fminf(qyh, (float)by1 - 0.125f);
+            const float zq = g.pz0 + g.pzx * ((g.pzx >= 0.0f ? xl : xh) - fbx0) +
+                             g.pzy * ((g.pzy >= 0.0f ? yl : yh) - fby0);
+            const float zlb = fmaxf(zq, g.zlo);
+            if (!(zq >= qzmax) && !(g.zlo >= qzmax)) {
                 const bool inb = (px >= bx0) & (px < bx1) & (py >= by0) & (py < by1);
                 // mod.rs:134-146 + 148-170: coverage of the 4 samples
                 const bool t0 = g.flags & 1u, t1 = g.flags & 2u, t2 = g.flags & 4u;
@@ -964,72 +1011,77 @@ __global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict_
                     e2s[s] = e2;
                 }
                 if (!inb) cov = 0;
+                // a covered lane can pass LESS only if some stored depth exceeds the
+                // bound; when no lane can, the wave skips the depth math
+                if (zlb >= lane_zbmax) cov = 0;
                 if (STATS) {
                     st_hits += 1;
                     st_cov += __ballot(cov != 0) ? 1u : 0u;
                 }
-                // mod.rs:225-253: linear depth at covered samples; mod.rs:363-378: strict LESS
-                float sd[4];
-                uint32_t pass = 0;
-                const float zbuf[4] = {zb0, zb1, zb2, zb3};
+                if (__ballot(cov != 0) != 0) {
+                    // mod.rs:225-253: linear depth at covered samples; mod.rs:363-378: strict LESS
+                    float sd[4];
+                    uint32_t pass = 0;
+                    const float zbuf[4] = {zb0, zb1, zb2, zb3};
 #pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const float b0 = clamp_bary(e1s[s] * g.inv_area);
-                    const float b1 = clamp_bary(e2s[s] * g.inv_area);
-                    const float b2 = clamp_bary(1.0f - b0 - b1);
-                    const float z = b0 * g.z0 + b1 * g.z1 + b2 * g.z2;
-                    const bool c = (cov >> s) & 1u;
-                    sd[s] = c ? z : 0.0f;
-                    pass |= (c & (z < zbuf[s])) ? (1u << s) : 0u;
-                }
-                if (pass != 0) {
-                    // mod.rs:70-99: perspective-correct barycentrics at the pixel
-                    // centre or at the first depth-passing sample
-                    float ix = fx + 0.5f, iy = fy + 0.5f;
-                    if (pass != 0xFu) {
-                        const int s = __builtin_ctz(pass);
-                        ix = fx + sample_x(s);
-                        iy = fy + sample_y(s);
+                    for (int s = 0; s < 4; ++s) {
+                        const float b0 = clamp_bary(e1s[s] * g.inv_area);
+                        const float b1 = clamp_bary(e2s[s] * g.inv_area);
+                        const float b2 = clamp_bary(1.0f - b0 - b1);
+                        const float z = b0 * g.z0 + b1 * g.z1 + b2 * g.z2;
+                        const bool c = (cov >> s) & 1u;
+                        sd[s] = c ? z : 0.0f;
+                        pass |= (c & (z < zbuf[s])) ? (1u << s) : 0u;
                     }
-                    const float ef0 = edge(g.nx0, g.ny0, g.ax0, g.ay0, ix, iy);
-                    const float ef1 = edge(g.nx1, g.ny1, g.ax1, g.ay1, ix, iy);
-                    const float ef2 = edge(g.nx2, g.ny2, g.ax2, g.ay2, ix, iy);
-                    const float f_u = ef1 / g.w0;
-                    const float f_v = ef2 / g.w1;
-                    const float f_w = ef0 / g.w2;
-                    const float sum = f_u + f_v + f_w;
-                    const float u = clamp_bary(f_u / sum);
-                    const float v = clamp_bary(f_v / sum);
-                    const float w = clamp_bary(1.0f - u - v);
-                    // main.rs:67-77 fragment shaders; Debug is Color::grayscale(depths[0])
-                    const uint32_t fs = g.flags >> 8;
-                    const float* at = attrs + (size_t)prim * ATTR_STRIDE;
-                    float cr = sd[0], cg = sd[0], cbl = sd[0], ca = 1.0f;
-                    if (fs == RZ_FS_COLOR) {
-                        cr = at[0] * u + at[4] * v + at[8] * w;
-                        cg = at[1] * u + at[5] * v + at[9] * w;
-                        cbl = at[2] * u + at[6] * v + at[10] * w;
-                        ca = at[3] * u + at[7] * v + at[11] * w;
-                    } else if (fs == RZ_FS_TEXTURE) {
-                        const DrawDev& dd = draws[g.draw];
-                        const float tu = at[0] * u + at[2] * v + at[4] * w;
-                        const float tv = at[1] * u + at[3] * v + at[5] * w;
-                        const RGBA c = texture_sample(dd.tex, dd.tw, dd.th, tu, tv, lut, oob);
-                        cr = c.r; cg = c.g; cbl = c.b; ca = c.a;
+                    if (pass != 0) {
+                        // mod.rs:70-99: perspective-correct barycentrics at the pixel
+                        // centre or at the first depth-passing sample
+                        float ix = fx + 0.5f, iy = fy + 0.5f;
+                        if (pass != 0xFu) {
+                            const int s = __builtin_ctz(pass);
+                            ix = fx + sample_x(s);
+                            iy = fy + sample_y(s);
+                        }
+                        const float ef0 = edge(g.nx0, g.ny0, g.ax0, g.ay0, ix, iy);
+                        const float ef1 = edge(g.nx1, g.ny1, g.ax1, g.ay1, ix, iy);
+                        const float ef2 = edge(g.nx2, g.ny2, g.ax2, g.ay2, ix, iy);
+                        const float f_u = ef1 / g.w0;
+                        const float f_v = ef2 / g.w1;
+                        const float f_w = ef0 / g.w2;
+                        const float sum = f_u + f_v + f_w;
+                        const float u = clamp_bary(f_u / sum);
+                        const float v = clamp_bary(f_v / sum);
+                        const float w = clamp_bary(1.0f - u - v);
+                        // main.rs:67-77 fragment shaders; Debug is Color::grayscale(depths[0])
+                        const uint32_t fs = (g.flags >> 8) & 3u;
+                        const float* at = attrs + (size_t)prim * ATTR_STRIDE;
+                        float cr = sd[0], cg = sd[0], cbl = sd[0], ca = 1.0f;
+                        if (fs == RZ_FS_COLOR) {
+                            cr = at[0] * u + at[4] * v + at[8] * w;
+                            cg = at[1] * u + at[5] * v + at[9] * w;
+                            cbl = at[2] * u + at[6] * v + at[10] * w;
+                            ca = at[3] * u + at[7] * v + at[11] * w;
+                        } else if (fs == RZ_FS_TEXTURE) {
+                            const DrawDev& dd = draws[g.flags >> 16];
+                            const float tu = at[0] * u + at[2] * v + at[4] * w;
+                            const float tv = at[1] * u + at[3] * v + at[5] * w;
+                            const RGBA c = texture_sample(dd.tex, dd.tw, dd.th, tu, tv, lut, oob);
+                            cr = c.r; cg = c.g; cbl = c.b; ca = c.a;
+                        }
+                        // mod.rs:380-397
+                        const uint32_t argb = to_argb(cr, cg, cbl, ca);
+                        if (pass & 1u) { cb0 = argb; zb0 = sd[0]; }
+                        if (pass & 2u) { cb1 = argb; zb1 = sd[1]; }
+                        if (pass & 4u) { cb2 = argb; zb2 = sd[2]; }
+                        if (pass & 8u) { cb3 = argb; zb3 = sd[3]; }
+                        lane_zbmax = fmaxf(fmaxf(zb0, zb1), fmaxf(zb2, zb3));
+                        wrote = true;
                     }
-                    // mod.rs:380-397
-                    const uint32_t argb = to_argb(cr, cg, cbl, ca);
-                    if (pass & 1u) { cb0 = argb; zb0 = sd[0]; }
-                    if (pass & 2u) { cb1 = argb; zb1 = sd[1]; }
-                    if (pass & 4u) { cb2 = argb; zb2 = sd[2]; }
-                    if (pass & 8u) { cb3 = argb; zb3 = sd[3]; }
-                    wrote = true;
                 }
             }
             if (STATS) st_shaded += __ballot(wrote) ? 1u : 0u;
             if (__ballot(wrote)) {
-                const float lm = onscreen ? fmaxf(fmaxf(zb0, zb1), fmaxf(zb2, zb3)) : -FLT_MAX_F;
-                qzmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(wave_max(lm))));
+                qzmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(wave_max(lane_zbmax))));
             }
             if (m == 0) break;
         }
