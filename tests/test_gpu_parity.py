@@ -178,3 +178,15 @@ def test_consecutive_frames_reset():
     fresh = run_gpu(s2)
     for k in ("img", "col", "dep"):
         assert np.array_equal(second[k], fresh[k])
+
+
+@pytest.mark.parametrize("dz", [0.0, -1e-6, 1e-6, -1e-4])
+def test_zfight_redraw(dz):
+    """A soup drawn twice into one frame, the second copy shifted in depth by
+    dz (0: identical depths, which strict LESS must reject everywhere). The
+    depth culls work closest to their margins here."""
+    a = scenes.soup_scene(20_000, 640, 480, seed=41)
+    b = scenes.soup_scene(20_000, 640, 480, seed=41)
+    b.pos = (b.pos + np.array([0.0, 0.0, dz], np.float32)).astype(np.float32)
+    b.color = np.ascontiguousarray(b.color[:, ::-1])
+    check([a, b])

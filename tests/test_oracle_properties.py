@@ -1,0 +1,299 @@
+"""Pins for the oracle's functions that the reference's own unit tests leave
+open (test infrastructure checks; CPU only).
+
+- texture sampling (texture.rs:47-83): texel centres, quarter weights, the
+  reference's linear-index row wrap, and the out-of-range flag;
+- to_argb (color.rs:15-20): truncation, saturation, overflow into alpha;
+- clip attribute interpolation (clipping.rs:149,161): dyadic alpha, exact;
+- perspective-correct interpolation with unequal w (mod.rs:70-99): a
+  world-affine attribute equals its value at the ray/plane hit point;
+- whole frames: an independent numpy model of coverage, strict-LESS depth,
+  order, shading point and resolve on tiny scenes, bit for bit;
+- the tile-parallel oracle equals the serial one bit for bit.
+"""
+import numpy as np
+import pytest
+
+from rusterizer_amd import hostmath as hm
+from rusterizer_amd import meshes, scenes
+
+F = np.float32
+
+
+# ---------------------------------------------------------------- texture
+def _tex(w, h, seed=0):
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+
+
+def test_texture_texel_centres(oracle):
+    """With tw-1 and th-1 powers of two, u = i/(tw-1) lands exactly on texel
+    i: the sample is that texel (catches transposed axes, row stride and
+    channel order)."""
+    tex = _tex(257, 129, 1)
+    for (i, j) in [(0, 0), (256, 128), (17, 3), (255, 127), (100, 64)]:
+        out, fl = oracle.texture_sample(tex, F(i) / F(256), F(j) / F(128))
+        assert fl == 0
+        exp = tex[j, i].astype(np.float32) / F(255)
+        assert np.array_equal(out, exp), (i, j, out, exp)
+
+
+def test_texture_quarter_weights(oracle):
+    """x_f = 0.25, y_f = 0.5 on a 0/255 pattern: exact lerps distinguish the
+    (1-f, f) weights from their swap."""
+    tex = np.zeros((3, 5, 4), np.uint8)
+    tex[:, :, 3] = 255
+    tex[1, 3, 0] = 255          # right column of the cell, top row
+    tex[2, 2, 1] = 255          # left column, bottom row
+    u = (F(2) + F(0.25)) / F(4)  # x = 2.25 -> texels 2, 3
+    v = (F(1) + F(0.5)) / F(2)   # y = 1.5  -> rows 1, 2
+    out, _ = oracle.texture_sample(tex, u, v)
+    # r: only (3, 1) -> weight x_f * (1 - y_f) = 0.25 * 0.5
+    assert out[0] == F(0.125)
+    # g: only (2, 2) -> weight (1 - x_f) * y_f = 0.75 * 0.5
+    assert out[1] == F(0.375)
+    assert out[3] == F(1.0)
+
+
+def test_texture_row_wrap_and_flag(oracle):
+    """texel index x + y*w: x == w addresses the next row's first texel
+    (texture.rs:51); past the last texel is flagged."""
+    tex = np.zeros((4, 4, 4), np.uint8)
+    tex[2, 0] = (255, 0, 0, 255)  # row 2, column 0
+    # u slightly above 1 -> ceil(x) = 4 == width -> wraps to (0, y+1)
+    u = F(1.0) + F(2 ** -22)
+    v = F(1) / F(3)               # y = 1 exactly -> rows 1 and 1
+    out, fl = oracle.texture_sample(tex, u, v)
+    assert fl == 0
+    x = F(u * F(3))
+    xf = F(x - np.trunc(x))
+    assert out[0] == F(F(1 - xf) * F(0) + F(1.0) * xf)  # topright came from (0, 2)
+    out, fl = oracle.texture_sample(tex, u, F(1.0))   # last row: past the end
+    assert fl & oracle.FLAG_TEXEL_OOB
+
+
+# ---------------------------------------------------------------- to_argb
+@pytest.mark.parametrize("rgba,exp", [
+    ((0.5, 0.25, 0.1, 1.0), 0xFF7F3F19),      # 127.5, 63.75, 25.5 truncate
+    ((0.0, 0.0, 0.0, 0.0), 0x00000000),
+    ((-0.5, 2.0, float("nan"), 1.0), 0xFF000000 | (510 << 8)),  # negative/NaN -> 0; 510 spills into red
+    ((1.0, 1.0, 1.0, 0.5), 0x7FFFFFFF),
+])
+def test_to_argb_closed_form(oracle, rgba, exp):
+    assert oracle.to_argb(rgba) == exp
+
+
+# ---------------------------------------------------------------- clipping
+def test_clip_attribute_lerp_exact(oracle):
+    """prev inside (d = 0.75), cur outside (d = -0.25) of the right plane:
+    alpha = 0.75 exactly, the new vertex is x = 2.0 and its attribute
+    (A_cur - A_prev) * alpha + A_prev = 0.75 (0.25 if prev/cur were swapped)."""
+    verts = [[2.25, 0.0, 0.0, 2.0], [1.25, 0.0, 0.0, 2.0], [1.25, 1.0, 0.0, 2.0]]
+    attrs = np.zeros((3, 6), np.float32)
+    attrs[0] = 1.0  # the outside vertex
+    r, tris, tattr = oracle.try_clip(verts, attrs)
+    assert r == 2
+    got = {(float(v[0]), float(v[1])): a for t, at in zip(tris, tattr) for v, a in zip(t, at)}
+    assert (2.0, 0.0) in got
+    assert np.all(got[(2.0, 0.0)] == F(0.75))
+    # the second intersection on edge (v2 -> v0): prev v2 inside (d=0.75), cur v0 outside
+    # (d=-0.25): alpha 0.75 again, point (2.0, 0.25)
+    assert (2.0, 0.25) in got
+    assert np.all(got[(2.0, 0.25)] == F(0.75))
+
+
+# ------------------------------------------- perspective-correct interpolation
+def test_perspective_correct_world_affine(oracle):
+    """Attributes equal to the vertices' world x and z: at a pixel centre the
+    interpolated values must match the world point the pixel's ray hits on
+    the triangle's plane (f64), to f32 accuracy -- linear (non-perspective)
+    interpolation misses by ~1e-1 here."""
+    W = H = 256
+    proj = hm.project(0.5, 100.0, F(1.0), hm.FRAC_PI_2)
+    view = hm.Camera(pos=(0.0, 0.0, 0.0), dir=(0.0, 0.0, 1.0)).get_view_matrix()
+    P = np.array([[-2.0, -1.5, 1.2], [0.5, 2.0, 9.0], [2.5, -1.0, 3.0]], np.float32)  # unequal depths
+    M = hm.mat_mul(proj, view)
+    clip = np.array([oracle.mat_vec(M, list(p) + [1.0]) for p in P], np.float32)
+    ndc = oracle.perspective_divide(clip)
+    s = oracle.viewport_setup(W, H, ndc)
+    # orient clockwise like the rasterizer expects
+    if s.inv_2x_area < 0:
+        P = P[[0, 2, 1]]
+        clip = clip[[0, 2, 1]]
+        ndc = oracle.perspective_divide(clip)
+        s = oracle.viewport_setup(W, H, ndc)
+    attrs = np.zeros((3, 6), np.float32)
+    attrs[:, 4] = P[:, 0] / 4 + 0.5     # uv.x = world x
+    attrs[:, 5] = P[:, 2] / 10          # uv.y = world z
+    Md = M.astype(np.float64)
+    inv = np.linalg.inv(Md)
+    n = np.cross(P[1] - P[0], P[2] - P[0]).astype(np.float64)
+    checked = 0
+    for y in range(0, H, 7):
+        for x in range(0, W, 7):
+            m, _ = oracle.coverage(s, x, y)
+            if m != 0b1111:
+                continue
+            # ray through the pixel centre
+            ndc_x = (x + 0.5) / W * 2 - 1
+            ndc_y = 1 - (y + 0.5) / H * 2
+            a = inv @ np.array([ndc_x, ndc_y, -1.0, 1.0])
+            b = inv @ np.array([ndc_x, ndc_y, 1.0, 1.0])
+            a, b = a[:3] / a[3], b[:3] / b[3]
+            t = np.dot(n, P[0] - a) / np.dot(n, b - a)
+            hit = a + t * (b - a)
+            out = oracle.interpolate(s, attrs, x, y, m)
+            assert abs(out[4] - (hit[0] / 4 + 0.5)) < 2e-4, (x, y, out[4], hit)
+            assert abs(out[5] - hit[2] / 10) < 2e-4, (x, y, out[5], hit)
+            checked += 1
+    assert checked > 100
+
+
+# ---------------------------------------------------------------- whole frames
+RG = np.array([[5 / 8, 1 / 8], [7 / 8, 5 / 8], [3 / 8, 7 / 8], [1 / 8, 3 / 8]], np.float32)
+
+
+def numpy_frame(tris_ndc, cols, W, H):
+    """Independent model of the raster for triangles already inside the NDC
+    cube: per triangle (in order) per pixel, 4-sample coverage with the tie
+    rule, strict-LESS depth, shading point = pixel centre if all 4 samples
+    pass else the first passing one, colour FS, box-filter resolve."""
+    cbuf = np.full((H, W, 4), 0xFF191919, np.uint32)
+    zbuf = np.full((H, W, 4), np.finfo(np.float32).max, np.float32)
+    ys, xs = np.mgrid[0:H, 0:W]
+    xs, ys = xs.astype(np.float32), ys.astype(np.float32)
+    for tri, col in zip(tris_ndc, cols):
+        sp = np.array([[F(F(W) * F(v[0] + F(1)) / F(2)), F(F(H) * F(F(1) - F(F(v[1] + F(1)) / F(2)))),
+                        F(F(F(v[2] + F(1)) * F(0.5)) * F(1)) + F(0)] for v in tri], np.float32)
+        w = np.array([v[3] for v in tri], np.float32)
+        nrm = [(F(-(sp[(k + 1) % 3, 1] - sp[k, 1])), F(sp[(k + 1) % 3, 0] - sp[k, 0])) for k in range(3)]
+        tie = [(nx > 0) or (not (nx < 0) and ny < 0) for nx, ny in nrm]
+        area = F(F(F(sp[1, 0] - sp[0, 0]) * F(sp[2, 1] - sp[0, 1])) - F(F(sp[2, 0] - sp[0, 0]) * F(sp[1, 1] - sp[0, 1])))
+        inv = F(F(1) / area)
+
+        def edges(px, py):
+            return [((F(0) + (nrm[k][0] * (px - sp[k, 0]).astype(np.float32)).astype(np.float32)).astype(np.float32)
+                     + (nrm[k][1] * (py - sp[k, 1]).astype(np.float32)).astype(np.float32)).astype(np.float32)
+                    for k in range(3)]
+
+        x0 = max(int(np.floor(sp[:, 0].min())), 0)
+        x1 = min(int(np.ceil(sp[:, 0].max())), W)
+        y0 = max(int(np.floor(sp[:, 1].min())), 0)
+        y1 = min(int(np.ceil(sp[:, 1].max())), H)
+        inb = (xs >= x0) & (xs < x1) & (ys >= y0) & (ys < y1)
+        cov = np.zeros((H, W, 4), bool)
+        dep = np.zeros((H, W, 4), np.float32)
+        for s in range(4):
+            e = edges((xs + RG[s, 0]).astype(np.float32), (ys + RG[s, 1]).astype(np.float32))
+            c = inb.copy()
+            for k in range(3):
+                c &= (e[k] > 0) | (~(e[k] < 0) & tie[k])
+            cov[..., s] = c
+            b0 = np.clip((e[1] * inv).astype(np.float32), 0, 1).astype(np.float32)
+            b1 = np.clip((e[2] * inv).astype(np.float32), 0, 1).astype(np.float32)
+            b2 = np.clip(((F(1) - b0).astype(np.float32) - b1).astype(np.float32), 0, 1).astype(np.float32)
+            z = ((b0 * sp[0, 2]).astype(np.float32) + (b1 * sp[1, 2]).astype(np.float32)).astype(np.float32)
+            z = (z + (b2 * sp[2, 2]).astype(np.float32)).astype(np.float32)
+            dep[..., s] = np.where(c, z, F(0))
+        passm = cov & (dep < zbuf)
+        anyp = passm.any(-1)
+        allp = passm.all(-1)
+        first = np.argmax(passm, -1)
+        ix = np.where(allp, (xs + F(0.5)).astype(np.float32), (xs + RG[first, 0]).astype(np.float32))
+        iy = np.where(allp, (ys + F(0.5)).astype(np.float32), (ys + RG[first, 1]).astype(np.float32))
+        e = edges(ix, iy)
+        fu = (e[1] / w[0]).astype(np.float32)
+        fv = (e[2] / w[1]).astype(np.float32)
+        fw = (e[0] / w[2]).astype(np.float32)
+        ssum = ((fu + fv).astype(np.float32) + fw).astype(np.float32)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            u = (fu / ssum).astype(np.float32)
+            v = (fv / ssum).astype(np.float32)
+        u = np.where(u < 0, F(0), np.where(u > 1, F(1), u)).astype(np.float32)
+        v = np.where(v < 0, F(0), np.where(v > 1, F(1), v)).astype(np.float32)
+        ww = ((F(1) - u).astype(np.float32) - v).astype(np.float32)
+        ww = np.where(ww < 0, F(0), np.where(ww > 1, F(1), ww)).astype(np.float32)
+        argb = np.zeros((H, W), np.uint32)
+        for ch, sh in ((3, 24), (0, 16), (1, 8), (2, 0)):
+            val = ((col[0][ch] * u).astype(np.float32) + (col[1][ch] * v).astype(np.float32)).astype(np.float32)
+            val = (val + (col[2][ch] * ww).astype(np.float32)).astype(np.float32)
+            q = (val * F(255)).astype(np.float32)
+            qi = np.where(np.isnan(q) | (q <= 0), 0, np.minimum(q, 4294967295.0)).astype(np.uint64).astype(np.uint32)
+            argb |= (qi << np.uint32(sh)).astype(np.uint32)
+        upd = passm & anyp[..., None]
+        cbuf = np.where(upd, argb[..., None], cbuf)
+        zbuf = np.where(upd, dep, zbuf)
+    r = sum((cbuf[..., s] >> 16) & 0xFF for s in range(4)) // 4
+    g = sum((cbuf[..., s] >> 8) & 0xFF for s in range(4)) // 4
+    b = sum(cbuf[..., s] & 0xFF for s in range(4)) // 4
+    img = (0xFF000000 | (r << 16) | (g << 8) | b).astype(np.uint32)
+    return img, cbuf, zbuf
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_tiny_frames_match_numpy_model(oracle, seed):
+    rng = np.random.default_rng(100 + seed)
+    W, H = 24, 16
+    n = 6
+    # triangles inside the NDC cube (no clipping), clip-space w = 1..3
+    ndc = rng.uniform(-0.95, 0.95, (n, 3, 3)).astype(np.float32)
+    ndc[:, :, 2] = rng.uniform(-0.9, 0.9, (n, 3)).astype(np.float32)
+    if seed % 2 == 0:  # coplanar duplicates exercise the strict LESS tie
+        ndc[1] = ndc[0]
+    # identity matrices: positions are NDC with w = 1 (unequal w is pinned by
+    # test_perspective_correct_world_affine)
+    cols = rng.random((n, 3, 4), dtype=np.float32)
+    r = oracle.OracleRenderer(W, H)
+    pos = ndc.reshape(-1, 3)
+    r.set_uniforms(hm.identity(), hm.identity(), hm.identity())
+    r.render(pos, cols.reshape(-1, 4), None, np.arange(3 * n), oracle.FS_COLOR)
+    col_o, dep_o = r.read_msaa()
+    img_o = r.framebuffer()
+    tris = np.concatenate([ndc, np.ones((n, 3, 1), np.float32)], -1)
+    img_m, col_m, dep_m = numpy_frame(tris, cols, W, H)
+    assert np.array_equal(dep_o.view(np.uint32), dep_m.view(np.uint32))
+    assert np.array_equal(col_o, col_m)
+    assert np.array_equal(img_o, img_m)
+    assert (img_o != 0xFF191919).mean() > 0.1  # the scene draws something
+
+
+def test_hand_checked_depth_order(oracle):
+    """Two full-screen-ish triangles: near first then far -> far is hidden;
+    equal depth -> the first drawn wins (strict LESS)."""
+    W, H = 8, 8
+    big = np.array([[-3, -3, 0], [0, 3, 0], [3, -3, 0]], np.float32)  # clockwise in screen
+    for z_first, z_second, winner in [(-0.5, 0.5, 0), (0.5, -0.5, 1), (0.2, 0.2, 0)]:
+        pos = np.concatenate([big + [0, 0, z_first], big + [0, 0, z_second]]).astype(np.float32)
+        cols = np.array([[1, 0, 0, 1]] * 3 + [[0, 0, 1, 1]] * 3, np.float32)
+        r = oracle.OracleRenderer(W, H)
+        r.set_uniforms(hm.identity(), hm.identity(), hm.identity())
+        r.render(pos, cols, None, np.arange(6), oracle.FS_COLOR)
+        img = r.framebuffer()
+        exp = 0xFFFF0000 if winner == 0 else 0xFF0000FF
+        assert (img[2:6, 2:6] == exp).all(), (z_first, z_second, hex(int(img[4, 4])))
+
+
+def test_tile_parallel_oracle_equals_serial(oracle):
+    sc = scenes.soup_scene(4000, 300, 200, seed=9, textured=True)
+    outs = []
+    for th in (1, 5):
+        r = oracle.OracleRenderer(sc.width, sc.height)
+        r.set_uniforms(sc.world, sc.view, sc.proj)
+        r.bind_texture(0, sc.texture)
+        r.render(sc.pos, sc.color, sc.uv, sc.idx, sc.fs, threads=th)
+        c, d = r.read_msaa()
+        outs.append((c, d, r.framebuffer()))
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+
+
+def test_demo_texture_matches_reference_png_digest():
+    """images/checkerboard.png decodes to the closed form of meshes.demo_checkerboard
+    (digest recorded by tests/golden/make_golden.py)."""
+    import hashlib
+    import os
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    want = open(os.path.join(here, "golden", "checkerboard_png_rgba_sha256.txt")).read().split()[0]
+    got = hashlib.sha256(meshes.demo_checkerboard().tobytes()).hexdigest()
+    assert got == want
