@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--seed", type=int, default=1001)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-sample-tris", type=int, default=120_000)
+    ap.add_argument("--cpu-sample-tris", type=int, default=300_000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy (HBM counter calibration)")
     return ap.parse_args()
@@ -67,8 +67,9 @@ def load_traffic():
 
 def cpu_baseline(sc, n_sample: int):
     """The oracle (single-threaded C++ restatement of the reference) on the
-    first n_sample triangles of the same soup; the frame time is scaled by
-    n_tris / n_sample (triangles are i.i.d., so the sample is representative)."""
+    first n_sample triangles of the same soup (about 10 s of CPU work); the
+    frame time is scaled by n_tris / n_sample (triangles are i.i.d., so the
+    sample is representative)."""
     from oracle.oracle import OracleRenderer
 
     n_sample = min(n_sample, sc.ntri)
@@ -131,8 +132,10 @@ def main():
         ctx.draw(mesh, sc.fs)
         ctx.resolve_device(optr)
 
+    warm_stats = []
     for _ in range(a.warmup):
         frame()
+        warm_stats.append(ctx.frame_stats())
     torch.cuda.synchronize()
     ctx.profile(True)
     lat = []
@@ -179,6 +182,7 @@ def main():
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
         "stage_ms": {k: prof[k] / frames for k in ("geometry", "binning", "raster")},
         "frame_stats": stats,
+        "warmup_replays": [(w["replays"], w["overflow_bits"]) for w in warm_stats],
         "roofline": {
             "bound": "hbm",
             "kernel": "k_raster",

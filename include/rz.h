@@ -121,8 +121,9 @@ uint32_t rz_frame_flags(const rz_ctx *ctx);
  * the context was created with RZ_RASTER_STATS=1 in the environment, else 0 --
  * raster list entries scanned, candidates after the bbox / depth / edge
  * culls, hits past the depth-plane cull, hits that ran the depth test, hits
- * that wrote samples]. */
-int rz_frame_stats(const rz_ctx *ctx, uint64_t out[10]);
+ * that wrote samples, then the buffer-overflow reasons that caused replays
+ * (bits: 1 triangles, 2 clip slots, 4 coarse entries, 16 fine entries)]. */
+int rz_frame_stats(const rz_ctx *ctx, uint64_t out[11]);
 
 /* ------------------------------------------------------------ tuning/timing */
 

@@ -96,7 +96,7 @@ struct rz_ctx {
     DBuf<float> msaa_depth;
 
     uint32_t frame_flags = 0;
-    uint64_t frame_stats[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t frame_stats[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     bool raster_stats = false;  // RZ_RASTER_STATS=1 at rz_create
     uint64_t frame_tris = 0;
 
@@ -179,7 +179,7 @@ uint32_t fine_matrix_len(const rz_ctx* c) {
 // capacities for the frame's rasterizer triangles, clip slots and bin entries
 int ensure_frame_buffers(rz_ctx* c, uint64_t prims_need, uint64_t clip_need, uint64_t pairs_need,
                          uint64_t max_draw_tris, bool* grew_prims, uint64_t fine_need = 0) {
-    if (fine_need == 0) fine_need = std::max<uint64_t>(c->flist.cap, pairs_need * 4);
+    if (fine_need == 0) fine_need = std::max<uint64_t>(c->flist.cap, pairs_need * 8);
     int rc;
     if ((rc = grow(c, c->cls, max_draw_tris))) return rc;
     if ((rc = grow(c, c->nsub, max_draw_tris))) return rc;
@@ -275,6 +275,7 @@ int resolve_impl(rz_ctx* c, uint32_t* host_out, uint32_t* dev_out, uint32_t* hos
         }
     }
     uint32_t replays = 0;
+    uint32_t ovf_seen = 0;
     for (;;) {
         const uint32_t nb = c->bg.nbx * c->bg.nby;
         const uint32_t nc_max = (uint32_t)((c->geom.cap + CHUNK - 1) / CHUNK);
@@ -296,6 +297,7 @@ int resolve_impl(rz_ctx* c, uint32_t* host_out, uint32_t* dev_out, uint32_t* hos
         HIP_TRY(c, hipMemcpyAsync(c->h_cnt, c->d_cnt, sizeof(FrameCounters), hipMemcpyDeviceToHost, s));
         HIP_TRY(c, hipStreamSynchronize(s));
         const FrameCounters fc = *c->h_cnt;
+        ovf_seen |= fc.overflow;
         if (fc.overflow == 0) break;
         if (fc.overflow & OVF_POLY) {
             end_frame(c);
@@ -333,6 +335,7 @@ int resolve_impl(rz_ctx* c, uint32_t* host_out, uint32_t* dev_out, uint32_t* hos
     c->frame_stats[3] = fc.pairs_binned;
     c->frame_stats[4] = replays;
     for (int i = 0; i < 5; ++i) c->frame_stats[5 + i] = fc.rstat[i];
+    c->frame_stats[10] = ovf_seen;
     if (c->prof) {
         float a = 0, b = 0, d = 0;
         HIP_TRY(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
@@ -630,7 +633,7 @@ int rz_resolve_msaa(rz_ctx* c, uint32_t* argb_out, uint32_t* color4_out, float* 
 
 uint32_t rz_frame_flags(const rz_ctx* c) { return c ? c->frame_flags : 0u; }
 
-int rz_frame_stats(const rz_ctx* c, uint64_t out[10]) {
+int rz_frame_stats(const rz_ctx* c, uint64_t out[11]) {
     if (!c || !out) return RZ_ERR_ARG;
     memcpy(out, c->frame_stats, sizeof(c->frame_stats));
     return RZ_OK;

@@ -157,10 +157,10 @@ class Context:
         return int(self._L.rz_frame_flags(self.handle))
 
     def frame_stats(self) -> dict:
-        a = (C.c_uint64 * 10)()
+        a = (C.c_uint64 * 11)()
         self._check(self._L.rz_frame_stats(self.handle, a))
         keys = ["tris", "clipped", "prims", "bin_entries", "replays", "raster_entries", "raster_candidates",
-                "raster_hits", "raster_covered", "raster_shaded"]
+                "raster_hits", "raster_covered", "raster_shaded", "overflow_bits"]
         return dict(zip(keys, (int(x) for x in a)))
 
     def profile(self, enable: bool):
