@@ -63,7 +63,12 @@ enum rz_stage { RZ_STAGE_GEOMETRY = 0, RZ_STAGE_BINNING = 1, RZ_STAGE_RASTER = 2
  * bins of 256x256 pixels (e.g. up to 4096x4096 or 3840x2160), else RZ_ERR_LIMIT.
  * Uses the current HIP device. */
 int rz_create(uint32_t width, uint32_t height, rz_ctx **out);
-/* Drop for Renderer. Accepts NULL. */
+/* A further context on the same device sharing ctx's meshes and textures
+ * (no reference counterpart: lets several jobs -- frames or batches -- be in
+ * flight at once on their own streams over the same resident data). The
+ * shared data lives until the last context of the group is destroyed. */
+int rz_create_shared(rz_ctx *ctx, uint32_t width, uint32_t height, rz_ctx **out);
+/* Drop for Renderer. Accepts NULL. Waits for the context's work. */
 int rz_destroy(rz_ctx *ctx);
 /* Message of the last failed call on ctx ("" if none); static text for NULL ctx. */
 const char *rz_last_error(const rz_ctx *ctx);
@@ -82,6 +87,8 @@ int rz_set_uniforms(rz_ctx *ctx, const float world[16], const float view[16], co
  * of textures bound so far. rgba8 is th rows of tw texels, (0,0) top-left
  * (src/texture.rs:47-63); copied to the device. */
 int rz_bind_texture(rz_ctx *ctx, uint32_t index, const uint8_t *rgba8, uint32_t tw, uint32_t th);
+/* Number of textures bound in ctx's group (the next valid bind index). */
+uint32_t rz_texture_count(const rz_ctx *ctx);
 
 /* --------------------------------------------------------------------- mesh */
 
@@ -93,14 +100,15 @@ int rz_mesh_create(rz_ctx *ctx, const float *pos, const float *color, const floa
 /* Same, from device pointers already in HBM (copied device-to-device). */
 int rz_mesh_create_device(rz_ctx *ctx, const float *pos, const float *color, const float *uv,
                           uint32_t nv, const uint32_t *idx, uint64_t ni, rz_mesh **out);
-/* Fails with RZ_ERR_STATE while a draw of the mesh waits for rz_resolve*. */
+/* Fails with RZ_ERR_STATE while a draw of the mesh waits for rz_resolve* or a
+ * batch that uses it is unfinished. */
 int rz_mesh_destroy(rz_ctx *ctx, rz_mesh *mesh);
 
 /* -------------------------------------------------------------------- draws */
 
 /* Renderer::render(mesh, vertex_shader, fragment_shader) (src/render.rs:98-114):
- * queues vertex shading, clipping and triangle setup of every triangle of the
- * mesh with the current uniforms, in index order. Asynchronous. */
+ * records the draw with the current uniforms; its triangles are processed in
+ * index order, after earlier draws of the frame, when the frame is resolved. */
 int rz_draw(rz_ctx *ctx, const rz_mesh *mesh, int vertex_shader, int fragment_shader);
 
 /* Rasterizer::framebuffer (src/rasterizer/mod.rs:478-522): rasterizes all
@@ -114,9 +122,10 @@ int rz_resolve_device(rz_ctx *ctx, uint32_t *argb_dev);
  * (width*height*4 colour words / depths, sample-major per pixel;
  * ColorBuffer/DepthBuffer of src/rasterizer/buffers.rs:82-157). Any pointer may be NULL. */
 int rz_resolve_msaa(rz_ctx *ctx, uint32_t *argb_out, uint32_t *color4_out, float *depth4_out);
-/* RZ_FLAG_* bits raised while rendering the last resolved frame. */
+/* RZ_FLAG_* bits raised while rendering the last resolved frame (or, after a
+ * batch, by any of its scenes). */
 uint32_t rz_frame_flags(const rz_ctx *ctx);
-/* Counts of the last resolved frame: [input triangles, triangles clipped,
+/* Counts of the last resolved frame or batch: [input triangles, triangles clipped,
  * rasterizer triangles, coarse bin entries, frame replays, then -- only when
  * the context was created with RZ_RASTER_STATS=1 in the environment, else 0 --
  * raster list entries scanned, candidates after the bbox / depth / edge
@@ -125,6 +134,48 @@ uint32_t rz_frame_flags(const rz_ctx *ctx);
  * (bits: 1 triangles, 2 clip slots, 4 coarse entries, 16 fine entries)]. */
 int rz_frame_stats(const rz_ctx *ctx, uint64_t out[11]);
 
+/* ------------------------------------------------------------------ batches
+ * Many independent scenes rendered by one job (no reference counterpart: the
+ * reference renders one frame per Renderer). Each scene is a render target
+ * with its own size, clear depth and draws; a scene's draws are rendered in
+ * the order they were recorded, exactly as rz_draw/rz_resolve would render
+ * them in a context of that size, so every scene's output is identical to
+ * rendering it alone. The batch runs as one launch sequence (captured once
+ * as a HIP graph and replayed while the buffers fit). */
+
+/* Start recording a batch (no frame may be pending, no batch in flight). */
+int rz_batch_begin(rz_ctx *ctx);
+/* Add a scene: width x height (same limits as rz_create), clear depth, and
+ * device memory for its width*height resolved 0xAARRGGBB words. Returns its
+ * index in *scene_out (may be NULL). */
+int rz_batch_scene(rz_ctx *ctx, uint32_t width, uint32_t height, float clear_depth, uint32_t *argb_dev,
+                   uint32_t *scene_out);
+/* Debug: also write the scene's pre-resolve MSAA colours / depths
+ * (width*height*4 each, sample-major per pixel, as rz_resolve_msaa) to
+ * device memory. Either pointer may be NULL. */
+int rz_batch_scene_msaa(rz_ctx *ctx, uint32_t scene, uint32_t *color4_dev, float *depth4_dev);
+/* Renderer::render into a scene with explicit uniforms (row-major world,
+ * view, projection) and texture index (for RZ_FS_TEXTURE; see
+ * rz_bind_texture). */
+int rz_batch_draw(rz_ctx *ctx, uint32_t scene, const rz_mesh *mesh, const float world[16], const float view[16],
+                  const float proj[16], uint32_t texture, int vertex_shader, int fragment_shader);
+/* Queue the batch on the context stream and return. */
+int rz_batch_submit(rz_ctx *ctx);
+/* 1 if the submitted batch has finished (or none is in flight), 0 if it is
+ * still running, < 0 on error. Never blocks except to grow buffers. */
+int rz_batch_query(rz_ctx *ctx);
+/* Block until the submitted batch has finished. */
+int rz_batch_wait(rz_ctx *ctx);
+/* RZ_FLAG_* bits of one scene of the last finished batch. */
+int rz_batch_scene_flags(const rz_ctx *ctx, uint32_t scene, uint32_t *flags);
+/* Batch tuning: bin_target = most coarse bins per scene when choosing the
+ * bin side automatically (1..256); raster_per_cu = persistent raster
+ * workgroups per CU (0: one workgroup per tile); use_graph = replay the
+ * captured HIP graph (1) or launch the kernels directly (0). Between jobs. */
+int rz_batch_tuning(rz_ctx *ctx, uint32_t bin_target, uint32_t raster_per_cu, int use_graph);
+/* The current tuning: [bin_target, raster_per_cu, use_graph]. */
+int rz_batch_get_tuning(const rz_ctx *ctx, uint32_t out[3]);
+
 /* ------------------------------------------------------------ tuning/timing */
 
 /* Coarse bin side in pixels: a multiple of 16 up to 256 with at most 256 bins
@@ -132,7 +183,9 @@ int rz_frame_stats(const rz_ctx *ctx, uint64_t out[11]);
 int rz_set_coarse_bin(rz_ctx *ctx, uint32_t pixels);
 /* Enable (1) / disable (0) per-stage HIP event timing; enabling resets sums. */
 int rz_profile(rz_ctx *ctx, int enable);
-/* Summed milliseconds per stage (RZ_NSTAGES floats) and the number of frames. */
+/* Summed milliseconds per stage (RZ_NSTAGES floats) and the number of frames
+ * or batches. A batch replayed from its graph is timed as a whole, under
+ * RZ_STAGE_RASTER. */
 int rz_profile_read(const rz_ctx *ctx, float ms[RZ_NSTAGES], uint32_t *frames);
 
 #ifdef __cplusplus

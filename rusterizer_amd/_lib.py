@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_lib", "librz.so")
@@ -23,12 +24,14 @@ _u32 = C.POINTER(C.c_uint32)
 _u8 = C.POINTER(C.c_uint8)
 SIGNATURES = {
     "rz_create": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
+    "rz_create_shared": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
     "rz_destroy": (C.c_int, [_vp]),
     "rz_last_error": (C.c_char_p, [_vp]),
     "rz_stream": (_vp, [_vp]),
     "rz_set_clear_depth": (C.c_int, [_vp, C.c_float]),
     "rz_set_uniforms": (C.c_int, [_vp, _f, _f, _f]),
     "rz_bind_texture": (C.c_int, [_vp, C.c_uint32, _u8, C.c_uint32, C.c_uint32]),
+    "rz_texture_count": (C.c_uint32, [_vp]),
     "rz_mesh_create": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp, C.c_uint64, C.POINTER(_vp)]),
     "rz_mesh_create_device": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp, C.c_uint64, C.POINTER(_vp)]),
     "rz_mesh_destroy": (C.c_int, [_vp, _vp]),
@@ -38,6 +41,16 @@ SIGNATURES = {
     "rz_resolve_msaa": (C.c_int, [_vp, _vp, _vp, _vp]),
     "rz_frame_flags": (C.c_uint32, [_vp]),
     "rz_frame_stats": (C.c_int, [_vp, C.POINTER(C.c_uint64)]),
+    "rz_batch_begin": (C.c_int, [_vp]),
+    "rz_batch_scene": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_float, _vp, _u32]),
+    "rz_batch_scene_msaa": (C.c_int, [_vp, C.c_uint32, _vp, _vp]),
+    "rz_batch_draw": (C.c_int, [_vp, C.c_uint32, _vp, _f, _f, _f, C.c_uint32, C.c_int, C.c_int]),
+    "rz_batch_submit": (C.c_int, [_vp]),
+    "rz_batch_query": (C.c_int, [_vp]),
+    "rz_batch_wait": (C.c_int, [_vp]),
+    "rz_batch_scene_flags": (C.c_int, [_vp, C.c_uint32, _u32]),
+    "rz_batch_tuning": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_int]),
+    "rz_batch_get_tuning": (C.c_int, [_vp, _u32]),
     "rz_set_coarse_bin": (C.c_int, [_vp, C.c_uint32]),
     "rz_profile": (C.c_int, [_vp, C.c_int]),
     "rz_profile_read": (C.c_int, [_vp, _f, _u32]),
@@ -58,6 +71,11 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(
             f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+    # PyTorch ships its own HIP runtime; in one process it must initialise
+    # the GPU before this library's runtime does, or it finds no device.
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_available():
+        torch.cuda.init()
     L = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(L, name)

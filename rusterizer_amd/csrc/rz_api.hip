@@ -1,17 +1,22 @@
 // rz_api.hip -- host side of the C ABI declared in include/rz.h.
 //
-// A context owns one HIP stream, the frame's draw list, and the device
-// buffers of the pipeline. Draws only enqueue the geometry stages; the
-// frame is binned and rasterized at resolve time, so the 4x MSAA colour and
-// depth of each pixel stay in registers for the whole frame (no MSAA
-// traffic to HBM). Buffers that a frame outgrows are detected on the device
-// (overflow bits); the resolve then grows them and replays the frame, so no
-// triangle is ever dropped.
+// A context owns one HIP stream and the device buffers of the pipeline.
+// Draws are recorded on the host; the work runs as one job at resolve time
+// (plain frame: one scene) or at rz_batch_submit (batch: many scenes), so
+// the 4x MSAA colour and depth of each pixel stay in registers for the whole
+// frame (no MSAA traffic to HBM). Buffers that a job outgrows are detected
+// on the device (overflow bits); the host then grows them and replays the
+// job, so no triangle is ever dropped.
+//
+// Meshes and textures live in a pool shared by every context created from
+// the same root context (rz_create_shared), so several contexts -- several
+// jobs in flight on their own streams -- can render the same resident data.
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -28,7 +33,9 @@ constexpr uint32_t MAX_DIM = 16384;
 // a coarse bin holds at most 256 16x16 tiles (fine binning keeps one running
 // count per tile in 4 registers per lane)
 constexpr uint32_t MAX_COARSE_PX = 256;
-constexpr uint32_t MAX_DRAWS = 65536;
+constexpr uint32_t MAX_DRAWS = 65536;   // draw index lives in 16 bits of a prim's flags
+constexpr uint32_t MAX_SCENES = 65536;
+constexpr uint32_t MAX_REPLAYS = 4;
 
 const char* kNullCtxMsg = "null context";
 
@@ -45,60 +52,120 @@ struct DBuf {
 
 }  // namespace
 
+// resources shared by the contexts of one root (same device)
+struct rz_pool {
+    int device = 0;
+    std::vector<Texture> textures;
+    std::vector<rz_ctx*> ctxs;
+};
+
 struct rz_mesh {
-    rz_ctx* ctx = nullptr;
+    rz_pool* pool = nullptr;
     float* pos = nullptr;
     float* col = nullptr;
     float* uv = nullptr;
     uint32_t* idx = nullptr;
     uint32_t nv = 0;
     uint32_t ntri = 0;
-    uint32_t pending = 0;  // draws of the unresolved frame using this mesh
+    uint32_t pending = 0;  // recorded draws not yet finished, over all contexts
 };
 
+namespace {
+
+struct PendingDraw {
+    rz_mesh* mesh;
+    uint32_t scene;
+    uint32_t fs;
+    uint32_t tex;  // texture index (RZ_FS_TEXTURE)
+    Mat3 m;
+};
+
+struct PendingScene {
+    uint32_t W, H;
+    float clear_depth;
+    uint32_t* out;
+    uint32_t* msaa_color;
+    float* msaa_depth;
+};
+
+// host-side layout of one job
+struct JobPlan {
+    uint32_t ndraws = 0, nscenes = 0;
+    uint32_t ntri = 0, ntiles = 0, nbins = 0;
+    uint32_t max_nb = 1, max_T = 1;
+    uint64_t tris = 0;  // input triangles
+};
+
+enum class JobKind { None, Frame, Batch };
+
+}  // namespace
+
 struct rz_ctx {
-    int device = 0;
+    rz_pool* pool = nullptr;
     hipStream_t stream = nullptr;
-    uint32_t W = 0, H = 0;
+    uint32_t W = 0, H = 0;  // plain-frame screen
     float clear_depth = FLT_MAX;
     float next_clear_depth = FLT_MAX;
     Mat3 uni{};
-    std::vector<Texture> textures;
-    BinGeom bg{};
-    uint32_t coarse_req = 0;
+    uint32_t coarse_req = 0;  // plain frame: coarse bin side, 0 = automatic
+    // batch tuning
+    uint32_t bin_target = MAX_BINS;  // automatic bins: smallest power-of-2 side with at most this many bins
+    uint32_t raster_per_cu = 0;      // persistent raster workgroups per CU (0: one workgroup per tile)
+    bool use_graph = true;
+    int n_cu = 256;
 
-    // frame state
-    std::vector<rz_mesh*> draw_mesh;
-    DrawDev* d_draws = nullptr;  // MAX_DRAWS entries
-    DrawDev* h_draws = nullptr;  // pinned mirror
-    Mat3* h_mats = nullptr;      // pinned: uniforms of each draw
+    // recorded work
+    std::vector<PendingDraw> draws;
+    std::vector<PendingScene> scenes;
+    JobKind recording = JobKind::None;
+    bool in_flight = false;
+    JobPlan plan;
+
+    // job tables: pinned staging block and its device copy (same layout)
+    uint8_t* h_job = nullptr;
+    uint8_t* d_job = nullptr;
+    size_t job_bytes = 0;
+    uint32_t dcap = 0, scap = 0;
+    size_t off_draws = 0, off_mats = 0, off_scenes = 0;
+    SceneRun* d_runs = nullptr;
+    SceneRun* h_runs = nullptr;  // pinned
     FrameCounters* d_cnt = nullptr;
     FrameCounters* h_cnt = nullptr;  // pinned
 
-    // per-draw scratch
+    // pipeline buffers
     DBuf<uint32_t> cls, nsub, suboff, parts;
-    // clipper
+    DBuf<unsigned long long> parts64;
     DBuf<uint32_t> clip_tri, clip_n;
     DBuf<float> clip_out;  // MAXV * 10 floats per slot
-    // rasterizer triangles
     DBuf<PrimGeom> geom;
     DBuf<float> attrs;  // ATTR_STRIDE per prim
     DBuf<uint4> pbin;
-    // coarse bins
     DBuf<uint32_t> bin_counts, bin_offsets;
     DBuf<uint4> list;
-    // fine bins (16x16 tiles)
-    DBuf<uint32_t> chunk_base, fine_counts, fine_off;
+    DBuf<uint2> bin_rng;
+    DBuf<unsigned long long> fpack, fpre;
+    DBuf<uint32_t> fine_counts, fine_off;
     DBuf<uint4> flist;
-    // outputs
+    // plain-frame outputs
     DBuf<uint32_t> out;
     DBuf<uint32_t> msaa_color;
     DBuf<float> msaa_depth;
+    uint64_t gen = 0;  // bumped whenever a buffer moves (invalidates the graph)
+
+    // graph of the batch launch sequence
+    hipGraphExec_t graph = nullptr;
+    uint64_t graph_gen = ~0ull;
+    uint32_t graph_dcap = 0, graph_scap = 0, graph_raster = 0;
+    bool graph_stats = false;
+
+    hipEvent_t done = nullptr;
+    uint32_t replays = 0, ovf_seen = 0;
+    bool job_graph = false;  // the job in flight was launched from the graph
 
     uint32_t frame_flags = 0;
     uint64_t frame_stats[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    std::vector<uint32_t> scene_flags;
     bool raster_stats = false;  // RZ_RASTER_STATS=1 at rz_create
-    uint64_t frame_tris = 0;
 
     bool prof = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -123,7 +190,7 @@ int fail(rz_ctx* c, int code, const std::string& msg) {
     } while (0)
 
 template <class T>
-int grow(rz_ctx* c, DBuf<T>& b, size_t need, bool* grew = nullptr) {
+int grow(rz_ctx* c, DBuf<T>& b, size_t need) {
     if (b.cap >= need && b.p) return RZ_OK;
     size_t cap = std::max<size_t>(need, 1024);
     cap = cap + cap / 4;
@@ -138,7 +205,7 @@ int grow(rz_ctx* c, DBuf<T>& b, size_t need, bool* grew = nullptr) {
         return fail(c, RZ_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     }
     b.cap = cap;
-    if (grew) *grew = true;
+    c->gen++;
     return RZ_OK;
 }
 
@@ -149,111 +216,391 @@ void release(DBuf<T>& b) {
     b.cap = 0;
 }
 
-BinGeom choose_bins(uint32_t W, uint32_t H, uint32_t req) {
-    BinGeom g{};
-    uint32_t cb = req;
-    if (cb == 0) {
-        cb = 16;
-        while (((W + cb - 1) / cb) * ((H + cb - 1) / cb) > MAX_BINS) cb *= 2;
+// coarse bin side for a W x H scene: req if given, else the smallest
+// power-of-two multiple of 16 giving at most `target` bins (at most 256 px)
+uint32_t coarse_side(uint32_t W, uint32_t H, uint32_t req, uint32_t target) {
+    if (req) return req;
+    uint32_t cb = 16;
+    while (cb < MAX_COARSE_PX && ((W + cb - 1) / cb) * ((H + cb - 1) / cb) > target) cb *= 2;
+    return cb;
+}
+
+uint32_t clamp_u32(uint64_t v) { return v > 0xFFFFFFF0ull ? 0xFFFFFFF0u : (uint32_t)v; }
+
+// table block layout at capacities (dcap draws, scap scenes)
+int ensure_tables(rz_ctx* c, uint32_t nd, uint32_t ns) {
+    if (nd <= c->dcap && ns <= c->scap && c->h_job) return RZ_OK;
+    uint32_t dcap = std::max<uint32_t>(c->dcap, 16), scap = std::max<uint32_t>(c->scap, 4);
+    while (dcap < nd) dcap *= 2;
+    while (scap < ns) scap *= 2;
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    size_t off_d = al(sizeof(JobHead));
+    size_t off_m = al(off_d + sizeof(DrawDev) * dcap);
+    size_t off_s = al(off_m + sizeof(Mat3) * dcap);
+    size_t bytes = al(off_s + sizeof(SceneCfg) * scap);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->h_job) (void)hipHostFree(c->h_job);
+    if (c->d_job) (void)hipFree(c->d_job);
+    if (c->h_runs) (void)hipHostFree(c->h_runs);
+    if (c->d_runs) (void)hipFree(c->d_runs);
+    c->h_job = nullptr;
+    c->d_job = nullptr;
+    c->h_runs = nullptr;
+    c->d_runs = nullptr;
+    c->dcap = c->scap = 0;
+    HIP_TRY(c, hipHostMalloc(&c->h_job, bytes, hipHostMallocDefault));
+    HIP_TRY(c, hipMalloc(&c->d_job, bytes));
+    HIP_TRY(c, hipHostMalloc(&c->h_runs, sizeof(SceneRun) * scap, hipHostMallocDefault));
+    HIP_TRY(c, hipMalloc(&c->d_runs, sizeof(SceneRun) * scap));
+    memset(c->h_job, 0, bytes);
+    c->job_bytes = bytes;
+    c->off_draws = off_d;
+    c->off_mats = off_m;
+    c->off_scenes = off_s;
+    c->dcap = dcap;
+    c->scap = scap;
+    c->gen++;
+    return RZ_OK;
+}
+
+// Lay out the recorded scenes and draws (draws must be grouped by scene,
+// in order) into the pinned table block.
+int build_job(rz_ctx* c, JobPlan& pl) {
+    const uint32_t nd = (uint32_t)c->draws.size(), ns = (uint32_t)c->scenes.size();
+    int rc;
+    if ((rc = ensure_tables(c, nd, ns))) return rc;
+    JobHead* head = reinterpret_cast<JobHead*>(c->h_job);
+    DrawDev* dd = reinterpret_cast<DrawDev*>(c->h_job + c->off_draws);
+    Mat3* mats = reinterpret_cast<Mat3*>(c->h_job + c->off_mats);
+    SceneCfg* sc = reinterpret_cast<SceneCfg*>(c->h_job + c->off_scenes);
+    pl = JobPlan{};
+    pl.ndraws = nd;
+    pl.nscenes = ns;
+    uint64_t tri = 0, tiles = 0, bins = 0;
+    uint32_t d = 0;
+    for (uint32_t s = 0; s < ns; ++s) {
+        const PendingScene& ps = c->scenes[s];
+        SceneCfg& g = sc[s];
+        memset(&g, 0, sizeof(g));
+        g.W = ps.W;
+        g.H = ps.H;
+        g.clear_depth = ps.clear_depth;
+        g.cb = c->recording == JobKind::Frame ? coarse_side(ps.W, ps.H, c->coarse_req, MAX_BINS)
+                                               : coarse_side(ps.W, ps.H, 0, c->bin_target);
+        g.nbx = (ps.W + g.cb - 1) / g.cb;
+        g.nby = (ps.H + g.cb - 1) / g.cb;
+        g.tile_base = (uint32_t)tiles;
+        g.bin_base = (uint32_t)bins;
+        g.out = ps.out;
+        g.msaa_color = ps.msaa_color;
+        g.msaa_depth = ps.msaa_depth;
+        g.tri_begin = (uint32_t)tri;
+        for (; d < nd && c->draws[d].scene == s; ++d) {
+            const PendingDraw& pd = c->draws[d];
+            DrawDev& h = dd[d];
+            memset(&h, 0, sizeof(h));
+            h.pos = pd.mesh->pos;
+            h.col = pd.mesh->col;
+            h.uv = pd.mesh->uv;
+            h.idx = pd.mesh->idx;
+            h.ntri = pd.mesh->ntri;
+            h.fs = pd.fs;
+            h.scene = s;
+            h.tri_base = (uint32_t)tri;
+            if (pd.fs == RZ_FS_TEXTURE) {
+                const Texture& t = c->pool->textures[pd.tex];
+                h.tex = t.data;
+                h.tw = t.tw;
+                h.th = t.th;
+            }
+            mats[d] = pd.m;
+            tri += ((uint64_t)pd.mesh->ntri + TRI_ALIGN - 1) / TRI_ALIGN * TRI_ALIGN;
+            pl.tris += pd.mesh->ntri;
+            if (tri > 0xFFFF0000ull) return fail(c, RZ_ERR_LIMIT, "more than 2^32 triangles in one job");
+        }
+        g.tri_end = (uint32_t)tri;
+        tiles += (uint64_t)((ps.W + 15) / 16) * ((ps.H + 15) / 16);
+        bins += (uint64_t)g.nbx * g.nby;
+        pl.max_nb = std::max(pl.max_nb, g.nbx * g.nby);
+        pl.max_T = std::max(pl.max_T, (g.cb / 16) * (g.cb / 16));
     }
-    g.cb = cb;
-    g.nbx = (W + cb - 1) / cb;
-    g.nby = (H + cb - 1) / cb;
+    if (d != nd) return fail(c, RZ_ERR_STATE, "draws are not grouped by scene");
+    if (tiles > 0xFFFF0000ull || bins > 0xFFFF0000ull) return fail(c, RZ_ERR_LIMIT, "job too large");
+    pl.ntri = (uint32_t)tri;
+    pl.ntiles = (uint32_t)tiles;
+    pl.nbins = (uint32_t)bins;
+    memset(head, 0, sizeof(*head));
+    head->ndraws = nd;
+    head->nscenes = ns;
+    head->ntri = pl.ntri;
+    head->ntiles = pl.ntiles;
+    head->nbins = pl.nbins;
+    return RZ_OK;
+}
+
+// capacities for a job (estimates; the device reports exact needs on overflow)
+struct Needs {
+    uint64_t prims = 0, clip = 0, pairs = 0, fine = 0, mat = 0, fmat = 0;
+};
+
+int ensure_buffers(rz_ctx* c, const JobPlan& pl, const Needs& n_in) {
+    Needs n = n_in;
+    const uint64_t NT = pl.ntri;
+    n.prims = std::max<uint64_t>(n.prims, NT + NT / 4 + 4096 + (uint64_t)pl.nscenes * CHUNK);
+    n.clip = std::max<uint64_t>(n.clip, NT / 8 + 1024);
+    n.pairs = std::max<uint64_t>(n.pairs, n.prims * 2);
+    n.fine = std::max<uint64_t>(n.fine, std::max<uint64_t>(c->flist.cap, n.pairs * 8));
+    const uint64_t lim = 0xF0000000ull;
+    if (n.prims > lim || n.pairs > lim || n.fine > lim || n.clip > lim)
+        return fail(c, RZ_ERR_LIMIT, "job needs more than 2^32 pipeline entries");
+    int rc;
+    if ((rc = grow(c, c->cls, std::max<uint64_t>(NT, 256)))) return rc;
+    if ((rc = grow(c, c->nsub, c->cls.cap))) return rc;
+    if ((rc = grow(c, c->suboff, c->cls.cap))) return rc;
+    if ((rc = grow(c, c->geom, n.prims))) return rc;
+    if ((rc = grow(c, c->attrs, c->geom.cap * ATTR_STRIDE))) return rc;
+    if ((rc = grow(c, c->pbin, c->geom.cap))) return rc;
+    if ((rc = grow(c, c->clip_tri, n.clip))) return rc;
+    if ((rc = grow(c, c->clip_n, c->clip_tri.cap))) return rc;
+    if ((rc = grow(c, c->clip_out, c->clip_tri.cap * MAXV * 10))) return rc;
+    if ((rc = grow(c, c->list, n.pairs))) return rc;
+    // coarse matrix: sum over scenes of bins * chunks <= max_nb * (slots / CHUNK)
+    const uint64_t mat = std::max<uint64_t>(n.mat, (uint64_t)pl.max_nb * (c->geom.cap / CHUNK + 1));
+    if ((rc = grow(c, c->bin_counts, mat))) return rc;
+    if ((rc = grow(c, c->bin_offsets, c->bin_counts.cap))) return rc;
+    if ((rc = grow(c, c->bin_rng, std::max<uint64_t>(pl.nbins, 1)))) return rc;
+    if ((rc = grow(c, c->fpack, c->bin_rng.cap))) return rc;
+    if ((rc = grow(c, c->fpre, c->bin_rng.cap))) return rc;
+    if ((rc = grow(c, c->flist, n.fine))) return rc;
+    // fine matrix: sum over bins of T * ceil(len / CHUNK) <= max_T * (entries / CHUNK + bins)
+    const uint64_t fmat =
+        std::max<uint64_t>(n.fmat, (uint64_t)pl.max_T * (c->list.cap / CHUNK + c->bin_rng.cap + 1));
+    if (mat > lim || fmat > lim) return fail(c, RZ_ERR_LIMIT, "bin matrices exceed 2^32 entries");
+    if ((rc = grow(c, c->fine_counts, fmat))) return rc;
+    if ((rc = grow(c, c->fine_off, c->fine_counts.cap))) return rc;
+    const size_t scan_n = std::max({c->cls.cap, c->bin_counts.cap, c->fine_counts.cap});
+    if ((rc = grow(c, c->parts, scan_n / SCAN_TILE + 2))) return rc;
+    if ((rc = grow(c, c->parts64, c->bin_rng.cap / SCAN_TILE + 2))) return rc;
+    return RZ_OK;
+}
+
+JobBufs job_bufs(const rz_ctx* c) {
+    JobBufs b{};
+    b.head = reinterpret_cast<const JobHead*>(c->d_job);
+    b.draws = reinterpret_cast<DrawDev*>(c->d_job + c->off_draws);
+    b.mats = reinterpret_cast<const Mat3*>(c->d_job + c->off_mats);
+    b.scenes = reinterpret_cast<const SceneCfg*>(c->d_job + c->off_scenes);
+    b.runs = c->d_runs;
+    b.cnt = c->d_cnt;
+    b.cls = c->cls.p;
+    b.nsub = c->nsub.p;
+    b.suboff = c->suboff.p;
+    b.clip_tri = c->clip_tri.p;
+    b.clip_n = c->clip_n.p;
+    b.clip_out = c->clip_out.p;
+    b.geom = c->geom.p;
+    b.attrs = c->attrs.p;
+    b.pbin = c->pbin.p;
+    b.bin_counts = c->bin_counts.p;
+    b.bin_offsets = c->bin_offsets.p;
+    b.list = c->list.p;
+    b.bin_rng = c->bin_rng.p;
+    b.fpack = c->fpack.p;
+    b.fpre = c->fpre.p;
+    b.fine_counts = c->fine_counts.p;
+    b.fine_off = c->fine_off.p;
+    b.flist = c->flist.p;
+    b.parts = c->parts.p;
+    b.parts64 = c->parts64.p;
+    b.tri_cap = clamp_u32(c->cls.cap);
+    b.clip_cap = clamp_u32(c->clip_tri.cap);
+    b.prim_cap = clamp_u32(c->geom.cap);
+    b.pair_cap = clamp_u32(c->list.cap);
+    b.mat_cap = clamp_u32(c->bin_counts.cap);
+    b.fine_cap = clamp_u32(c->flist.cap);
+    b.fmat_cap = clamp_u32(c->fine_counts.cap);
+    b.bins_cap = clamp_u32(c->bin_rng.cap);
+    return b;
+}
+
+// grids: exact where the host knows the work (plain launches), capacity
+// bounds otherwise (graph launches serve any job that fits the buffers)
+JobGrid job_grid(const rz_ctx* c, const JobPlan& pl, bool for_graph) {
+    JobGrid g{};
+    const uint32_t nd = for_graph ? c->dcap : pl.ndraws;
+    const uint32_t nt = for_graph ? (uint32_t)c->cls.cap : pl.ntri;
+    g.draws = (nd + 255) / 256;
+    g.tris = std::min<uint32_t>((nt + 255) / 256, 8192);
+    g.clip = (uint32_t)std::min<size_t>(c->clip_tri.cap, 8192);
+    const uint64_t chunks = c->geom.cap / CHUNK + (for_graph ? c->scap : pl.nscenes);
+    g.bins = (uint32_t)std::min<uint64_t>((chunks + 3) / 4, 4096);
+    const uint64_t fch = c->list.cap / CHUNK + c->bin_rng.cap;
+    g.fine = (uint32_t)std::min<uint64_t>((fch + 3) / 4, 4096);
+    g.raster = c->raster_per_cu ? c->raster_per_cu * (uint32_t)c->n_cu : 0;
+    if (for_graph && g.raster == 0) g.raster = 8 * (uint32_t)c->n_cu;  // graph: persistent raster
+    g.ntiles = pl.ntiles;
+    g.scenes = for_graph ? c->scap : pl.nscenes;
     return g;
 }
 
-uint32_t bin_matrix_len(const rz_ctx* c) {
-    uint32_t nb = c->bg.nbx * c->bg.nby;
-    size_t nc = (c->geom.cap + CHUNK - 1) / CHUNK;
-    return (uint32_t)(nb * nc);
-}
-
-// upper bound of coarse-list chunks: every bin adds at most one partial chunk
-uint32_t nch_max(const rz_ctx* c) { return (uint32_t)(c->list.cap / CHUNK + MAX_BINS + 1); }
-
-uint32_t fine_matrix_len(const rz_ctx* c) {
-    uint32_t T = (c->bg.cb / 16) * (c->bg.cb / 16);
-    return T * nch_max(c);
-}
-
-// capacities for the frame's rasterizer triangles, clip slots and bin entries
-int ensure_frame_buffers(rz_ctx* c, uint64_t prims_need, uint64_t clip_need, uint64_t pairs_need,
-                         uint64_t max_draw_tris, bool* grew_prims, uint64_t fine_need = 0) {
-    if (fine_need == 0) fine_need = std::max<uint64_t>(c->flist.cap, pairs_need * 8);
-    int rc;
-    if ((rc = grow(c, c->cls, max_draw_tris))) return rc;
-    if ((rc = grow(c, c->nsub, max_draw_tris))) return rc;
-    if ((rc = grow(c, c->suboff, max_draw_tris))) return rc;
-    bool g = false;
-    if ((rc = grow(c, c->geom, prims_need, &g))) return rc;
-    if ((rc = grow(c, c->attrs, c->geom.cap * ATTR_STRIDE))) return rc;
-    if ((rc = grow(c, c->pbin, c->geom.cap))) return rc;
-    if ((rc = grow(c, c->clip_tri, clip_need, &g))) return rc;
-    if ((rc = grow(c, c->clip_n, c->clip_tri.cap))) return rc;
-    if ((rc = grow(c, c->clip_out, c->clip_tri.cap * MAXV * 10))) return rc;
-    if ((rc = grow(c, c->list, pairs_need))) return rc;
-    size_t bm = bin_matrix_len(c);
-    if ((rc = grow(c, c->bin_counts, bm))) return rc;
-    if ((rc = grow(c, c->bin_offsets, bm))) return rc;
-    if ((rc = grow(c, c->chunk_base, MAX_BINS + 1))) return rc;
-    if ((rc = grow(c, c->flist, fine_need))) return rc;
-    size_t fm = fine_matrix_len(c);
-    if ((rc = grow(c, c->fine_counts, fm))) return rc;
-    if ((rc = grow(c, c->fine_off, fm))) return rc;
-    size_t scan_n = std::max<size_t>(std::max<size_t>(c->cls.cap, bm), fm);
-    if ((rc = grow(c, c->parts, (scan_n + SCAN_TILE - 1) / SCAN_TILE + 1))) return rc;
-    if (grew_prims) *grew_prims = g;
-    return RZ_OK;
-}
-
-void enqueue_draw_geometry(rz_ctx* c, uint32_t d) {
-    const DrawDev& h = c->h_draws[d];
+int enqueue_eager(rz_ctx* c, const JobPlan& pl) {
     hipStream_t s = c->stream;
-    launch_draw_reset(s, c->d_cnt);
-    launch_classify(s, c->d_draws, d, h.ntri, c->d_cnt, c->cls.p, c->nsub.p, c->clip_tri.p,
-                    (uint32_t)c->clip_tri.cap);
-    launch_clip(s, c->d_draws, d, c->d_cnt, c->clip_tri.p, (uint32_t)c->clip_tri.cap, c->nsub.p, c->clip_out.p,
-                c->clip_n.p);
-    launch_scan(s, c->nsub.p, c->suboff.p, h.ntri, nullptr, 0, 1, c->parts.p, &c->d_cnt->nprims, nullptr);
-    launch_setup(s, c->d_draws, d, h.ntri, c->d_cnt, c->cls.p, c->nsub.p, c->suboff.p, c->clip_out.p,
-                 (uint32_t)c->clip_tri.cap, c->geom.p, c->attrs.p, c->pbin.p, (uint32_t)c->geom.cap, c->W, c->H,
-                 c->bg);
-}
-
-// upload draw d's descriptor and uniforms, compute its MVP on the device
-int enqueue_draw_desc(rz_ctx* c, uint32_t d) {
-    HIP_TRY(c, hipMemcpyAsync(c->d_draws + d, c->h_draws + d, sizeof(DrawDev), hipMemcpyHostToDevice, c->stream));
-    launch_mvp(c->stream, c->d_draws, d, c->h_mats[d]);
+    if (c->prof) HIP_TRY(c, hipEventRecord(c->ev[0], s));
+    HIP_TRY(c, hipMemcpyAsync(c->d_job, c->h_job, c->job_bytes, hipMemcpyHostToDevice, s));
+    launch_job(s, job_bufs(c), job_grid(c, pl, false), c->raster_stats, c->prof ? c->ev[1] : nullptr,
+               c->prof ? c->ev[2] : nullptr);
+    if (c->prof) HIP_TRY(c, hipEventRecord(c->ev[3], s));
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipMemcpyAsync(c->h_cnt, c->d_cnt, sizeof(FrameCounters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(c->h_runs, c->d_runs, sizeof(SceneRun) * std::max<uint32_t>(pl.nscenes, 1),
+                              hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipEventRecord(c->done, s));
+    c->job_graph = false;
     return RZ_OK;
 }
 
-int replay_geometry(rz_ctx* c, uint32_t upto) {
-    launch_frame_reset(c->stream, c->d_cnt);
-    for (uint32_t d = 0; d < upto; ++d) {
-        int rc = enqueue_draw_desc(c, d);
-        if (rc) return rc;
-        enqueue_draw_geometry(c, d);
+// capture (once per buffer generation / table capacity / tuning) and launch
+int enqueue_graph(rz_ctx* c, const JobPlan& pl) {
+    hipStream_t s = c->stream;
+    const JobGrid g = job_grid(c, pl, true);
+    if (!c->graph || c->graph_gen != c->gen || c->graph_dcap != c->dcap || c->graph_scap != c->scap ||
+        c->graph_raster != g.raster || c->graph_stats != c->raster_stats) {
+        if (c->graph) {
+            HIP_TRY(c, hipStreamSynchronize(s));
+            (void)hipGraphExecDestroy(c->graph);
+            c->graph = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        HIP_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipMemcpyAsync(c->d_job, c->h_job, c->job_bytes, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) {
+            launch_job(s, job_bufs(c), g, c->raster_stats, nullptr, nullptr);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(c->h_cnt, c->d_cnt, sizeof(FrameCounters), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(c->h_runs, c->d_runs, sizeof(SceneRun) * c->scap, hipMemcpyDeviceToHost, s);
+        hipError_t e2 = hipStreamEndCapture(s, &graph);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(c, RZ_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        e = hipGraphInstantiate(&c->graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            c->graph = nullptr;
+            return fail(c, RZ_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        c->graph_gen = c->gen;
+        c->graph_dcap = c->dcap;
+        c->graph_scap = c->scap;
+        c->graph_raster = g.raster;
+        c->graph_stats = c->raster_stats;
     }
+    if (c->prof) HIP_TRY(c, hipEventRecord(c->ev[0], s));
+    HIP_TRY(c, hipGraphLaunch(c->graph, s));
+    if (c->prof) HIP_TRY(c, hipEventRecord(c->ev[3], s));
+    HIP_TRY(c, hipEventRecord(c->done, s));
+    c->job_graph = true;
     return RZ_OK;
 }
 
-uint32_t max_draw_tris(const rz_ctx* c) {
-    uint32_t m = 0;
-    for (size_t d = 0; d < c->draw_mesh.size(); ++d) m = std::max(m, c->h_draws[d].ntri);
-    return m;
+int enqueue(rz_ctx* c, bool graph) {
+    int rc = ensure_buffers(c, c->plan, Needs{});
+    if (rc) return rc;
+    return graph ? enqueue_graph(c, c->plan) : enqueue_eager(c, c->plan);
 }
 
-void end_frame(rz_ctx* c) {
-    for (rz_mesh* m : c->draw_mesh) m->pending--;
-    c->draw_mesh.clear();
-    c->frame_tris = 0;
-    c->clear_depth = c->next_clear_depth;
+// the recorded draws hold their meshes (pending) until the job is done
+void release_job(rz_ctx* c) {
+    for (PendingDraw& d : c->draws) d.mesh->pending--;
+    c->draws.clear();
+    c->scenes.clear();
+    c->recording = JobKind::None;
+    c->in_flight = false;
 }
 
-int resolve_impl(rz_ctx* c, uint32_t* host_out, uint32_t* dev_out, uint32_t* host_color4, float* host_depth4) {
+// After the job's completion event: on overflow grow and replay
+// (asynchronously); else record the results. Returns 1 when finished,
+// 0 when a replay was queued, < 0 on error.
+int complete_job(rz_ctx* c) {
+    const FrameCounters fc = *c->h_cnt;
+    c->ovf_seen |= fc.overflow;
+    if (fc.overflow) {
+        if (fc.overflow & OVF_POLY) return fail(c, RZ_ERR_LIMIT, "a clipped polygon exceeded 16 vertices");
+        if (++c->replays > MAX_REPLAYS) return fail(c, RZ_ERR_LIMIT, "job did not fit after repeated growth");
+        Needs n;
+        n.prims = std::max<uint64_t>(fc.nprims, c->geom.cap);
+        n.clip = std::max<uint64_t>(fc.nclip, c->clip_tri.cap);
+        n.pairs = std::max<uint64_t>(fc.npairs, c->list.cap);
+        n.fine = std::max<uint64_t>(fc.fine_total, c->flist.cap);
+        n.mat = std::max<uint64_t>(fc.mat_len, c->bin_counts.cap);
+        n.fmat = std::max<uint64_t>(fc.fmat_len, c->fine_counts.cap);
+        if (fc.overflow & OVF_PRIMS) n.prims = std::max<uint64_t>(n.prims, c->geom.cap * 2);
+        if (fc.overflow & OVF_CLIP) n.clip = std::max<uint64_t>(n.clip, c->clip_tri.cap * 2);
+        if (fc.overflow & OVF_PAIRS) n.pairs = std::max<uint64_t>(n.pairs, c->list.cap * 2);
+        if (fc.overflow & OVF_FINE) n.fine = std::max<uint64_t>(n.fine, c->flist.cap * 2);
+        int rc = ensure_buffers(c, c->plan, n);
+        if (rc) return rc;
+        rc = c->job_graph ? enqueue_graph(c, c->plan) : enqueue_eager(c, c->plan);
+        return rc ? rc : 0;
+    }
+    const uint32_t ns = c->plan.nscenes;
+    c->scene_flags.assign(ns, 0);
+    uint32_t fl = 0;
+    for (uint32_t s = 0; s < ns; ++s) {
+        c->scene_flags[s] = c->h_runs[s].flags;
+        fl |= c->h_runs[s].flags;
+    }
+    c->frame_flags = fl;
+    c->frame_stats[0] = c->plan.tris;
+    c->frame_stats[1] = fc.nclip;
+    c->frame_stats[2] = fc.nprims_raw;
+    c->frame_stats[3] = fc.pairs_binned;
+    c->frame_stats[4] = c->replays;
+    for (int i = 0; i < 5; ++i) c->frame_stats[5 + i] = fc.rstat[i];
+    c->frame_stats[10] = c->ovf_seen;
+    if (c->prof) {
+        float a = 0, b = 0, d = 0;
+        if (c->job_graph) {
+            HIP_TRY(c, hipEventElapsedTime(&d, c->ev[0], c->ev[3]));
+            c->prof_ms[RZ_STAGE_RASTER] += d;  // the whole graph
+        } else {
+            HIP_TRY(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+            HIP_TRY(c, hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+            HIP_TRY(c, hipEventElapsedTime(&d, c->ev[2], c->ev[3]));
+            c->prof_ms[RZ_STAGE_GEOMETRY] += a;
+            c->prof_ms[RZ_STAGE_BINNING] += b;
+            c->prof_ms[RZ_STAGE_RASTER] += d;
+        }
+        c->prof_frames++;
+    }
+    return 1;
+}
+
+int start_job(rz_ctx* c, bool graph) {
+    int rc = build_job(c, c->plan);
+    if (rc) return rc;
+    c->replays = 0;
+    c->ovf_seen = 0;
+    c->in_flight = true;
+    return enqueue(c, graph);
+}
+
+int wait_job(rz_ctx* c) {
+    for (;;) {
+        HIP_TRY(c, hipEventSynchronize(c->done));
+        int rc = complete_job(c);
+        if (rc < 0) return rc;
+        if (rc == 1) return RZ_OK;
+    }
+}
+
+int frame_impl(rz_ctx* c, uint32_t* host_out, uint32_t* dev_out, uint32_t* host_color4, float* host_depth4) {
     if (!c) return RZ_ERR_ARG;
+    if (c->recording == JobKind::Batch || c->in_flight) return fail(c, RZ_ERR_STATE, "a batch is being recorded or in flight");
     int rc;
-    hipStream_t s = c->stream;
     const size_t npx = (size_t)c->W * c->H;
     uint32_t* target = dev_out;
     if (!target) {
@@ -265,88 +612,15 @@ int resolve_impl(rz_ctx* c, uint32_t* host_out, uint32_t* dev_out, uint32_t* hos
         if ((rc = grow(c, c->msaa_color, npx * 4))) return rc;
         if ((rc = grow(c, c->msaa_depth, npx * 4))) return rc;
     }
-    const uint32_t ndraws = (uint32_t)c->draw_mesh.size();
-    if (ndraws == 0) {
-        if ((rc = ensure_frame_buffers(c, 1024, 1024, 4096, 1024, nullptr))) return rc;
-        launch_frame_reset(s, c->d_cnt);
-        if (c->prof) {
-            HIP_TRY(c, hipEventRecord(c->ev[0], s));
-            HIP_TRY(c, hipEventRecord(c->ev[1], s));
-        }
-    }
-    uint32_t replays = 0;
-    uint32_t ovf_seen = 0;
-    for (;;) {
-        const uint32_t nb = c->bg.nbx * c->bg.nby;
-        const uint32_t nc_max = (uint32_t)((c->geom.cap + CHUNK - 1) / CHUNK);
-        const uint32_t bin_grid = std::min<uint32_t>((nc_max + 3) / 4, 4096);
-        launch_bin(s, false, bin_grid, c->d_cnt, c->pbin.p, c->bg, c->bin_counts.p, nullptr, nullptr, 0);
-        launch_scan(s, c->bin_counts.p, c->bin_offsets.p, nb * nc_max, &c->d_cnt->nprims, nb, CHUNK, c->parts.p,
-                    nullptr, &c->d_cnt->pairs_binned);
-        launch_bin(s, true, bin_grid, c->d_cnt, c->pbin.p, c->bg, nullptr, c->bin_offsets.p, c->list.p,
-                   (uint32_t)c->list.cap);
-        launch_fine(s, c->d_cnt, c->bin_offsets.p, c->list.p, c->chunk_base.p, c->bg, c->fine_counts.p,
-                    c->fine_off.p, c->flist.p, (uint32_t)c->flist.cap, nch_max(c), c->parts.p);
-        if (c->prof) HIP_TRY(c, hipEventRecord(c->ev[2], s));
-        launch_raster(s, c->d_cnt, c->geom.p, c->attrs.p, c->d_draws, c->chunk_base.p, c->fine_off.p, c->flist.p,
-                      (uint32_t)c->flist.cap, c->bg, c->W, c->H, c->clear_depth, target,
-                      want_msaa ? c->msaa_color.p : nullptr, want_msaa ? c->msaa_depth.p : nullptr,
-                      c->raster_stats);
-        if (c->prof) HIP_TRY(c, hipEventRecord(c->ev[3], s));
-        HIP_TRY(c, hipGetLastError());
-        HIP_TRY(c, hipMemcpyAsync(c->h_cnt, c->d_cnt, sizeof(FrameCounters), hipMemcpyDeviceToHost, s));
-        HIP_TRY(c, hipStreamSynchronize(s));
-        const FrameCounters fc = *c->h_cnt;
-        ovf_seen |= fc.overflow;
-        if (fc.overflow == 0) break;
-        if (fc.overflow & OVF_POLY) {
-            end_frame(c);
-            return fail(c, RZ_ERR_LIMIT, "a clipped polygon exceeded 16 vertices");
-        }
-        if (++replays > 4) {
-            end_frame(c);
-            return fail(c, RZ_ERR_LIMIT, "frame did not fit after repeated growth");
-        }
-        uint64_t prims_need = std::max<uint64_t>(fc.nprims, c->geom.cap);
-        uint64_t clip_need = std::max<uint64_t>(fc.clip_need, c->clip_tri.cap);
-        uint64_t pairs_need = std::max<uint64_t>(fc.npairs, c->list.cap);
-        if (fc.overflow & OVF_PRIMS) prims_need = std::max<uint64_t>(fc.nprims, c->geom.cap * 2);
-        if (fc.overflow & OVF_CLIP) clip_need = std::max<uint64_t>(fc.clip_need, c->clip_tri.cap * 2);
-        if (fc.overflow & OVF_PAIRS) pairs_need = std::max<uint64_t>(fc.npairs, c->list.cap * 2);
-        uint64_t fine_need = std::max<uint64_t>(fc.fine_total, c->flist.cap);
-        if (fc.overflow & OVF_FINE) fine_need = std::max<uint64_t>(fc.fine_total, c->flist.cap * 2);
-        if ((rc = ensure_frame_buffers(c, prims_need, clip_need, pairs_need, std::max(max_draw_tris(c), 1024u),
-                                       nullptr, fine_need))) {
-            end_frame(c);
-            return rc;
-        }
-        if (c->prof) HIP_TRY(c, hipEventRecord(c->ev[0], s));
-        if ((rc = replay_geometry(c, ndraws))) {
-            end_frame(c);
-            return rc;
-        }
-        if (c->prof) HIP_TRY(c, hipEventRecord(c->ev[1], s));
-    }
-    const FrameCounters fc = *c->h_cnt;
-    c->frame_flags = fc.flags;
-    c->frame_stats[0] = c->frame_tris;
-    c->frame_stats[1] = fc.nclipped;
-    c->frame_stats[2] = fc.nprims;
-    c->frame_stats[3] = fc.pairs_binned;
-    c->frame_stats[4] = replays;
-    for (int i = 0; i < 5; ++i) c->frame_stats[5 + i] = fc.rstat[i];
-    c->frame_stats[10] = ovf_seen;
-    if (c->prof) {
-        float a = 0, b = 0, d = 0;
-        HIP_TRY(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
-        HIP_TRY(c, hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
-        HIP_TRY(c, hipEventElapsedTime(&d, c->ev[2], c->ev[3]));
-        c->prof_ms[RZ_STAGE_GEOMETRY] += a;
-        c->prof_ms[RZ_STAGE_BINNING] += b;
-        c->prof_ms[RZ_STAGE_RASTER] += d;
-        c->prof_frames++;
-    }
-    end_frame(c);
+    c->recording = JobKind::Frame;
+    c->scenes.clear();
+    c->scenes.push_back(PendingScene{c->W, c->H, c->clear_depth, target, want_msaa ? c->msaa_color.p : nullptr,
+                                     want_msaa ? c->msaa_depth.p : nullptr});
+    rc = start_job(c, false);
+    if (!rc) rc = wait_job(c);
+    release_job(c);
+    c->clear_depth = c->next_clear_depth;
+    if (rc) return rc;
     if (host_out) HIP_TRY(c, hipMemcpy(host_out, target, npx * sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (host_color4)
         HIP_TRY(c, hipMemcpy(host_color4, c->msaa_color.p, npx * 4 * sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -367,7 +641,7 @@ int mesh_create(rz_ctx* c, const float* pos, const float* color, const float* uv
             if (idx[i] >= nv) return fail(c, RZ_ERR_ARG, "index out of range");
     }
     rz_mesh* m = new rz_mesh();
-    m->ctx = c;
+    m->pool = c->pool;
     m->nv = nv;
     m->ntri = (uint32_t)(ni / 3);
     auto up = [&](void** dst, const void* src, size_t bytes) -> int {
@@ -411,14 +685,14 @@ int mesh_create(rz_ctx* c, const float* pos, const float* color, const float* uv
     return RZ_OK;
 }
 
-}  // namespace
+int check_shaders(rz_ctx* c, int vs, int fs) {
+    if (vs != RZ_VS_MVP) return fail(c, RZ_ERR_ARG, "unknown vertex shader");
+    if (fs != RZ_FS_TEXTURE && fs != RZ_FS_COLOR && fs != RZ_FS_DEBUG)
+        return fail(c, RZ_ERR_ARG, "unknown fragment shader");
+    return RZ_OK;
+}
 
-// ===========================================================================
-// C ABI
-// ===========================================================================
-extern "C" {
-
-int rz_create(uint32_t width, uint32_t height, rz_ctx** out) {
+int create_impl(rz_pool* pool, uint32_t width, uint32_t height, rz_ctx** out) {
     if (!out) return RZ_ERR_ARG;
     *out = nullptr;
     if (width == 0 || height == 0 || width > MAX_DIM || height > MAX_DIM) return RZ_ERR_ARG;
@@ -432,42 +706,72 @@ int rz_create(uint32_t width, uint32_t height, rz_ctx** out) {
         float v = (i % 5 == 0) ? 1.0f : 0.0f;  // Uniforms::new: identities (uniform.rs:17-27)
         c->uni.world[i] = c->uni.view[i] = c->uni.proj[i] = v;
     }
-    c->bg = choose_bins(width, height, 0);
     const char* st = getenv("RZ_RASTER_STATS");
     c->raster_stats = st && st[0] == '1';
-    auto bad = [&](hipError_t e) {
-        std::string m = hipGetErrorString(e);
+    if (pool) {
+        c->pool = pool;
+    } else {
+        c->pool = new rz_pool();
+        if (hipGetDevice(&c->pool->device) != hipSuccess) {
+            delete c->pool;
+            delete c;
+            return RZ_ERR_HIP;
+        }
+    }
+    c->pool->ctxs.push_back(c);
+    auto bad = [&](hipError_t) {
         rz_destroy(c);
         return RZ_ERR_HIP;
     };
     hipError_t e;
-    if ((e = hipGetDevice(&c->device)) != hipSuccess) return bad(e);
+    int dev = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return bad(e);
+    if (dev != c->pool->device) {
+        rz_destroy(c);
+        return RZ_ERR_ARG;  // contexts sharing a pool must be on its device
+    }
+    if ((e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+        return bad(e);
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return bad(e);
-    if ((e = hipMalloc(&c->d_draws, sizeof(DrawDev) * MAX_DRAWS)) != hipSuccess) return bad(e);
-    if ((e = hipHostMalloc(&c->h_draws, sizeof(DrawDev) * MAX_DRAWS, hipHostMallocDefault)) != hipSuccess)
-        return bad(e);
-    if ((e = hipHostMalloc(&c->h_mats, sizeof(Mat3) * MAX_DRAWS, hipHostMallocDefault)) != hipSuccess)
-        return bad(e);
     if ((e = hipMalloc(&c->d_cnt, sizeof(FrameCounters))) != hipSuccess) return bad(e);
     if ((e = hipHostMalloc(&c->h_cnt, sizeof(FrameCounters), hipHostMallocDefault)) != hipSuccess) return bad(e);
     for (int i = 0; i < 4; ++i)
         if ((e = hipEventCreate(&c->ev[i])) != hipSuccess) return bad(e);
+    if ((e = hipEventCreateWithFlags(&c->done, hipEventDisableTiming)) != hipSuccess) return bad(e);
     memset(c->h_cnt, 0, sizeof(FrameCounters));
-    launch_frame_reset(c->stream, c->d_cnt);
-    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bad(e);
+    if (ensure_tables(c, 16, 4) != RZ_OK) {
+        rz_destroy(c);
+        return RZ_ERR_NOMEM;
+    }
     *out = c;
     return RZ_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int rz_create(uint32_t width, uint32_t height, rz_ctx** out) { return create_impl(nullptr, width, height, out); }
+
+int rz_create_shared(rz_ctx* parent, uint32_t width, uint32_t height, rz_ctx** out) {
+    if (!parent || !out) return RZ_ERR_ARG;
+    return create_impl(parent->pool, width, height, out);
 }
 
 int rz_destroy(rz_ctx* c) {
     if (!c) return RZ_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (Texture& t : c->textures)
-        if (t.data) (void)hipFree(t.data);
+    for (PendingDraw& d : c->draws) d.mesh->pending--;
+    c->draws.clear();
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
     release(c->cls);
     release(c->nsub);
     release(c->suboff);
     release(c->parts);
+    release(c->parts64);
     release(c->clip_tri);
     release(c->clip_n);
     release(c->clip_out);
@@ -477,21 +781,34 @@ int rz_destroy(rz_ctx* c) {
     release(c->bin_counts);
     release(c->bin_offsets);
     release(c->list);
-    release(c->chunk_base);
+    release(c->bin_rng);
+    release(c->fpack);
+    release(c->fpre);
     release(c->fine_counts);
     release(c->fine_off);
     release(c->flist);
     release(c->out);
     release(c->msaa_color);
     release(c->msaa_depth);
-    if (c->d_draws) (void)hipFree(c->d_draws);
-    if (c->h_draws) (void)hipHostFree(c->h_draws);
-    if (c->h_mats) (void)hipHostFree(c->h_mats);
+    if (c->h_job) (void)hipHostFree(c->h_job);
+    if (c->d_job) (void)hipFree(c->d_job);
+    if (c->h_runs) (void)hipHostFree(c->h_runs);
+    if (c->d_runs) (void)hipFree(c->d_runs);
     if (c->d_cnt) (void)hipFree(c->d_cnt);
     if (c->h_cnt) (void)hipHostFree(c->h_cnt);
     for (int i = 0; i < 4; ++i)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->done) (void)hipEventDestroy(c->done);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    rz_pool* pool = c->pool;
+    if (pool) {
+        pool->ctxs.erase(std::remove(pool->ctxs.begin(), pool->ctxs.end(), c), pool->ctxs.end());
+        if (pool->ctxs.empty()) {
+            for (Texture& t : pool->textures)
+                if (t.data) (void)hipFree(t.data);
+            delete pool;
+        }
+    }
     delete c;
     return RZ_OK;
 }
@@ -503,7 +820,7 @@ void* rz_stream(const rz_ctx* c) { return c ? (void*)c->stream : nullptr; }
 int rz_set_clear_depth(rz_ctx* c, float depth) {
     if (!c) return RZ_ERR_ARG;
     c->next_clear_depth = depth;
-    if (c->draw_mesh.empty()) c->clear_depth = depth;
+    if (c->draws.empty() || c->recording != JobKind::Frame) c->clear_depth = depth;
     return RZ_OK;
 }
 
@@ -518,7 +835,7 @@ int rz_set_uniforms(rz_ctx* c, const float world[16], const float view[16], cons
 
 int rz_bind_texture(rz_ctx* c, uint32_t index, const uint8_t* rgba8, uint32_t tw, uint32_t th) {
     if (!c) return RZ_ERR_ARG;
-    if (index != c->textures.size())
+    if (index != c->pool->textures.size())
         return fail(c, RZ_ERR_ARG, "texture index must equal the number of bound textures");
     if (!rgba8 || tw == 0 || th == 0) return fail(c, RZ_ERR_ARG, "empty texture");
     if ((uint64_t)tw * th >= (1ull << 31)) return fail(c, RZ_ERR_LIMIT, "texture too large");
@@ -532,9 +849,11 @@ int rz_bind_texture(rz_ctx* c, uint32_t index, const uint8_t* rgba8, uint32_t tw
         (void)hipFree(t.data);
         return fail(c, RZ_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
     }
-    c->textures.push_back(t);
+    c->pool->textures.push_back(t);
     return RZ_OK;
 }
+
+uint32_t rz_texture_count(const rz_ctx* c) { return c ? (uint32_t)c->pool->textures.size() : 0u; }
 
 int rz_mesh_create(rz_ctx* c, const float* pos, const float* color, const float* uv, uint32_t nv,
                    const uint32_t* idx, uint64_t ni, rz_mesh** out) {
@@ -549,9 +868,9 @@ int rz_mesh_create_device(rz_ctx* c, const float* pos, const float* color, const
 int rz_mesh_destroy(rz_ctx* c, rz_mesh* m) {
     if (!c) return RZ_ERR_ARG;
     if (!m) return RZ_OK;
-    if (m->ctx != c) return fail(c, RZ_ERR_ARG, "mesh belongs to another context");
-    if (m->pending) return fail(c, RZ_ERR_STATE, "mesh is used by an unresolved frame");
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (m->pool != c->pool) return fail(c, RZ_ERR_ARG, "mesh belongs to another context pool");
+    if (m->pending) return fail(c, RZ_ERR_STATE, "mesh is used by an unfinished frame or batch");
+    for (rz_ctx* o : c->pool->ctxs) HIP_TRY(c, hipStreamSynchronize(o->stream));
     if (m->pos) (void)hipFree(m->pos);
     if (m->col) (void)hipFree(m->col);
     if (m->uv) (void)hipFree(m->uv);
@@ -563,72 +882,35 @@ int rz_mesh_destroy(rz_ctx* c, rz_mesh* m) {
 int rz_draw(rz_ctx* c, const rz_mesh* mesh_c, int vs, int fs) {
     if (!c) return RZ_ERR_ARG;
     rz_mesh* mesh = const_cast<rz_mesh*>(mesh_c);
-    if (!mesh || mesh->ctx != c) return fail(c, RZ_ERR_ARG, "bad mesh");
-    if (vs != RZ_VS_MVP) return fail(c, RZ_ERR_ARG, "unknown vertex shader");
-    if (fs != RZ_FS_TEXTURE && fs != RZ_FS_COLOR && fs != RZ_FS_DEBUG)
-        return fail(c, RZ_ERR_ARG, "unknown fragment shader");
-    if (fs == RZ_FS_TEXTURE && c->textures.empty())
+    if (!mesh || mesh->pool != c->pool) return fail(c, RZ_ERR_ARG, "bad mesh");
+    int rc = check_shaders(c, vs, fs);
+    if (rc) return rc;
+    if (c->recording == JobKind::Batch || c->in_flight)
+        return fail(c, RZ_ERR_STATE, "a batch is being recorded or in flight");
+    if (fs == RZ_FS_TEXTURE && c->pool->textures.empty())
         return fail(c, RZ_ERR_STATE, "texture fragment shader with no texture bound");
-    const uint32_t d = (uint32_t)c->draw_mesh.size();
-    if (d >= MAX_DRAWS) return fail(c, RZ_ERR_LIMIT, "too many draws in one frame");
-    int rc;
-    if (d == 0) {
-        // the previous frame's resolve has finished with the pinned mirrors
-        launch_frame_reset(c->stream, c->d_cnt);
-        if (c->prof) HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
-    }
-    // capacities: rasterizer triangles ~ input triangles (+ clip fans)
-    uint64_t tris = c->frame_tris + mesh->ntri;
-    uint64_t prims_est = tris + tris / 4 + 4096;
-    uint64_t clip_est = std::max<uint64_t>(c->clip_tri.cap, mesh->ntri / 8 + 1024);
-    uint64_t pairs_est = std::max<uint64_t>(c->list.cap, prims_est * 2);
-    bool grew_prims = false;
-    if ((rc = ensure_frame_buffers(c, std::max<uint64_t>(prims_est, c->geom.cap), clip_est, pairs_est,
-                                   std::max(mesh->ntri, max_draw_tris(c)), &grew_prims)))
-        return rc;
-    DrawDev& h = c->h_draws[d];
-    memset(&h, 0, sizeof(h));
-    h.pos = mesh->pos;
-    h.col = mesh->col;
-    h.uv = mesh->uv;
-    h.idx = mesh->idx;
-    h.ntri = mesh->ntri;
-    h.fs = (uint32_t)fs;
-    if (fs == RZ_FS_TEXTURE) {  // uniforms.get_texture(0) (main.rs:69-71)
-        h.tex = c->textures[0].data;
-        h.tw = c->textures[0].tw;
-        h.th = c->textures[0].th;
-    }
-    c->h_mats[d] = c->uni;
-    c->draw_mesh.push_back(mesh);
+    if (c->draws.size() >= MAX_DRAWS) return fail(c, RZ_ERR_LIMIT, "too many draws in one frame");
+    c->recording = JobKind::Frame;
+    // uniforms.get_texture(0) (main.rs:69-71)
+    c->draws.push_back(PendingDraw{mesh, 0u, (uint32_t)fs, 0u, c->uni});
     mesh->pending++;
-    c->frame_tris = tris;
-    if (grew_prims && d > 0) {
-        // earlier draws' triangles lived in the old buffers: redo them
-        if ((rc = replay_geometry(c, d + 1))) return rc;
-    } else {
-        if ((rc = enqueue_draw_desc(c, d))) return rc;
-        enqueue_draw_geometry(c, d);
-    }
-    if (c->prof) HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
-    HIP_TRY(c, hipGetLastError());
     return RZ_OK;
 }
 
 int rz_resolve(rz_ctx* c, uint32_t* argb_out) {
     if (!c) return RZ_ERR_ARG;
-    return resolve_impl(c, argb_out, nullptr, nullptr, nullptr);
+    return frame_impl(c, argb_out, nullptr, nullptr, nullptr);
 }
 
 int rz_resolve_device(rz_ctx* c, uint32_t* argb_dev) {
     if (!c) return RZ_ERR_ARG;
     if (!argb_dev) return fail(c, RZ_ERR_ARG, "null output");
-    return resolve_impl(c, nullptr, argb_dev, nullptr, nullptr);
+    return frame_impl(c, nullptr, argb_dev, nullptr, nullptr);
 }
 
 int rz_resolve_msaa(rz_ctx* c, uint32_t* argb_out, uint32_t* color4_out, float* depth4_out) {
     if (!c) return RZ_ERR_ARG;
-    return resolve_impl(c, argb_out, nullptr, color4_out, depth4_out);
+    return frame_impl(c, argb_out, nullptr, color4_out, depth4_out);
 }
 
 uint32_t rz_frame_flags(const rz_ctx* c) { return c ? c->frame_flags : 0u; }
@@ -643,11 +925,11 @@ int rz_set_coarse_bin(rz_ctx* c, uint32_t pixels) {
     if (!c) return RZ_ERR_ARG;
     if (pixels % 16 != 0 || pixels > MAX_COARSE_PX)
         return fail(c, RZ_ERR_ARG, "coarse bin must be a multiple of 16 and at most 256 pixels (0 = automatic)");
-    if (!c->draw_mesh.empty()) return fail(c, RZ_ERR_STATE, "cannot change bins inside a frame");
-    BinGeom g = choose_bins(c->W, c->H, pixels);
-    if (g.nbx * g.nby > MAX_BINS) return fail(c, RZ_ERR_LIMIT, "more than 256 coarse bins");
+    if (!c->draws.empty()) return fail(c, RZ_ERR_STATE, "cannot change bins inside a frame");
+    const uint32_t cb = coarse_side(c->W, c->H, pixels, MAX_BINS);
+    if (((c->W + cb - 1) / cb) * ((c->H + cb - 1) / cb) > MAX_BINS)
+        return fail(c, RZ_ERR_LIMIT, "more than 256 coarse bins");
     c->coarse_req = pixels;
-    c->bg = g;
     return RZ_OK;
 }
 
@@ -663,6 +945,126 @@ int rz_profile_read(const rz_ctx* c, float ms[RZ_NSTAGES], uint32_t* frames) {
     if (!c || !ms) return RZ_ERR_ARG;
     memcpy(ms, c->prof_ms, sizeof(c->prof_ms));
     if (frames) *frames = c->prof_frames;
+    return RZ_OK;
+}
+
+// ------------------------------------------------------------------ batches
+
+int rz_batch_begin(rz_ctx* c) {
+    if (!c) return RZ_ERR_ARG;
+    if (c->in_flight) return fail(c, RZ_ERR_STATE, "previous batch still in flight (rz_batch_wait)");
+    if (c->recording != JobKind::None) return fail(c, RZ_ERR_STATE, "a frame or batch is being recorded");
+    c->recording = JobKind::Batch;
+    c->scenes.clear();
+    c->draws.clear();
+    return RZ_OK;
+}
+
+int rz_batch_scene(rz_ctx* c, uint32_t width, uint32_t height, float clear_depth, uint32_t* argb_dev,
+                   uint32_t* scene_out) {
+    if (!c) return RZ_ERR_ARG;
+    if (c->recording != JobKind::Batch || c->in_flight) return fail(c, RZ_ERR_STATE, "no batch being recorded");
+    if (!argb_dev) return fail(c, RZ_ERR_ARG, "null output");
+    if (width == 0 || height == 0 || width > MAX_DIM || height > MAX_DIM) return fail(c, RZ_ERR_ARG, "bad size");
+    if (((width + MAX_COARSE_PX - 1) / MAX_COARSE_PX) * ((height + MAX_COARSE_PX - 1) / MAX_COARSE_PX) > MAX_BINS)
+        return fail(c, RZ_ERR_LIMIT, "scene needs more than 256 coarse bins");
+    if (c->scenes.size() >= MAX_SCENES) return fail(c, RZ_ERR_LIMIT, "too many scenes in one batch");
+    if (scene_out) *scene_out = (uint32_t)c->scenes.size();
+    c->scenes.push_back(PendingScene{width, height, clear_depth, argb_dev, nullptr, nullptr});
+    return RZ_OK;
+}
+
+int rz_batch_scene_msaa(rz_ctx* c, uint32_t scene, uint32_t* color4_dev, float* depth4_dev) {
+    if (!c) return RZ_ERR_ARG;
+    if (c->recording != JobKind::Batch || c->in_flight) return fail(c, RZ_ERR_STATE, "no batch being recorded");
+    if (scene >= c->scenes.size()) return fail(c, RZ_ERR_ARG, "unknown scene");
+    c->scenes[scene].msaa_color = color4_dev;
+    c->scenes[scene].msaa_depth = depth4_dev;
+    return RZ_OK;
+}
+
+int rz_batch_draw(rz_ctx* c, uint32_t scene, const rz_mesh* mesh_c, const float world[16], const float view[16],
+                  const float proj[16], uint32_t texture, int vs, int fs) {
+    if (!c) return RZ_ERR_ARG;
+    if (c->recording != JobKind::Batch || c->in_flight) return fail(c, RZ_ERR_STATE, "no batch being recorded");
+    rz_mesh* mesh = const_cast<rz_mesh*>(mesh_c);
+    if (!mesh || mesh->pool != c->pool) return fail(c, RZ_ERR_ARG, "bad mesh");
+    if (scene >= c->scenes.size()) return fail(c, RZ_ERR_ARG, "unknown scene");
+    if (!world || !view || !proj) return fail(c, RZ_ERR_ARG, "null matrix");
+    int rc = check_shaders(c, vs, fs);
+    if (rc) return rc;
+    if (fs == RZ_FS_TEXTURE && texture >= c->pool->textures.size())
+        return fail(c, RZ_ERR_ARG, "texture index not bound");
+    if (c->draws.size() >= MAX_DRAWS) return fail(c, RZ_ERR_LIMIT, "too many draws in one batch");
+    PendingDraw d{mesh, scene, (uint32_t)fs, texture, Mat3{}};
+    memcpy(d.m.world, world, sizeof(d.m.world));
+    memcpy(d.m.view, view, sizeof(d.m.view));
+    memcpy(d.m.proj, proj, sizeof(d.m.proj));
+    c->draws.push_back(d);
+    mesh->pending++;
+    return RZ_OK;
+}
+
+int rz_batch_submit(rz_ctx* c) {
+    if (!c) return RZ_ERR_ARG;
+    if (c->recording != JobKind::Batch || c->in_flight) return fail(c, RZ_ERR_STATE, "no batch being recorded");
+    if (c->scenes.empty()) return fail(c, RZ_ERR_STATE, "empty batch");
+    // group draws by scene, keeping each scene's submission order
+    std::stable_sort(c->draws.begin(), c->draws.end(),
+                     [](const PendingDraw& a, const PendingDraw& b) { return a.scene < b.scene; });
+    int rc = start_job(c, c->use_graph);
+    if (rc) {
+        release_job(c);
+        return rc;
+    }
+    return RZ_OK;
+}
+
+int rz_batch_query(rz_ctx* c) {
+    if (!c) return RZ_ERR_ARG;
+    if (!c->in_flight) return 1;
+    hipError_t e = hipEventQuery(c->done);
+    if (e == hipErrorNotReady) return 0;
+    if (e != hipSuccess) {
+        release_job(c);
+        return fail(c, RZ_ERR_HIP, std::string("hipEventQuery: ") + hipGetErrorString(e));
+    }
+    int rc = complete_job(c);
+    if (rc < 0 || rc == 1) release_job(c);
+    return rc;
+}
+
+int rz_batch_wait(rz_ctx* c) {
+    if (!c) return RZ_ERR_ARG;
+    if (!c->in_flight) return RZ_OK;
+    int rc = wait_job(c);
+    release_job(c);
+    return rc;
+}
+
+int rz_batch_scene_flags(const rz_ctx* c, uint32_t scene, uint32_t* flags) {
+    if (!c || !flags) return RZ_ERR_ARG;
+    if (scene >= c->scene_flags.size()) return RZ_ERR_ARG;
+    *flags = c->scene_flags[scene];
+    return RZ_OK;
+}
+
+int rz_batch_tuning(rz_ctx* c, uint32_t bin_target, uint32_t raster_per_cu, int use_graph) {
+    if (!c) return RZ_ERR_ARG;
+    if (bin_target == 0 || bin_target > MAX_BINS) return fail(c, RZ_ERR_ARG, "bin target must be in [1, 256]");
+    if (raster_per_cu > 32) return fail(c, RZ_ERR_ARG, "at most 32 raster workgroups per CU");
+    if (c->recording != JobKind::None || c->in_flight) return fail(c, RZ_ERR_STATE, "not between jobs");
+    c->bin_target = bin_target;
+    c->raster_per_cu = raster_per_cu;
+    c->use_graph = use_graph != 0;
+    return RZ_OK;
+}
+
+int rz_batch_get_tuning(const rz_ctx* c, uint32_t out[3]) {
+    if (!c || !out) return RZ_ERR_ARG;
+    out[0] = c->bin_target;
+    out[1] = c->raster_per_cu;
+    out[2] = c->use_graph ? 1u : 0u;
     return RZ_OK;
 }
 

@@ -1,7 +1,15 @@
 // rz_kernels.h -- shared layouts and kernel launchers of the gfx950 rasterizer.
+//
+// Unit of work: a job = S scenes (render targets) and D draws (meshes with
+// their uniforms and fragment shader), each draw belonging to one scene. A
+// plain rz_draw/rz_resolve frame is a job with one scene; a batch is a job
+// with many. Every kernel reads its sizes from the device (job header and
+// frame counters), so the same launch sequence serves any job that fits the
+// buffers -- which is what lets a batch be captured once as a HIP graph.
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "../../include/rz.h"
@@ -13,43 +21,91 @@ constexpr float FLT_MAX_F = 3.40282347e+38f;   // f32::MAX
 constexpr uint32_t MAXV = 16;         // vertices kept per clipped polygon
 constexpr uint32_t CLIP_LDS_V = 66;   // polygon vertices a clip wave can stage
 constexpr uint32_t CHUNK = 256;       // prims per binning chunk (one wave)
-constexpr uint32_t MAX_BINS = 256;    // coarse bins per frame (4 per lane)
+constexpr uint32_t MAX_BINS = 256;    // coarse bins per scene (4 per lane)
+constexpr uint32_t TRI_ALIGN = 256;   // draws start at multiples of this in the triangle index space
 constexpr uint32_t ATTR_STRIDE = 12;  // floats of attributes per prim
 constexpr int SCAN_ITEMS = 8;
 constexpr uint32_t SCAN_TILE = 256 * SCAN_ITEMS;
+constexpr uint32_t NO_TILE = 0xFFFFFFFFu;
 
-// overflow bits: the frame must be replayed with larger buffers
-constexpr uint32_t OVF_PRIMS = 1, OVF_CLIP = 2, OVF_PAIRS = 4, OVF_POLY = 8, OVF_FINE = 16;
+// overflow bits: the job must be replayed with larger buffers
+constexpr uint32_t OVF_PRIMS = 1, OVF_CLIP = 2, OVF_PAIRS = 4, OVF_POLY = 8, OVF_FINE = 16, OVF_MATRIX = 32;
 
 struct FrameCounters {
-    uint32_t nprims;        // rasterizer triangles of the frame so far
-    uint32_t nclip;         // clip slots used by the current draw
-    uint32_t clip_need;     // most clip slots any draw of the frame asked for
+    uint32_t nprims;        // prim slots of the job (each scene starts at a chunk boundary)
+    uint32_t nprims_raw;    // rasterizer triangles of the job
+    uint32_t nclip;         // clip slots asked for
     uint32_t overflow;      // OVF_* bits
-    uint32_t flags;         // RZ_FLAG_* bits
-    uint32_t nclipped;      // input triangles sent to the clipper
-    unsigned long long npairs;  // coarse-bin entries counted by k_setup
+    uint32_t nchunks;       // coarse-binning chunks (sum over scenes)
+    uint32_t mat_len;       // coarse count matrix length (sum over scenes of bins * chunks)
     uint32_t pairs_binned;  // entries placed by the coarse bin scan
-    uint32_t nch;           // chunks of the coarse lists (fine binning)
     uint32_t fine_total;    // entries placed by the fine bin scan
-    uint32_t pad;
+    unsigned long long npairs;  // coarse-bin entries counted by k_setup
+    uint32_t fine_chunks;   // chunks of the coarse lists (fine binning work items)
+    uint32_t fmat_len;      // fine count matrix length (written with fine_chunks as one u64)
+    uint32_t q[8];          // raster work queue heads, one per XCD band
     // raster work counters (stats variant only): list entries scanned,
     // candidates after the bbox / depth / edge culls, hits processed,
     // hits with coverage, hits that shaded
     unsigned long long rstat[5];
 };
+static_assert(offsetof(FrameCounters, fine_chunks) % 8 == 0 &&
+                  offsetof(FrameCounters, fmat_len) == offsetof(FrameCounters, fine_chunks) + 4,
+              "fine_chunks | fmat_len << 32 is stored as one u64");
 
-// Per-draw state on the device (uniforms + mesh streams + fragment shader).
+struct JobHead {
+    uint32_t ndraws, nscenes;
+    uint32_t ntri;    // padded triangle index space (sum of draws rounded up to TRI_ALIGN)
+    uint32_t ntiles;  // 16x16 raster tiles of all scenes
+    uint32_t nbins;   // coarse bins of all scenes
+    uint32_t pad[3];
+};
+
+// Per-draw state on the device (mesh streams, fragment shader, MVP).
 struct DrawDev {
     float M[16];             // ((P*V)*W), written by k_mvp
     const float* pos;        // nv*3
     const float* col;        // nv*4 or null
     const float* uv;         // nv*2 or null
     const uint32_t* idx;     // ntri*3
-    const uint32_t* tex;     // texture 0 as RGBA8 words, for RZ_FS_TEXTURE
+    const uint32_t* tex;     // the draw's texture as RGBA8 words, for RZ_FS_TEXTURE
     uint32_t tw, th;
     uint32_t ntri;
     uint32_t fs;
+    uint32_t scene;          // owning scene
+    uint32_t tri_base;       // first index of the draw in the padded triangle space
+};
+
+struct Mat3 {
+    float world[16], view[16], proj[16];
+};
+
+// Per-scene configuration, written by the host.
+struct SceneCfg {
+    uint32_t W, H;
+    float clear_depth;
+    uint32_t cb, nbx, nby;    // coarse bin side (multiple of 16, <= 256) and grid
+    uint32_t tile_base;       // first global 16x16 raster tile of the scene
+    uint32_t bin_base;        // first global coarse bin of the scene
+    uint32_t tri_begin, tri_end;  // padded triangle range of the scene's draws
+    uint32_t pad0, pad1;
+    uint32_t* out;            // resolved ARGB, W*H words (device)
+    uint32_t* msaa_color;     // optional W*H*4 pre-resolve colours
+    float* msaa_depth;        // optional W*H*4 pre-resolve depths
+    uint64_t pad2;
+};
+static_assert(sizeof(SceneCfg) == 80, "SceneCfg layout");
+
+// Per-scene values the device derives each frame.
+struct SceneRun {
+    uint32_t prim_off;    // rasterizer triangles of all earlier scenes
+    uint32_t nprims;      // rasterizer triangles of this scene
+    uint32_t prim_base;   // first prim slot (multiple of CHUNK)
+    uint32_t chunk_base;  // prim_base / CHUNK
+    uint32_t nch;         // ceil(nprims / CHUNK)
+    uint32_t mat_base;    // first entry of the scene's coarse count matrix
+    uint32_t flags;       // RZ_FLAG_* raised while rendering the scene
+    uint32_t pad;
 };
 
 // One rasterizer triangle (RasterizerTriangle, rasterizer/mod.rs:177-184)
@@ -68,44 +124,63 @@ struct PrimGeom {
 };
 static_assert(sizeof(PrimGeom) == 112, "PrimGeom layout");
 
-// Per-prim binning record and coarse-bin list entry: {prim, bbx, bby, zlo bits}.
-// (uint4; for the per-prim record .x is unused.)
+// Per-prim binning record and bin list entry: uint4 {prim, bbx, bby, zlo bits}
+// (for the per-prim record .x is unused).
 
-struct BinGeom {
-    uint32_t cb;        // coarse bin side in pixels (multiple of 16)
-    uint32_t nbx, nby;  // bins across / down
+// Device pointers of one job's tables and working buffers.
+struct JobBufs {
+    const JobHead* head;
+    DrawDev* draws;
+    const Mat3* mats;
+    const SceneCfg* scenes;
+    SceneRun* runs;
+    FrameCounters* cnt;
+    // geometry
+    uint32_t* cls;
+    uint32_t* nsub;
+    uint32_t* suboff;
+    uint32_t* clip_tri;
+    uint32_t* clip_n;
+    float* clip_out;
+    PrimGeom* geom;
+    float* attrs;
+    uint4* pbin;
+    // coarse bins
+    uint32_t* bin_counts;
+    uint32_t* bin_offsets;
+    uint4* list;
+    // fine bins
+    uint2* bin_rng;              // coarse list range of each global bin
+    unsigned long long* fpack;   // per global bin: T * chunks << 32 | chunks
+    unsigned long long* fpre;    // its exclusive scan
+    uint32_t* fine_counts;
+    uint32_t* fine_off;
+    uint4* flist;
+    // scan partials
+    uint32_t* parts;
+    unsigned long long* parts64;
+    // capacities (elements)
+    uint32_t tri_cap, clip_cap, prim_cap, pair_cap, mat_cap, fine_cap, fmat_cap, bins_cap;
 };
 
-struct Mat3 {
-    float world[16], view[16], proj[16];
+// Grid sizes of one launch sequence. Kernels loop over their work with the
+// device counts, so any grid >= 1 is correct; larger grids only add idle
+// workgroups.
+struct JobGrid {
+    uint32_t draws;      // k_mvp blocks
+    uint32_t tris;       // k_classify / k_setup blocks
+    uint32_t clip;       // k_clip waves
+    uint32_t bins;       // k_bin blocks
+    uint32_t fine;       // k_fine_bin blocks
+    uint32_t raster;     // raster workgroups; 0 = one per tile (static order)
+    uint32_t ntiles;     // tiles (for the static raster order)
+    uint32_t scenes;     // scene count (k_frame_reset / k_scene_prep loops)
 };
 
 // ---- launchers (all asynchronous on `s`) ---------------------------------
 void launch_max_index(hipStream_t s, const uint32_t* idx, uint64_t n, uint32_t* out);  // *out must start at 0
-void launch_frame_reset(hipStream_t s, FrameCounters* cnt);
-void launch_draw_reset(hipStream_t s, FrameCounters* cnt);
-void launch_mvp(hipStream_t s, DrawDev* draws, uint32_t d, const Mat3& m);
-void launch_classify(hipStream_t s, const DrawDev* draws, uint32_t d, uint32_t ntri, FrameCounters* cnt,
-                     uint32_t* cls, uint32_t* nsub, uint32_t* clip_tri, uint32_t clip_cap);
-void launch_clip(hipStream_t s, const DrawDev* draws, uint32_t d, FrameCounters* cnt, const uint32_t* clip_tri,
-                 uint32_t clip_cap, uint32_t* nsub, float* clip_out, uint32_t* clip_n);
-// exclusive scan; with n_dev the length is mul * ceil(*n_dev / div), at most
-// n_host (which also sizes the grid).
-void launch_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uint32_t n_host, const uint32_t* n_dev,
-                 uint32_t mul, uint32_t div, uint32_t* parts, uint32_t* base_total, uint32_t* total2);
-void launch_setup(hipStream_t s, const DrawDev* draws, uint32_t d, uint32_t ntri, FrameCounters* cnt,
-                  const uint32_t* cls, const uint32_t* nsub, const uint32_t* sub_off, const float* clip_out,
-                  uint32_t clip_cap, PrimGeom* geom, float* attrs, uint4* pbin, uint32_t prim_cap, uint32_t W,
-                  uint32_t H, BinGeom bg);
-void launch_bin(hipStream_t s, bool scatter, uint32_t grid, FrameCounters* cnt, const uint4* pbin, BinGeom bg,
-                uint32_t* counts, const uint32_t* offsets, uint4* list, uint32_t pair_cap);
-// fine binning of the coarse lists into 16x16-tile lists (nch_max bounds the chunk count)
-void launch_fine(hipStream_t s, FrameCounters* cnt, const uint32_t* coff, const uint4* clist, uint32_t* chunk_base,
-                 BinGeom bg, uint32_t* fcounts, uint32_t* foff, uint4* flist, uint32_t fcap, uint32_t nch_max,
-                 uint32_t* parts);
-void launch_raster(hipStream_t s, FrameCounters* cnt, const PrimGeom* geom, const float* attrs,
-                   const DrawDev* draws, const uint32_t* chunk_base, const uint32_t* foff, const uint4* list,
-                   uint32_t fcap, BinGeom bg, uint32_t W, uint32_t H, float clear_depth, uint32_t* out,
-                   uint32_t* msaa_color, float* msaa_depth, bool stats);
+// the whole pipeline of one job, from counter reset to resolved outputs
+void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_stats, hipEvent_t ev_bin,
+                hipEvent_t ev_raster);
 
 }  // namespace rzk

@@ -5,16 +5,23 @@
 // same bits as the reference's Rust expression it cites (no FMA fusion,
 // same association, Rust's saturating float->int casts).
 //
-// Pipeline of one frame (all on the context stream):
-//   per draw   k_mvp -> k_classify -> k_clip -> scan(nsub) -> k_setup
-//   at resolve k_bin<count> -> scan(bins) -> k_bin<scatter> -> k_raster
-// Rasterizer triangles ("prims") are stored in submission order
-// (input triangle, then clip-fan index), so every later stage can keep the
-// reference's order per pixel without sorting.
+// Pipeline of one job (S scenes, D draws; all on one stream):
+//   k_frame_reset -> k_mvp -> k_classify -> k_clip -> scan(nsub)
+//   -> k_scene_prep -> k_setup                                  (geometry)
+//   -> k_bin<count> -> scan -> k_bin<scatter>                   (coarse bins)
+//   -> k_fine_prep -> scan64 -> k_fine_bin<count> -> scan
+//   -> k_fine_bin<scatter>                                      (16x16 tiles)
+//   -> k_raster (coverage, depth, shading, resolve)
+// Rasterizer triangles ("prims") are stored per scene in submission order
+// (draw, input triangle, clip-fan index), so every later stage keeps the
+// reference's order per pixel without sorting. Each scene's prims start at a
+// CHUNK boundary, so a binning chunk never mixes scenes.
 
 #include "rz_kernels.h"
 
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 namespace rzk {
 
@@ -46,6 +53,64 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
 __device__ __forceinline__ uint32_t uniform(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
+__device__ __forceinline__ uint32_t lo32(unsigned long long v) { return (uint32_t)v; }
+__device__ __forceinline__ uint32_t hi32(unsigned long long v) { return (uint32_t)(v >> 32); }
+
+// ---------------------------------------------------------------------------
+// Table lookups (binary searches). Called with a wave-uniform argument the
+// compiler keeps the loads scalar; k_fine_prep calls find_scene_bin per lane.
+// ---------------------------------------------------------------------------
+
+// draw holding padded triangle t: largest d < nd with draws[d].tri_base <= t
+__device__ __forceinline__ uint32_t find_draw(const DrawDev* __restrict__ draws, uint32_t nd, uint32_t t) {
+    uint32_t lo = 0, hi = nd;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (draws[mid].tri_base <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+// scene holding global coarse-binning chunk c (scenes without prims share
+// their successor's chunk_base; the largest index is the owner)
+__device__ __forceinline__ uint32_t find_scene_chunk(const SceneRun* __restrict__ runs, uint32_t ns, uint32_t c) {
+    uint32_t lo = 0, hi = ns;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (runs[mid].chunk_base <= c) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ uint32_t find_scene_tile(const SceneCfg* __restrict__ sc, uint32_t ns, uint32_t t) {
+    uint32_t lo = 0, hi = ns;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (sc[mid].tile_base <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ uint32_t find_scene_bin(const SceneCfg* __restrict__ sc, uint32_t ns, uint32_t g) {
+    uint32_t lo = 0, hi = ns;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (sc[mid].bin_base <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+// global coarse bin holding fine-binning chunk gc
+__device__ __forceinline__ uint32_t find_bin_fchunk(const unsigned long long* __restrict__ fpre, uint32_t nb,
+                                                    uint32_t gc) {
+    uint32_t lo = 0, hi = nb;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (lo32(fpre[mid]) <= gc) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
 
 // ---------------------------------------------------------------------------
 // Vertex stage: ((P*V)*W) * (x, y, z, 1) with the reference's dot order
@@ -62,23 +127,26 @@ __device__ __forceinline__ float4 mat_vec_p(const float* __restrict__ M, float x
 }
 
 // main.rs:147-152: projection * view * world, evaluated left to right
-// (matrix.rs:56-79). One thread; written into the draw descriptor.
-__global__ void k_mvp(DrawDev* __restrict__ draws, uint32_t d, Mat3 m) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    float pv[16], out[16];
-    for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 4; ++j) {
-            float s = 0.0f;
-            for (int k = 0; k < 4; ++k) s = s + m.proj[i * 4 + k] * m.view[k * 4 + j];
-            pv[i * 4 + j] = s;
-        }
-    for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 4; ++j) {
-            float s = 0.0f;
-            for (int k = 0; k < 4; ++k) s = s + pv[i * 4 + k] * m.world[k * 4 + j];
-            out[i * 4 + j] = s;
-        }
-    for (int i = 0; i < 16; ++i) draws[d].M[i] = out[i];
+// (matrix.rs:56-79). One thread per draw; written into the draw descriptor.
+__global__ __launch_bounds__(256) void k_mvp(JobBufs b) {
+    const uint32_t nd = b.head->ndraws;
+    for (uint32_t d = blockIdx.x * blockDim.x + threadIdx.x; d < nd; d += gridDim.x * blockDim.x) {
+        const Mat3& m = b.mats[d];
+        float pv[16], out[16];
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) {
+                float s = 0.0f;
+                for (int k = 0; k < 4; ++k) s = s + m.proj[i * 4 + k] * m.view[k * 4 + j];
+                pv[i * 4 + j] = s;
+            }
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) {
+                float s = 0.0f;
+                for (int k = 0; k < 4; ++k) s = s + pv[i * 4 + k] * m.world[k * 4 + j];
+                out[i * 4 + j] = s;
+            }
+        for (int i = 0; i < 16; ++i) b.draws[d].M[i] = out[i];
+    }
 }
 
 __device__ __forceinline__ void load_vertex(const DrawDev& dd, uint32_t vi, float4& p, float a[6]) {
@@ -99,102 +167,109 @@ __device__ __forceinline__ void load_vertex(const DrawDev& dd, uint32_t vi, floa
 }
 
 // ---------------------------------------------------------------------------
-// Frame bookkeeping kernels (single thread; no memset in the stream)
+// Job start: zero the counters and the per-scene flags (a kernel, not a
+// memset, so the sequence can be captured in a graph).
 // ---------------------------------------------------------------------------
-__global__ void k_frame_reset(FrameCounters* __restrict__ cnt) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        cnt->nprims = 0;
-        cnt->nclip = 0;
-        cnt->clip_need = 0;
-        cnt->overflow = 0;
-        cnt->flags = 0;
-        cnt->nclipped = 0;
-        cnt->npairs = 0;
-        cnt->pairs_binned = 0;
-        cnt->nch = 0;
-        cnt->fine_total = 0;
-        for (int i = 0; i < 5; ++i) cnt->rstat[i] = 0;
+__global__ __launch_bounds__(256) void k_frame_reset(JobBufs b) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        FrameCounters* c = b.cnt;
+        c->nprims = 0;
+        c->nprims_raw = 0;
+        c->nclip = 0;
+        c->overflow = 0;
+        c->nchunks = 0;
+        c->mat_len = 0;
+        c->pairs_binned = 0;
+        c->fine_chunks = 0;
+        c->npairs = 0;
+        c->fmat_len = 0;
+        c->fine_total = 0;
+        for (int i = 0; i < 8; ++i) c->q[i] = 0;
+        for (int i = 0; i < 5; ++i) c->rstat[i] = 0;
     }
-}
-__global__ void k_draw_reset(FrameCounters* __restrict__ cnt) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) cnt->nclip = 0;
+    const uint32_t ns = b.head->nscenes;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x)
+        b.runs[s].flags = 0;
 }
 
 // ---------------------------------------------------------------------------
-// k_classify: one thread per input triangle.
+// k_classify: one thread per input triangle, one workgroup per TRI_ALIGN
+// triangles of one draw.
 // render.rs:104-108 (VS), render.rs:75-96 (assembly), clipping.rs:62-112
 // (degenerate cull, trivial reject / accept). Triangles that need the
 // Sutherland-Hodgman clipper get a slot in the clip list.
-// cls[t]: 0 = Outside, 1 = Inside, 2 + slot = clipped.
+// cls[t]: 0 = Outside (or padding), 1 = Inside, 2 + slot = clipped.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_classify(const DrawDev* __restrict__ draws, uint32_t d,
-                                                  FrameCounters* __restrict__ cnt,
-                                                  uint32_t* __restrict__ cls, uint32_t* __restrict__ nsub,
-                                                  uint32_t* __restrict__ clip_tri, uint32_t clip_cap) {
-    const DrawDev& dd = draws[d];
-    uint32_t ntri = dd.ntri;
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    bool live = t < ntri;
-    uint32_t c = 0;
-    if (live) {
-        float4 v[3];
-        for (int k = 0; k < 3; ++k) {
-            uint32_t vi = dd.idx[(size_t)t * 3 + k];
-            const float* pos = dd.pos + (size_t)vi * 3;
-            v[k] = mat_vec_p(dd.M, pos[0], pos[1], pos[2]);
-        }
-        // clipping.rs:63-65 with mod.rs:15-21 on clip-space xy
-        float v10x = v[1].x - v[0].x, v10y = v[1].y - v[0].y;
-        float v20x = v[2].x - v[0].x, v20y = v[2].y - v[0].y;
-        float area2 = v10x * v20y - v20x * v10y;
-        if (fabsf(area2) < 0.000001f) {
-            c = 0;
-        } else {
-            bool in_all = true, out_any = false;
-            bool ox0 = true, ox1 = true, oy0 = true, oy1 = true, oz0 = true, oz1 = true;
-            for (int k = 0; k < 3; ++k) {
-                float x = v[k].x, y = v[k].y, z = v[k].z, w = v[k].w;
-                in_all = in_all && (x >= -w) && (y >= -w) && (z >= -w) && (x <= w) && (y <= w) && (z <= w);
-                ox0 = ox0 && (x < -w);
-                oy0 = oy0 && (y < -w);
-                oz0 = oz0 && (z < -w);
-                ox1 = ox1 && (x > w);
-                oy1 = oy1 && (y > w);
-                oz1 = oz1 && (z > w);
-            }
-            out_any = ox0 || oy0 || oz0 || ox1 || oy1 || oz1;
-            c = out_any ? 0u : (in_all ? 1u : 2u);
-        }
-    }
-    // one clip-list reservation per workgroup: wave counts -> LDS -> one atomic
+__global__ __launch_bounds__(256) void k_classify(JobBufs b) {
     __shared__ uint32_t wcount[4], wbase[4];
-    const bool need = live && c == 2;
-    const uint64_t m = __ballot(need);
-    const uint32_t wave = threadIdx.x >> 6;
-    if (lane_id() == 0) wcount[wave] = (uint32_t)__popcll(m);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t tot = wcount[0] + wcount[1] + wcount[2] + wcount[3];
-        uint32_t b = tot ? atomicAdd(&cnt->nclip, tot) : 0u;
-        for (int w = 0; w < 4; ++w) {
-            wbase[w] = b;
-            b += wcount[w];
+    const uint32_t NT = b.head->ntri, nd = b.head->ndraws;
+    for (uint32_t blk = blockIdx.x; blk * 256u < NT; blk += gridDim.x) {
+        const uint32_t t = blk * 256u + threadIdx.x;
+        const uint32_t d = uniform(find_draw(b.draws, nd, blk * 256u));
+        const DrawDev& dd = b.draws[d];
+        const uint32_t lt = t - dd.tri_base;
+        const bool live = lt < dd.ntri;
+        uint32_t c = 0;
+        if (live) {
+            float4 v[3];
+            for (int k = 0; k < 3; ++k) {
+                uint32_t vi = dd.idx[(size_t)lt * 3 + k];
+                const float* pos = dd.pos + (size_t)vi * 3;
+                v[k] = mat_vec_p(dd.M, pos[0], pos[1], pos[2]);
+            }
+            // clipping.rs:63-65 with mod.rs:15-21 on clip-space xy
+            float v10x = v[1].x - v[0].x, v10y = v[1].y - v[0].y;
+            float v20x = v[2].x - v[0].x, v20y = v[2].y - v[0].y;
+            float area2 = v10x * v20y - v20x * v10y;
+            if (fabsf(area2) < 0.000001f) {
+                c = 0;
+            } else {
+                bool in_all = true, out_any = false;
+                bool ox0 = true, ox1 = true, oy0 = true, oy1 = true, oz0 = true, oz1 = true;
+                for (int k = 0; k < 3; ++k) {
+                    float x = v[k].x, y = v[k].y, z = v[k].z, w = v[k].w;
+                    in_all = in_all && (x >= -w) && (y >= -w) && (z >= -w) && (x <= w) && (y <= w) && (z <= w);
+                    ox0 = ox0 && (x < -w);
+                    oy0 = oy0 && (y < -w);
+                    oz0 = oz0 && (z < -w);
+                    ox1 = ox1 && (x > w);
+                    oy1 = oy1 && (y > w);
+                    oz1 = oz1 && (z > w);
+                }
+                out_any = ox0 || oy0 || oz0 || ox1 || oy1 || oz1;
+                c = out_any ? 0u : (in_all ? 1u : 2u);
+            }
         }
-    }
-    __syncthreads();
-    const uint32_t slot = wbase[wave] + mbcnt(m);
-    if (live) {
-        if (need) {
-            cls[t] = 2u + slot;
-            nsub[t] = 0;  // written by k_clip
-            if (slot < clip_cap)
-                clip_tri[slot] = t;
-            else
-                atomicOr(&cnt->overflow, OVF_CLIP);
-        } else {
-            cls[t] = c;
-            nsub[t] = c;  // Inside -> 1 rasterizer triangle, Outside -> 0
+        // one clip-list reservation per workgroup: wave counts -> LDS -> one atomic
+        const bool need = live && c == 2;
+        const uint64_t m = __ballot(need);
+        const uint32_t wave = threadIdx.x >> 6;
+        if (lane_id() == 0) wcount[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+            uint32_t base = tot ? atomicAdd(&b.cnt->nclip, tot) : 0u;
+            for (int w = 0; w < 4; ++w) {
+                wbase[w] = base;
+                base += wcount[w];
+            }
         }
+        __syncthreads();
+        const uint32_t slot = wbase[wave] + mbcnt(m);
+        if (t < NT) {
+            if (need) {
+                b.cls[t] = 2u + slot;
+                b.nsub[t] = 0;  // written by k_clip
+                if (slot < b.clip_cap)
+                    b.clip_tri[slot] = t;
+                else
+                    atomicOr(&b.cnt->overflow, OVF_CLIP);
+            } else {
+                b.cls[t] = c;
+                b.nsub[t] = c;  // Inside -> 1 rasterizer triangle, Outside / padding -> 0
+            }
+        }
+        __syncthreads();  // wcount / wbase are reused by the next iteration
     }
 }
 
@@ -211,26 +286,19 @@ struct ClipV {
     float a[6];
 };
 
-__global__ __launch_bounds__(64) void k_clip(const DrawDev* __restrict__ draws, uint32_t d,
-                                             FrameCounters* __restrict__ cnt,
-                                             const uint32_t* __restrict__ clip_tri, uint32_t clip_cap,
-                                             uint32_t* __restrict__ nsub, float* __restrict__ clip_out,
-                                             uint32_t* __restrict__ clip_n) {
+__global__ __launch_bounds__(64) void k_clip(JobBufs b) {
     __shared__ float sh[CLIP_LDS_V * 10];
-    const DrawDev& dd = draws[d];
     const uint32_t lane = threadIdx.x;
-    uint32_t nslots = min(uniform(cnt->nclip), clip_cap);
-    if (blockIdx.x == 0 && lane == 0) {  // per-draw bookkeeping of the clip list
-        uint32_t used = cnt->nclip;
-        cnt->clip_need = max(cnt->clip_need, used);
-        cnt->nclipped += used;
-    }
-    if (uniform(cnt->overflow) != 0) return;
+    const uint32_t nd = b.head->ndraws;
+    const uint32_t nslots = min(uniform(b.cnt->nclip), b.clip_cap);
     for (uint32_t s = blockIdx.x; s < nslots; s += gridDim.x) {
-        uint32_t t = clip_tri[s];
+        const uint32_t t = uniform(b.clip_tri[s]);
+        const uint32_t d = uniform(find_draw(b.draws, nd, t));
+        const DrawDev& dd = b.draws[d];
+        const uint32_t lt = t - dd.tri_base;
         ClipV v;
         if (lane < 3) {
-            uint32_t vi = dd.idx[(size_t)t * 3 + lane];
+            uint32_t vi = dd.idx[(size_t)lt * 3 + lane];
             float4 p;
             load_vertex(dd, vi, p, v.a);
             v.x = p.x; v.y = p.y; v.z = p.z; v.w = p.w;
@@ -304,48 +372,50 @@ __global__ __launch_bounds__(64) void k_clip(const DrawDev* __restrict__ draws, 
         }
         uint32_t ntris = 0;
         if (poly_overflow || n > MAXV) {
-            if (lane == 0) atomicOr(&cnt->overflow, OVF_POLY);
+            if (lane == 0) atomicOr(&b.cnt->overflow, OVF_POLY);
         } else if (n >= 3) {
             ntris = n - 2;
             if (lane < n) {
-                float* q = clip_out + ((size_t)s * MAXV + lane) * 10;
+                float* q = b.clip_out + ((size_t)s * MAXV + lane) * 10;
                 q[0] = v.x; q[1] = v.y; q[2] = v.z; q[3] = v.w;
                 for (int j = 0; j < 6; ++j) q[4 + j] = v.a[j];
             }
         } else if (n > 0) {
-            if (lane == 0) atomicOr(&cnt->flags, RZ_FLAG_CLIP_FEWER);
+            if (lane == 0) atomicOr(&b.runs[dd.scene].flags, RZ_FLAG_CLIP_FEWER);
         }
         if (lane == 0) {
-            nsub[t] = ntris;
-            clip_n[s] = n;
+            b.nsub[t] = ntris;
+            b.clip_n[s] = n;
         }
         __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
-// Exclusive scan of u32 (3 launches: tile sums, scan of sums, tile scan).
-// Element count is n_host, or (when n_dev is set) bins * ceil(*n_dev / CHUNK).
+// Exclusive scan of u32 or u64 (3 launches: tile sums, scan of sums, tile
+// scan). The length is *n_dev when given (bounded by n_host, which also sizes
+// the grid), else n_host. The grand total goes to *total (if given).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t scan_len(uint32_t n_host, const uint32_t* n_dev, uint32_t mul, uint32_t div) {
+__device__ __forceinline__ uint32_t scan_len(uint32_t n_host, const uint32_t* n_dev) {
     if (n_dev == nullptr) return n_host;
     uint32_t s = uniform(*n_dev);
-    uint64_t n = (uint64_t)mul * ((s + div - 1) / div);
-    return n < n_host ? (uint32_t)n : n_host;  // never past the buffers the host sized
+    return s < n_host ? s : n_host;  // never past the buffers the host sized
 }
 
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds, uint32_t* total) {
+template <class T>
+__device__ __forceinline__ T block_exclusive_scan(T v, T* lds, T* total) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t x = v;
+    T x = v;
     for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(x, o);
+        T y = __shfl_up(x, o);
         if (lane >= (uint32_t)o) x += y;
     }
     if (lane == 63) lds[wave] = x;
     __syncthreads();
-    uint32_t nw = blockDim.x >> 6, pre = 0, tot = 0;
+    uint32_t nw = blockDim.x >> 6;
+    T pre = 0, tot = 0;
     for (uint32_t w = 0; w < nw; ++w) {
-        uint32_t s = lds[w];
+        T s = lds[w];
         if (w < wave) pre += s;
         tot += s;
     }
@@ -354,64 +424,58 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* l
     return pre + x - v;
 }
 
-__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ in, uint32_t n_host,
-                                                     const uint32_t* __restrict__ n_dev, uint32_t mul, uint32_t div,
-                                                     uint32_t* __restrict__ parts) {
-    __shared__ uint32_t lds[8];
-    uint32_t n = scan_len(n_host, n_dev, mul, div);
+template <class T>
+__global__ __launch_bounds__(256) void k_scan_reduce(const T* __restrict__ in, uint32_t n_host,
+                                                     const uint32_t* __restrict__ n_dev, T* __restrict__ parts) {
+    __shared__ T lds[8];
+    uint32_t n = scan_len(n_host, n_dev);
     uint32_t base = blockIdx.x * SCAN_TILE;
     if (base >= n) return;
-    uint32_t s = 0;
+    T s = 0;
     for (int i = 0; i < SCAN_ITEMS; ++i) {
         uint32_t k = base + threadIdx.x * SCAN_ITEMS + i;
         if (k < n) s += in[k];
     }
-    uint32_t tot;
-    block_exclusive_scan(s, lds, &tot);
+    T tot;
+    block_exclusive_scan<T>(s, lds, &tot);
     if (threadIdx.x == 0) parts[blockIdx.x] = tot;
 }
 
-// one block: exclusive scan of parts, plus base (*base_total, if given);
-// writes the grand total back to *base_total and to *total2 (if given).
-__global__ __launch_bounds__(1024) void k_scan_top(uint32_t* __restrict__ parts, uint32_t n_host,
-                                                   const uint32_t* __restrict__ n_dev, uint32_t mul, uint32_t div,
-                                                   uint32_t* __restrict__ base_total,
-                                                   uint32_t* __restrict__ total2) {
-    __shared__ uint32_t lds[16];
-    uint32_t n = scan_len(n_host, n_dev, mul, div);
+template <class T>
+__global__ __launch_bounds__(1024) void k_scan_top(T* __restrict__ parts, uint32_t n_host,
+                                                   const uint32_t* __restrict__ n_dev, T* __restrict__ total) {
+    __shared__ T lds[16];
+    uint32_t n = scan_len(n_host, n_dev);
     uint32_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    uint32_t run = base_total ? *base_total : 0u;
+    T run = 0;
     for (uint32_t off = 0; off < nb; off += blockDim.x) {
         uint32_t k = off + threadIdx.x;
-        uint32_t v = k < nb ? parts[k] : 0u;
-        uint32_t tot;
-        uint32_t ex = block_exclusive_scan(v, lds, &tot);
+        T v = k < nb ? parts[k] : T(0);
+        T tot;
+        T ex = block_exclusive_scan<T>(v, lds, &tot);
         if (k < nb) parts[k] = run + ex;
         run += tot;
     }
-    if (threadIdx.x == 0) {
-        if (base_total) *base_total = run;
-        if (total2) *total2 = run;
-    }
+    if (threadIdx.x == 0 && total) *total = run;
 }
 
-__global__ __launch_bounds__(256) void k_scan_down(const uint32_t* __restrict__ in, uint32_t n_host,
-                                                   const uint32_t* __restrict__ n_dev, uint32_t mul, uint32_t div,
-                                                   const uint32_t* __restrict__ parts,
-                                                   uint32_t* __restrict__ out) {
-    __shared__ uint32_t lds[8];
-    uint32_t n = scan_len(n_host, n_dev, mul, div);
+template <class T>
+__global__ __launch_bounds__(256) void k_scan_down(const T* __restrict__ in, uint32_t n_host,
+                                                   const uint32_t* __restrict__ n_dev, const T* __restrict__ parts,
+                                                   T* __restrict__ out) {
+    __shared__ T lds[8];
+    uint32_t n = scan_len(n_host, n_dev);
     uint32_t base = blockIdx.x * SCAN_TILE;
     if (base >= n) return;
-    uint32_t v[SCAN_ITEMS];
-    uint32_t s = 0;
+    T v[SCAN_ITEMS];
+    T s = 0;
     for (int i = 0; i < SCAN_ITEMS; ++i) {
         uint32_t k = base + threadIdx.x * SCAN_ITEMS + i;
-        v[i] = k < n ? in[k] : 0u;
+        v[i] = k < n ? in[k] : T(0);
         s += v[i];
     }
-    uint32_t tot;
-    uint32_t ex = block_exclusive_scan(s, lds, &tot) + parts[blockIdx.x];
+    T tot;
+    T ex = block_exclusive_scan<T>(s, lds, &tot) + parts[blockIdx.x];
     for (int i = 0; i < SCAN_ITEMS; ++i) {
         uint32_t k = base + threadIdx.x * SCAN_ITEMS + i;
         if (k < n) out[k] = ex;
@@ -419,9 +483,65 @@ __global__ __launch_bounds__(256) void k_scan_down(const uint32_t* __restrict__ 
     }
 }
 
+template <class T>
+void launch_scan(hipStream_t s, const T* in, T* out, uint32_t n_host, const uint32_t* n_dev, T* parts, T* total) {
+    uint32_t nb = (n_host + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb > 0) k_scan_reduce<T><<<nb, 256, 0, s>>>(in, n_host, n_dev, parts);
+    k_scan_top<T><<<1, 1024, 0, s>>>(parts, n_host, n_dev, total);
+    if (nb > 0) k_scan_down<T><<<nb, 256, 0, s>>>(in, n_host, n_dev, parts, out);
+}
+
+// ---------------------------------------------------------------------------
+// k_scene_prep: one workgroup. From the scanned fan counts, each scene's
+// rasterizer triangle range, its CHUNK-aligned prim slots, its binning
+// chunks and the base of its coarse count matrix (bins x chunks).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_scene_prep(JobBufs b) {
+    __shared__ uint32_t l1[16], l2[16];
+    const uint32_t ns = b.head->nscenes, NT = b.head->ntri;
+    const uint32_t total = b.cnt->nprims_raw;
+    uint32_t run_ch = 0, run_mat = 0;
+    for (uint32_t off = 0; off < ns; off += blockDim.x) {
+        const uint32_t s = off + threadIdx.x;
+        uint32_t nch = 0, nmat = 0, beg = 0, np = 0;
+        if (s < ns) {
+            const SceneCfg& sc = b.scenes[s];
+            beg = sc.tri_begin < NT ? b.suboff[sc.tri_begin] : total;
+            const uint32_t end = sc.tri_end < NT ? b.suboff[sc.tri_end] : total;
+            np = end - beg;
+            nch = (np + CHUNK - 1) / CHUNK;
+            nmat = sc.nbx * sc.nby * nch;
+        }
+        uint32_t tch, tmat;
+        const uint32_t exc = block_exclusive_scan<uint32_t>(nch, l1, &tch);
+        const uint32_t exm = block_exclusive_scan<uint32_t>(nmat, l2, &tmat);
+        if (s < ns) {
+            SceneRun& r = b.runs[s];
+            r.prim_off = beg;
+            r.nprims = np;
+            r.chunk_base = run_ch + exc;
+            r.prim_base = (run_ch + exc) * CHUNK;
+            r.nch = nch;
+            r.mat_base = run_mat + exm;
+        }
+        run_ch += tch;
+        run_mat += tmat;
+    }
+    if (threadIdx.x == 0) {
+        FrameCounters* c = b.cnt;
+        c->nchunks = run_ch;
+        c->nprims = run_ch * CHUNK;
+        c->mat_len = run_mat;
+        uint32_t o = 0;
+        if ((uint64_t)run_ch * CHUNK > b.prim_cap) o |= OVF_PRIMS;
+        if (run_mat > b.mat_cap) o |= OVF_MATRIX;
+        if (o) atomicOr(&c->overflow, o);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_setup: one thread per input triangle; writes its rasterizer triangles
-// at sub_off[t] + k in fan order.
+// at the scene's prim slots in fan order.
 // mod.rs:284-313 (divide), mod.rs:315-345 (viewport), mod.rs:187-222 (edge
 // setup), bounding_box.rs:13-42 + mod.rs:347-361 (clamped pixel bbox).
 // Besides the reference's values each prim carries derived, exactly
@@ -468,9 +588,9 @@ __device__ void depth_plane_bound(PrimGeom& o, uint32_t x0, uint32_t x1, uint32_
     o.pzy = ok ? (float)py : 0.0f;
 }
 
-__device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t H, uint32_t draw,
-                                           uint32_t fs, PrimGeom* __restrict__ g, float* __restrict__ attr,
-                                           uint4* __restrict__ pbin, BinGeom bg, uint32_t& pairs) {
+__device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t H, uint32_t draw, uint32_t fs,
+                                           uint32_t cb, PrimGeom* __restrict__ g, float* __restrict__ attr,
+                                           uint4* __restrict__ pbin, uint32_t& pairs) {
     const float zmin = 0.0f, zmax = 1.0f;
     float sx[3], sy[3], sz[3], w[3];
     for (int k = 0; k < 3; ++k) {
@@ -520,7 +640,7 @@ __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t
     *g = o;
     *pbin = make_uint4(0u, o.bbx, o.bby, __float_as_uint(o.zlo));
     if (x0 < x1 && y0 < y1) {
-        uint32_t cx0 = x0 / bg.cb, cx1 = (x1 - 1) / bg.cb, cy0 = y0 / bg.cb, cy1 = (y1 - 1) / bg.cb;
+        uint32_t cx0 = x0 / cb, cx1 = (x1 - 1) / cb, cy0 = y0 / cb, cy1 = (y1 - 1) / cb;
         pairs += (cx1 - cx0 + 1) * (cy1 - cy0 + 1);
     }
     if (fs == RZ_FS_COLOR) {
@@ -534,120 +654,125 @@ __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t
     }
 }
 
-__global__ __launch_bounds__(256) void k_setup(const DrawDev* __restrict__ draws, uint32_t d,
-                                               FrameCounters* __restrict__ cnt,
-                                               const uint32_t* __restrict__ cls, const uint32_t* __restrict__ nsub,
-                                               const uint32_t* __restrict__ sub_off,
-                                               const float* __restrict__ clip_out, uint32_t clip_cap,
-                                               PrimGeom* __restrict__ geom, float* __restrict__ attrs,
-                                               uint4* __restrict__ pbin, uint32_t prim_cap,
-                                               uint32_t W, uint32_t H, BinGeom bg) {
-    const DrawDev& dd = draws[d];
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t pairs = 0;
-    bool ovf = false;
-    if (t < dd.ntri) {
-        uint32_t c = cls[t];
-        uint32_t base = sub_off[t];
-        uint32_t fs = dd.fs;
-        if (c == 1) {
-            ClipV tv[3];
-            for (int k = 0; k < 3; ++k) {
-                uint32_t vi = dd.idx[(size_t)t * 3 + k];
-                float4 p;
-                load_vertex(dd, vi, p, tv[k].a);
-                tv[k].x = p.x; tv[k].y = p.y; tv[k].z = p.z; tv[k].w = p.w;
-            }
-            if (base < prim_cap)
-                setup_prim(tv, W, H, d, fs, geom + base, attrs + (size_t)base * ATTR_STRIDE, pbin + base, bg, pairs);
-            else
-                ovf = true;
-        } else if (c >= 2) {
-            uint32_t slot = c - 2;
-            uint32_t n = nsub[t];
-            if (slot < clip_cap) {
-                const float* poly = clip_out + (size_t)slot * MAXV * 10;
-                for (uint32_t k = 0; k < n; ++k) {
-                    // clipping.rs:185-190: (v0, v_{k+1}, v_{k+2})
-                    ClipV tv[3];
-                    const uint32_t vid[3] = {0u, k + 1, k + 2};
-                    for (int q = 0; q < 3; ++q) {
-                        const float* p = poly + vid[q] * 10;
-                        tv[q].x = p[0]; tv[q].y = p[1]; tv[q].z = p[2]; tv[q].w = p[3];
-                        for (int j = 0; j < 6; ++j) tv[q].a[j] = p[4 + j];
+__global__ __launch_bounds__(256) void k_setup(JobBufs b) {
+    const uint32_t NT = b.head->ntri, nd = b.head->ndraws;
+    const uint32_t prim_cap = b.prim_cap;
+    for (uint32_t blk = blockIdx.x; blk * 256u < NT; blk += gridDim.x) {
+        const uint32_t t = blk * 256u + threadIdx.x;
+        const uint32_t d = uniform(find_draw(b.draws, nd, blk * 256u));
+        const DrawDev& dd = b.draws[d];
+        const uint32_t scene = dd.scene;
+        const SceneCfg& sc = b.scenes[scene];
+        const SceneRun& run = b.runs[scene];
+        const uint32_t W = sc.W, H = sc.H, cb = sc.cb, fs = dd.fs;
+        const uint32_t lt = t - dd.tri_base;
+        uint32_t pairs = 0;
+        bool ovf = false;
+        if (lt < dd.ntri) {
+            const uint32_t c = b.cls[t];
+            const uint32_t base = run.prim_base + (b.suboff[t] - run.prim_off);
+            if (c == 1) {
+                ClipV tv[3];
+                for (int k = 0; k < 3; ++k) {
+                    uint32_t vi = dd.idx[(size_t)lt * 3 + k];
+                    float4 p;
+                    load_vertex(dd, vi, p, tv[k].a);
+                    tv[k].x = p.x; tv[k].y = p.y; tv[k].z = p.z; tv[k].w = p.w;
+                }
+                if (base < prim_cap)
+                    setup_prim(tv, W, H, d, fs, cb, b.geom + base, b.attrs + (size_t)base * ATTR_STRIDE,
+                               b.pbin + base, pairs);
+                else
+                    ovf = true;
+            } else if (c >= 2) {
+                const uint32_t slot = c - 2;
+                const uint32_t n = b.nsub[t];
+                if (slot < b.clip_cap) {
+                    const float* poly = b.clip_out + (size_t)slot * MAXV * 10;
+                    for (uint32_t k = 0; k < n; ++k) {
+                        // clipping.rs:185-190: (v0, v_{k+1}, v_{k+2})
+                        ClipV tv[3];
+                        const uint32_t vid[3] = {0u, k + 1, k + 2};
+                        for (int q = 0; q < 3; ++q) {
+                            const float* p = poly + vid[q] * 10;
+                            tv[q].x = p[0]; tv[q].y = p[1]; tv[q].z = p[2]; tv[q].w = p[3];
+                            for (int j = 0; j < 6; ++j) tv[q].a[j] = p[4 + j];
+                        }
+                        if (base + k < prim_cap)
+                            setup_prim(tv, W, H, d, fs, cb, b.geom + base + k,
+                                       b.attrs + (size_t)(base + k) * ATTR_STRIDE, b.pbin + base + k, pairs);
+                        else
+                            ovf = true;
                     }
-                    if (base + k < prim_cap)
-                        setup_prim(tv, W, H, d, fs, geom + base + k, attrs + (size_t)(base + k) * ATTR_STRIDE,
-                                   pbin + base + k, bg, pairs);
-                    else
-                        ovf = true;
                 }
             }
         }
+        if (__ballot(ovf) && lane_id() == 0) atomicOr(&b.cnt->overflow, OVF_PRIMS);
+        // wave sum of coarse-bin entries
+        uint32_t s = pairs;
+        for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
+        if (lane_id() == 0 && s) atomicAdd(&b.cnt->npairs, (unsigned long long)s);
     }
-    if (__ballot(ovf) && lane_id() == 0) atomicOr(&cnt->overflow, OVF_PRIMS);
-    // wave sum of coarse-bin entries
-    uint32_t s = pairs;
-    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
-    if (lane_id() == 0 && s) atomicAdd(&cnt->npairs, (unsigned long long)s);
 }
 
 // ---------------------------------------------------------------------------
-// k_bin: ordered coarse binning, one wave per chunk of CHUNK prims.
-// COUNT:   counts[b * NC + chunk] = prims of the chunk overlapping bin b.
-// SCATTER: writes {prim, bbox, zlo} to list position offsets[b * NC + chunk]
-//          + rank, rank = number of earlier prims of the chunk in bin b.
+// k_bin: ordered coarse binning, one wave per chunk of CHUNK prims of one
+// scene. With r = the scene's SceneRun and lc the chunk's index in the scene:
+// COUNT:   counts[r.mat_base + bin * r.nch + lc] = prims of the chunk overlapping bin
+// SCATTER: writes {prim, bbox, zlo} to list position offsets[same index]
+//          + rank, rank = number of earlier prims of the chunk in that bin.
 // Both loop bins in the same order and rank lanes with ballots, so each bin's
 // list holds its prims in submission order.
 // ---------------------------------------------------------------------------
 template <bool SCATTER>
-__global__ __launch_bounds__(256) void k_bin(const FrameCounters* __restrict__ cnt,
-                                             const uint4* __restrict__ pbin, BinGeom bg,
-                                             uint32_t* __restrict__ counts, const uint32_t* __restrict__ offsets,
-                                             uint4* __restrict__ list, uint32_t pair_cap,
-                                             FrameCounters* __restrict__ cnt_w) {
-    if (uniform(cnt->overflow) != 0) return;
-    const uint32_t S = uniform(cnt->nprims);
-    const uint32_t NC = (S + CHUNK - 1) / CHUNK;
+__global__ __launch_bounds__(256) void k_bin(JobBufs b) {
+    if (uniform(b.cnt->overflow) != 0) return;
+    const uint32_t NC = uniform(b.cnt->nchunks), ns = b.head->nscenes;
     const uint32_t lane = lane_id();
-    const uint32_t nb = bg.nbx * bg.nby;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t wave = uniform((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     bool ovf = false;
     for (uint32_t chunk = wave; chunk < NC; chunk += nwaves) {
-        // per-bin running rank, held by lane (b & 63) in register (b >> 6)
+        const uint32_t s = uniform(find_scene_chunk(b.runs, ns, chunk));
+        const SceneCfg& sc = b.scenes[s];
+        const SceneRun& r = b.runs[s];
+        const uint32_t cb = sc.cb, nbx = sc.nbx, nby = sc.nby, nb = nbx * nby;
+        const uint32_t lc = chunk - r.chunk_base, nch = r.nch;
+        const uint32_t pend = r.prim_base + r.nprims;
+        const size_t mb = (size_t)r.mat_base + lc;
+        // per-bin running rank, held by lane (bin & 63) in register (bin >> 6)
         uint32_t run0 = 0, run1 = 0, run2 = 0, run3 = 0;
         if (SCATTER) {
-            if (lane < nb) run0 = offsets[(size_t)lane * NC + chunk];
-            if (lane + 64 < nb) run1 = offsets[(size_t)(lane + 64) * NC + chunk];
-            if (lane + 128 < nb) run2 = offsets[(size_t)(lane + 128) * NC + chunk];
-            if (lane + 192 < nb) run3 = offsets[(size_t)(lane + 192) * NC + chunk];
+            if (lane < nb) run0 = b.bin_offsets[mb + (size_t)lane * nch];
+            if (lane + 64 < nb) run1 = b.bin_offsets[mb + (size_t)(lane + 64) * nch];
+            if (lane + 128 < nb) run2 = b.bin_offsets[mb + (size_t)(lane + 128) * nch];
+            if (lane + 192 < nb) run3 = b.bin_offsets[mb + (size_t)(lane + 192) * nch];
         }
         for (uint32_t sub = 0; sub < CHUNK / 64; ++sub) {
-            uint32_t p = chunk * CHUNK + sub * 64 + lane;
+            const uint32_t p = chunk * CHUNK + sub * 64 + lane;
             uint4 pb = make_uint4(0u, 0u, 0u, 0u);
-            if (p < S) pb = pbin[p];
-            uint32_t x0 = pb.y & 0xFFFFu, x1 = pb.y >> 16, y0 = pb.z & 0xFFFFu, y1 = pb.z >> 16;
-            bool ne = (x0 < x1) && (y0 < y1);
-            uint32_t cx0 = x0 / bg.cb, cx1 = ne ? (x1 - 1) / bg.cb : 0u;
-            uint32_t cy0 = y0 / bg.cb, cy1 = ne ? (y1 - 1) / bg.cb : 0u;
-            for (uint32_t by = 0; by < bg.nby; ++by) {
-                bool rowhit = ne && by >= cy0 && by <= cy1;
+            if (p < pend) pb = b.pbin[p];
+            const uint32_t x0 = pb.y & 0xFFFFu, x1 = pb.y >> 16, y0 = pb.z & 0xFFFFu, y1 = pb.z >> 16;
+            const bool ne = (x0 < x1) && (y0 < y1);
+            const uint32_t cx0 = x0 / cb, cx1 = ne ? (x1 - 1) / cb : 0u;
+            const uint32_t cy0 = y0 / cb, cy1 = ne ? (y1 - 1) / cb : 0u;
+            for (uint32_t by = 0; by < nby; ++by) {
+                const bool rowhit = ne && by >= cy0 && by <= cy1;
                 if (__ballot(rowhit) == 0) continue;
-                for (uint32_t bx = 0; bx < bg.nbx; ++bx) {
-                    bool hit = rowhit && bx >= cx0 && bx <= cx1;
-                    uint64_t m = __ballot(hit);
+                for (uint32_t bx = 0; bx < nbx; ++bx) {
+                    const bool hit = rowhit && bx >= cx0 && bx <= cx1;
+                    const uint64_t m = __ballot(hit);
                     if (m == 0) continue;
-                    uint32_t b = by * bg.nbx + bx;
-                    uint32_t j = b >> 6, bl = b & 63;
-                    uint32_t c = (uint32_t)__popcll(m);
+                    const uint32_t bi = by * nbx + bx;
+                    const uint32_t j = bi >> 6, bl = bi & 63;
+                    const uint32_t c = (uint32_t)__popcll(m);
                     if (SCATTER) {
-                        uint32_t rv = j == 0 ? run0 : (j == 1 ? run1 : (j == 2 ? run2 : run3));
-                        uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)rv, (int)bl);
+                        const uint32_t rv = j == 0 ? run0 : (j == 1 ? run1 : (j == 2 ? run2 : run3));
+                        const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)rv, (int)bl);
                         if (hit) {
-                            uint32_t pos = r0 + mbcnt(m);
-                            if (pos < pair_cap)
-                                list[pos] = make_uint4(p, pb.y, pb.z, pb.w);
+                            const uint32_t pos = r0 + mbcnt(m);
+                            if (pos < b.pair_cap)
+                                b.list[pos] = make_uint4(p, pb.y, pb.z, pb.w);
                             else
                                 ovf = true;
                         }
@@ -662,97 +787,96 @@ __global__ __launch_bounds__(256) void k_bin(const FrameCounters* __restrict__ c
             }
         }
         if (!SCATTER) {
-            if (lane < nb) counts[(size_t)lane * NC + chunk] = run0;
-            if (lane + 64 < nb) counts[(size_t)(lane + 64) * NC + chunk] = run1;
-            if (lane + 128 < nb) counts[(size_t)(lane + 128) * NC + chunk] = run2;
-            if (lane + 192 < nb) counts[(size_t)(lane + 192) * NC + chunk] = run3;
+            if (lane < nb) b.bin_counts[mb + (size_t)lane * nch] = run0;
+            if (lane + 64 < nb) b.bin_counts[mb + (size_t)(lane + 64) * nch] = run1;
+            if (lane + 128 < nb) b.bin_counts[mb + (size_t)(lane + 128) * nch] = run2;
+            if (lane + 192 < nb) b.bin_counts[mb + (size_t)(lane + 192) * nch] = run3;
         }
     }
-    if (SCATTER && __ballot(ovf) && lane == 0) atomicOr(&cnt_w->overflow, OVF_PAIRS);
+    if (SCATTER && __ballot(ovf) && lane == 0) atomicOr(&b.cnt->overflow, OVF_PAIRS);
 }
 
 // ---------------------------------------------------------------------------
 // Fine binning: each coarse bin's list is split again into its 16x16 screen
 // tiles (the raster workgroups), keeping submission order.
-// k_fine_prep numbers the coarse lists' chunks of CHUNK entries:
-//   chunk_base[b] = chunks of all bins before b, chunk_base[nb] = total (nch).
-// The fine count matrix holds, for coarse bin b, local tile lt and chunk c,
-// the entry (T * chunk_base[b] + lt * nch_b + c); its exclusive scan gives
-// every tile one contiguous, ordered list.
+// k_fine_prep, one thread per global coarse bin g: its list range and
+//   fpack[g] = (T * n) << 32 | n, n = chunks of CHUNK entries, T = tiles of the bin.
+// The exclusive scan fpre of fpack gives each bin its first fine-binning
+// chunk (low word) and the base of its fine count matrix (high word); the
+// matrix holds, for local tile lt and chunk c, the entry hi(fpre[g]) + lt*n + c.
+// Its exclusive scan gives every tile one contiguous, ordered list.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(512) void k_fine_prep(FrameCounters* __restrict__ cnt,
-                                                   const uint32_t* __restrict__ coff, BinGeom bg,
-                                                   uint32_t* __restrict__ chunk_base) {
-    __shared__ uint32_t lds[8];
-    if (uniform(cnt->overflow) != 0) return;
-    const uint32_t S = uniform(cnt->nprims);
-    const uint32_t NC = (S + CHUNK - 1) / CHUNK;
-    const uint32_t nb = bg.nbx * bg.nby;
-    const uint32_t b = threadIdx.x;
-    uint32_t len = 0;
-    if (b < nb && NC > 0) {
-        uint32_t beg = coff[(size_t)b * NC];
-        uint32_t end = (b + 1 < nb) ? coff[(size_t)(b + 1) * NC] : cnt->pairs_binned;
-        len = end - beg;
+__device__ __forceinline__ uint32_t coarse_ofs(const JobBufs& b, uint32_t i, uint32_t mat_len, uint32_t tot) {
+    return i < mat_len ? b.bin_offsets[i] : tot;
+}
+
+__global__ __launch_bounds__(256) void k_fine_prep(JobBufs b) {
+    const uint32_t NB = b.head->nbins, ns = b.head->nscenes;
+    const bool bad = b.cnt->overflow != 0;
+    const uint32_t mat_len = b.cnt->mat_len, tot = b.cnt->pairs_binned;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < NB; g += gridDim.x * blockDim.x) {
+        uint32_t beg = 0, end = 0, T = 1;
+        if (!bad) {
+            const uint32_t s = find_scene_bin(b.scenes, ns, g);
+            const SceneCfg& sc = b.scenes[s];
+            const SceneRun& r = b.runs[s];
+            const uint32_t bi = g - sc.bin_base;
+            const uint32_t i0 = r.mat_base + bi * r.nch;
+            beg = coarse_ofs(b, i0, mat_len, tot);
+            end = coarse_ofs(b, i0 + r.nch, mat_len, tot);
+            if (end < beg) end = beg;
+            T = (sc.cb / 16) * (sc.cb / 16);
+        }
+        const uint32_t n = (end - beg + CHUNK - 1) / CHUNK;
+        b.bin_rng[g] = make_uint2(beg, end);
+        b.fpack[g] = ((unsigned long long)(T * n) << 32) | n;
     }
-    uint32_t tot;
-    uint32_t ex = block_exclusive_scan((len + CHUNK - 1) / CHUNK, lds, &tot);
-    if (b <= nb) chunk_base[b] = ex;
-    if (threadIdx.x == 0) cnt->nch = tot;
 }
 
 template <bool SCATTER>
-__global__ __launch_bounds__(256) void k_fine_bin(const FrameCounters* __restrict__ cnt,
-                                                  const uint32_t* __restrict__ coff, const uint4* __restrict__ clist,
-                                                  const uint32_t* __restrict__ chunk_base, BinGeom bg,
-                                                  uint32_t* __restrict__ fcounts, const uint32_t* __restrict__ foff,
-                                                  uint4* __restrict__ flist, uint32_t fcap,
-                                                  FrameCounters* __restrict__ cnt_w) {
-    __shared__ uint32_t cbase[MAX_BINS + 1];
-    if (uniform(cnt->overflow) != 0) return;
-    const uint32_t nb = bg.nbx * bg.nby;
-    for (uint32_t i = threadIdx.x; i <= nb; i += blockDim.x) cbase[i] = chunk_base[i];
-    __syncthreads();
-    const uint32_t NCH = cbase[nb];
-    const uint32_t S = uniform(cnt->nprims);
-    const uint32_t NC = (S + CHUNK - 1) / CHUNK;
-    const uint32_t TS = bg.cb / 16, T = TS * TS;
-    if (T > 256) return;  // the host never configures more tiles per coarse bin
+__global__ __launch_bounds__(256) void k_fine_bin(JobBufs b) {
+    if (uniform(b.cnt->overflow) != 0) return;
+    const uint32_t NB = b.head->nbins, ns = b.head->nscenes;
+    const uint32_t NCH = uniform(b.cnt->fine_chunks);
+    if (uniform(b.cnt->fmat_len) > b.fmat_cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&b.cnt->overflow, OVF_FINE);
+        return;
+    }
     const uint32_t lane = lane_id();
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t wave = uniform((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     bool ovf = false;
-    for (uint32_t g = wave; g < NCH; g += nwaves) {
-        uint32_t lo = 0, hi = nb;  // largest b < nb with cbase[b] <= g
-        while (hi - lo > 1) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (cbase[mid] <= g) lo = mid;
-            else hi = mid;
-        }
-        const uint32_t b = lo;
-        const uint32_t c = g - cbase[b], nch_b = cbase[b + 1] - cbase[b];
-        const uint32_t cbeg = uniform(coff[(size_t)b * NC]);
-        const uint32_t cend = (b + 1 < nb) ? uniform(coff[(size_t)(b + 1) * NC]) : uniform(cnt->pairs_binned);
-        const uint32_t binx0 = (b % bg.nbx) * bg.cb, biny0 = (b / bg.nbx) * bg.cb;
-        const size_t mbase = (size_t)T * cbase[b] + c;
+    for (uint32_t gc = wave; gc < NCH; gc += nwaves) {
+        const uint32_t g = uniform(find_bin_fchunk(b.fpre, NB, gc));
+        const uint32_t s = uniform(find_scene_bin(b.scenes, ns, g));
+        const SceneCfg& sc = b.scenes[s];
+        const uint32_t cb = sc.cb, TS = cb / 16, T = TS * TS;
+        if (T > 256) continue;  // the host never configures more tiles per coarse bin
+        const unsigned long long fp = b.fpre[g];
+        const uint2 rng = b.bin_rng[g];
+        const uint32_t c = gc - lo32(fp);
+        const uint32_t nchb = (rng.y - rng.x + CHUNK - 1) / CHUNK;
+        const uint32_t bi = g - sc.bin_base;
+        const uint32_t binx0 = (bi % sc.nbx) * cb, biny0 = (bi / sc.nbx) * cb;
+        const size_t mbase = (size_t)hi32(fp) + c;
         uint32_t run0 = 0, run1 = 0, run2 = 0, run3 = 0;
         if (SCATTER) {
-            if (lane < T) run0 = foff[mbase + (size_t)lane * nch_b];
-            if (lane + 64 < T) run1 = foff[mbase + (size_t)(lane + 64) * nch_b];
-            if (lane + 128 < T) run2 = foff[mbase + (size_t)(lane + 128) * nch_b];
-            if (lane + 192 < T) run3 = foff[mbase + (size_t)(lane + 192) * nch_b];
+            if (lane < T) run0 = b.fine_off[mbase + (size_t)lane * nchb];
+            if (lane + 64 < T) run1 = b.fine_off[mbase + (size_t)(lane + 64) * nchb];
+            if (lane + 128 < T) run2 = b.fine_off[mbase + (size_t)(lane + 128) * nchb];
+            if (lane + 192 < T) run3 = b.fine_off[mbase + (size_t)(lane + 192) * nchb];
         }
         for (uint32_t sub = 0; sub < CHUNK / 64; ++sub) {
-            const uint32_t e = cbeg + c * CHUNK + sub * 64 + lane;
-            const bool valid = e < cend;
+            const uint32_t e = rng.x + c * CHUNK + sub * 64 + lane;
+            const bool valid = e < rng.y;
             uint4 en = make_uint4(0u, 0u, 0u, 0u);
-            if (valid) en = clist[e];
+            if (valid) en = b.list[e];
             const uint32_t x0 = en.y & 0xFFFFu, x1 = en.y >> 16, y0 = en.z & 0xFFFFu, y1 = en.z >> 16;
             // local tile range inside this coarse bin (the entry overlaps it)
             const uint32_t lx0 = (max(x0, binx0) - binx0) >> 4;
-            const uint32_t lx1 = valid ? (min(x1, binx0 + bg.cb) - 1 - binx0) >> 4 : 0u;
+            const uint32_t lx1 = valid ? (min(x1, binx0 + cb) - 1 - binx0) >> 4 : 0u;
             const uint32_t ly0 = (max(y0, biny0) - biny0) >> 4;
-            const uint32_t ly1 = valid ? (min(y1, biny0 + bg.cb) - 1 - biny0) >> 4 : 0u;
+            const uint32_t ly1 = valid ? (min(y1, biny0 + cb) - 1 - biny0) >> 4 : 0u;
             for (uint32_t ty = 0; ty < TS; ++ty) {
                 const bool rowhit = valid & (ty >= ly0) & (ty <= ly1);
                 if (__ballot(rowhit) == 0) continue;
@@ -768,7 +892,7 @@ __global__ __launch_bounds__(256) void k_fine_bin(const FrameCounters* __restric
                         const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)rv, (int)bl);
                         if (hit) {
                             const uint32_t pos = r0 + mbcnt(m);
-                            if (pos < fcap) flist[pos] = en;
+                            if (pos < b.fine_cap) b.flist[pos] = en;
                             else ovf = true;
                         }
                     }
@@ -782,22 +906,27 @@ __global__ __launch_bounds__(256) void k_fine_bin(const FrameCounters* __restric
             }
         }
         if (!SCATTER) {
-            if (lane < T) fcounts[mbase + (size_t)lane * nch_b] = run0;
-            if (lane + 64 < T) fcounts[mbase + (size_t)(lane + 64) * nch_b] = run1;
-            if (lane + 128 < T) fcounts[mbase + (size_t)(lane + 128) * nch_b] = run2;
-            if (lane + 192 < T) fcounts[mbase + (size_t)(lane + 192) * nch_b] = run3;
+            if (lane < T) b.fine_counts[mbase + (size_t)lane * nchb] = run0;
+            if (lane + 64 < T) b.fine_counts[mbase + (size_t)(lane + 64) * nchb] = run1;
+            if (lane + 128 < T) b.fine_counts[mbase + (size_t)(lane + 128) * nchb] = run2;
+            if (lane + 192 < T) b.fine_counts[mbase + (size_t)(lane + 192) * nchb] = run3;
         }
     }
-    if (SCATTER && __ballot(ovf) && lane == 0) atomicOr(&cnt_w->overflow, OVF_FINE);
+    if (SCATTER && __ballot(ovf) && lane == 0) atomicOr(&b.cnt->overflow, OVF_FINE);
 }
 
 // ---------------------------------------------------------------------------
-// k_raster: one workgroup per 16x16 screen tile, one wave per 8x8 quadrant,
-// one lane per pixel. The pixel's 4 MSAA colours and depths live in the
-// lane's registers for the whole frame; the wave walks its coarse bin's list
-// in submission order and applies mod.rs:443-473 to each rasterizer
+// k_raster: one workgroup per 16x16 screen tile at a time, one wave per 8x8
+// quadrant, one lane per pixel. The pixel's 4 MSAA colours and depths live in
+// the lane's registers for the whole frame; the wave walks its tile's fine
+// list in submission order and applies mod.rs:443-473 to each rasterizer
 // triangle that can touch the quadrant, then resolves (buffers.rs:111-125)
 // and writes the ARGB word once.
+//
+// Tiles of all scenes form one index space. With a grid of one workgroup
+// per tile the order is static; with a smaller (persistent) grid the
+// workgroups pull tiles from 8 queues, one band of tiles per XCD, and move
+// on to the other bands when theirs is empty.
 //
 // Work that provably cannot change a sample is skipped (exact culls):
 //  - list entries whose bbox misses the quadrant;
@@ -806,7 +935,9 @@ __global__ __launch_bounds__(256) void k_fine_bin(const FrameCounters* __restric
 //    LESS test on every sample of the quadrant;
 //  - edge reject: if an edge function is negative, with a margin that covers
 //    the rounding of the reference's evaluation, at the quadrant corner where
-//    it is largest, no sample of the quadrant is inside that edge.
+//    it is largest, no sample of the quadrant is inside that edge;
+//  - depth plane: a lower bound of the prim's depth over the quadrant (and
+//    per pixel) that is not below the stored depths.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float sample_x(int s) {  // mod.rs:108-114
     return s == 0 ? 0.625f : (s == 1 ? 0.875f : (s == 2 ? 0.375f : 0.125f));
@@ -897,27 +1028,37 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
-template <bool STATS>
-__global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict__ cnt,
-                                                const PrimGeom* __restrict__ geom, const float* __restrict__ attrs,
-                                                const DrawDev* __restrict__ draws,
-                                                const uint32_t* __restrict__ chunk_base,
-                                                const uint32_t* __restrict__ foff,
-                                                const uint4* __restrict__ list, uint32_t fcap, BinGeom bg,
-                                                uint32_t W, uint32_t H, float clear_depth,
-                                                uint32_t* __restrict__ out, uint32_t* __restrict__ msaa_color,
-                                                float* __restrict__ msaa_depth, FrameCounters* __restrict__ cnt_w) {
-    __shared__ float lut[256];
-    lut[threadIdx.x] = (float)threadIdx.x / 255.0f;  // color.rs:24-27
-    __syncthreads();
-    if (uniform(cnt->overflow) != 0) return;
-
-    // XCD-aware tile order: workgroups go round-robin to the 8 XCDs, so give
-    // each XCD a contiguous band of tiles (shared coarse lists stay in its L2).
-    const uint32_t ntx = (W + 15) / 16, nty = (H + 15) / 16, nt = ntx * nty;
-    const uint32_t b = blockIdx.x, xcd = b & 7u, k = b >> 3;
+// band x of the persistent queue: tiles [band_lo(x), band_lo(x + 1))
+__device__ __forceinline__ uint32_t band_lo(uint32_t nt, uint32_t x) {
     const uint32_t q = nt / 8u, r = nt % 8u;
-    const uint32_t tile = xcd * q + min(xcd, r) + k;
+    return x * q + min(x, r);
+}
+
+// Read-only inputs of the raster, passed as separate __restrict__ kernel
+// arguments: the no-alias guarantee is what lets the compiler fetch prim
+// records with scalar loads while the kernel writes its outputs.
+struct RasterIn {
+    const PrimGeom* __restrict__ geom;
+    const float* __restrict__ attrs;
+    const uint4* __restrict__ flist;
+    const uint32_t* __restrict__ fine_off;
+    const unsigned long long* __restrict__ fpre;
+    const uint2* __restrict__ bin_rng;
+    const SceneCfg* __restrict__ scenes;
+    const DrawDev* __restrict__ draws;
+};
+
+template <bool STATS>
+__device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in, uint32_t gtile, const float* lut,
+                                            uint32_t& st_entries, uint32_t& st_cand, uint32_t& st_hits,
+                                            uint32_t& st_cov, uint32_t& st_shaded) {
+    const uint32_t ns = b.head->nscenes;
+    const uint32_t s = uniform(find_scene_tile(in.scenes, ns, gtile));
+    const SceneCfg& sc = in.scenes[s];
+    const uint32_t W = sc.W, H = sc.H;
+    const float clear_depth = sc.clear_depth;
+    const uint32_t ntx = (W + 15) / 16;
+    const uint32_t tile = gtile - sc.tile_base;
     const uint32_t tx = tile % ntx, ty = tile / ntx;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t qx0 = tx * 16 + (wave & 1) * 8, qy0 = ty * 16 + (wave >> 1) * 8;
@@ -937,27 +1078,27 @@ __global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict_
     float lane_zbmax = onscreen ? clear_depth : -FLT_MAX_F;
 
     // this tile's fine list (see k_fine_bin for the layout)
-    const uint32_t nb = bg.nbx * bg.nby;
-    const uint32_t TS = bg.cb / 16, T = TS * TS;
-    const uint32_t cbx = (tx * 16) / bg.cb, cby = (ty * 16) / bg.cb;
-    const uint32_t cbin = cby * bg.nbx + cbx;
-    const uint32_t lt = ((ty * 16 - cby * bg.cb) >> 4) * TS + ((tx * 16 - cbx * bg.cb) >> 4);
-    const uint32_t chb = uniform(chunk_base[cbin]), nch_b = uniform(chunk_base[cbin + 1]) - chb;
-    const size_t nfine = (size_t)T * uniform(chunk_base[nb]);
-    const size_t s_idx = (size_t)T * chb + (size_t)lt * nch_b, e_idx = s_idx + nch_b;
-    const uint32_t ftot = uniform(cnt->fine_total);
+    const uint32_t cbs = sc.cb, TS = cbs / 16;
+    const uint32_t cbx = (tx * 16) / cbs, cby = (ty * 16) / cbs;
+    const uint32_t g = sc.bin_base + cby * sc.nbx + cbx;
+    const uint32_t lt = ((ty * 16 - cby * cbs) >> 4) * TS + ((tx * 16 - cbx * cbs) >> 4);
+    const unsigned long long fp = in.fpre[g];
+    const uint2 rng = in.bin_rng[g];
+    const uint32_t nchb = uniform((rng.y - rng.x + CHUNK - 1) / CHUNK);
+    const uint32_t fmat_len = uniform(b.cnt->fmat_len), ftot = uniform(b.cnt->fine_total);
+    const size_t s_idx = (size_t)uniform(hi32(fp)) + (size_t)lt * nchb, e_idx = s_idx + nchb;
     // (clamped to the list buffer: a bad range can never read past it)
-    const uint32_t lend = min(e_idx < nfine ? uniform(foff[e_idx]) : ftot, fcap);
-    const uint32_t lbeg = min(s_idx < nfine ? uniform(foff[s_idx]) : ftot, lend);
+    const uint32_t lend = min(e_idx < fmat_len ? uniform(in.fine_off[e_idx]) : ftot, b.fine_cap);
+    const uint32_t lbeg = min(s_idx < fmat_len ? uniform(in.fine_off[s_idx]) : ftot, lend);
 
-    uint32_t st_entries = 0, st_cand = 0, st_hits = 0, st_cov = 0, st_shaded = 0;
+    const PrimGeom* __restrict__ geom = in.geom;
     for (uint32_t base = lbeg; base < lend; base += 64) {
         const uint32_t i = base + lane;
         uint32_t myprim = 0;
         bool cand = false;
         if (STATS) st_entries += min(64u, lend - base);
         if (i < lend) {
-            const uint4 en = list[i];
+            const uint4 en = in.flist[i];
             myprim = en.x;
             const uint32_t x0 = en.y & 0xFFFFu, x1 = en.y >> 16, y0 = en.z & 0xFFFFu, y1 = en.z >> 16;
             cand = (x0 < qx0 + 8) & (x1 > qx0) & (y0 < qy0 + 8) & (y1 > qy0) & (__uint_as_float(en.w) < qzmax);
@@ -976,39 +1117,39 @@ __global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict_
         PrimGeom nxt = geom[nprim];
         for (;;) {
             m &= m - 1;
-            const PrimGeom g = nxt;
+            const PrimGeom gm = nxt;
             const uint32_t prim = nprim;
             if (m) {  // start loading the next hit's record now
                 nprim = (uint32_t)__builtin_amdgcn_readlane((int)myprim, (int)(__ffsll((unsigned long long)m) - 1));
                 nxt = geom[nprim];
             }
             bool wrote = false;
-            const uint32_t bx0 = g.bbx & 0xFFFFu, bx1 = g.bbx >> 16, by0 = g.bby & 0xFFFFu, by1 = g.bby >> 16;
+            const uint32_t bx0 = gm.bbx & 0xFFFFu, bx1 = gm.bbx >> 16, by0 = gm.bby & 0xFFFFu, by1 = gm.bby >> 16;
             // depth lower bound over the quadrant's samples inside the bbox
             const float fbx0 = (float)bx0, fby0 = (float)by0;
             const float xl = fmaxf(qxl, fbx0 + 0.125f), xh = fminf(qxh, (float)bx1 - 0.125f);
             const float yl = fmaxf(qyl, fby0 + 0.125f), yh = fminf(qyh, (float)by1 - 0.125f);
-            const float zq = g.pz0 + g.pzx * ((g.pzx >= 0.0f ? xl : xh) - fbx0) +
-                             g.pzy * ((g.pzy >= 0.0f ? yl : yh) - fby0);
-            const float zlb = fmaxf(zq, g.zlo);
-            if (!(zq >= qzmax) && !(g.zlo >= qzmax)) {
+            const float zq = gm.pz0 + gm.pzx * ((gm.pzx >= 0.0f ? xl : xh) - fbx0) +
+                             gm.pzy * ((gm.pzy >= 0.0f ? yl : yh) - fby0);
+            const float zlb = fmaxf(zq, gm.zlo);
+            if (!(zq >= qzmax) && !(gm.zlo >= qzmax)) {
                 const bool inb = (px >= bx0) & (px < bx1) & (py >= by0) & (py < by1);
                 // mod.rs:134-146 + 148-170: coverage of the 4 samples
-                const bool t0 = g.flags & 1u, t1 = g.flags & 2u, t2 = g.flags & 4u;
+                const bool t0 = gm.flags & 1u, t1 = gm.flags & 2u, t2 = gm.flags & 4u;
                 float e1s[4], e2s[4];
                 uint32_t cov = 0;
 #pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const float sx = fx + sample_x(s), sy = fy + sample_y(s);
-                    const float e0 = edge(g.nx0, g.ny0, g.ax0, g.ay0, sx, sy);
-                    const float e1 = edge(g.nx1, g.ny1, g.ax1, g.ay1, sx, sy);
-                    const float e2 = edge(g.nx2, g.ny2, g.ax2, g.ay2, sx, sy);
+                for (int k = 0; k < 4; ++k) {
+                    const float sx = fx + sample_x(k), sy = fy + sample_y(k);
+                    const float e0 = edge(gm.nx0, gm.ny0, gm.ax0, gm.ay0, sx, sy);
+                    const float e1 = edge(gm.nx1, gm.ny1, gm.ax1, gm.ay1, sx, sy);
+                    const float e2 = edge(gm.nx2, gm.ny2, gm.ax2, gm.ay2, sx, sy);
                     // bitwise on purpose: straight-line compares, no branches
                     const bool in = ((e0 > 0.0f) | (t0 & !(e0 < 0.0f))) & ((e1 > 0.0f) | (t1 & !(e1 < 0.0f))) &
                                     ((e2 > 0.0f) | (t2 & !(e2 < 0.0f)));
-                    cov |= (in ? 1u : 0u) << s;
-                    e1s[s] = e1;
-                    e2s[s] = e2;
+                    cov |= (in ? 1u : 0u) << k;
+                    e1s[k] = e1;
+                    e2s[k] = e2;
                 }
                 if (!inb) cov = 0;
                 // a covered lane can pass LESS only if some stored depth exceeds the
@@ -1024,37 +1165,37 @@ __global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict_
                     uint32_t pass = 0;
                     const float zbuf[4] = {zb0, zb1, zb2, zb3};
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) {
-                        const float b0 = clamp_bary(e1s[s] * g.inv_area);
-                        const float b1 = clamp_bary(e2s[s] * g.inv_area);
+                    for (int k = 0; k < 4; ++k) {
+                        const float b0 = clamp_bary(e1s[k] * gm.inv_area);
+                        const float b1 = clamp_bary(e2s[k] * gm.inv_area);
                         const float b2 = clamp_bary(1.0f - b0 - b1);
-                        const float z = b0 * g.z0 + b1 * g.z1 + b2 * g.z2;
-                        const bool c = (cov >> s) & 1u;
-                        sd[s] = c ? z : 0.0f;
-                        pass |= (c & (z < zbuf[s])) ? (1u << s) : 0u;
+                        const float z = b0 * gm.z0 + b1 * gm.z1 + b2 * gm.z2;
+                        const bool c = (cov >> k) & 1u;
+                        sd[k] = c ? z : 0.0f;
+                        pass |= (c & (z < zbuf[k])) ? (1u << k) : 0u;
                     }
                     if (pass != 0) {
                         // mod.rs:70-99: perspective-correct barycentrics at the pixel
                         // centre or at the first depth-passing sample
                         float ix = fx + 0.5f, iy = fy + 0.5f;
                         if (pass != 0xFu) {
-                            const int s = __builtin_ctz(pass);
-                            ix = fx + sample_x(s);
-                            iy = fy + sample_y(s);
+                            const int k = __builtin_ctz(pass);
+                            ix = fx + sample_x(k);
+                            iy = fy + sample_y(k);
                         }
-                        const float ef0 = edge(g.nx0, g.ny0, g.ax0, g.ay0, ix, iy);
-                        const float ef1 = edge(g.nx1, g.ny1, g.ax1, g.ay1, ix, iy);
-                        const float ef2 = edge(g.nx2, g.ny2, g.ax2, g.ay2, ix, iy);
-                        const float f_u = ef1 / g.w0;
-                        const float f_v = ef2 / g.w1;
-                        const float f_w = ef0 / g.w2;
+                        const float ef0 = edge(gm.nx0, gm.ny0, gm.ax0, gm.ay0, ix, iy);
+                        const float ef1 = edge(gm.nx1, gm.ny1, gm.ax1, gm.ay1, ix, iy);
+                        const float ef2 = edge(gm.nx2, gm.ny2, gm.ax2, gm.ay2, ix, iy);
+                        const float f_u = ef1 / gm.w0;
+                        const float f_v = ef2 / gm.w1;
+                        const float f_w = ef0 / gm.w2;
                         const float sum = f_u + f_v + f_w;
                         const float u = clamp_bary(f_u / sum);
                         const float v = clamp_bary(f_v / sum);
                         const float w = clamp_bary(1.0f - u - v);
                         // main.rs:67-77 fragment shaders; Debug is Color::grayscale(depths[0])
-                        const uint32_t fs = (g.flags >> 8) & 3u;
-                        const float* at = attrs + (size_t)prim * ATTR_STRIDE;
+                        const uint32_t fs = (gm.flags >> 8) & 3u;
+                        const float* at = in.attrs + (size_t)prim * ATTR_STRIDE;
                         float cr = sd[0], cg = sd[0], cbl = sd[0], ca = 1.0f;
                         if (fs == RZ_FS_COLOR) {
                             cr = at[0] * u + at[4] * v + at[8] * w;
@@ -1062,7 +1203,7 @@ __global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict_
                             cbl = at[2] * u + at[6] * v + at[10] * w;
                             ca = at[3] * u + at[7] * v + at[11] * w;
                         } else if (fs == RZ_FS_TEXTURE) {
-                            const DrawDev& dd = draws[g.flags >> 16];
+                            const DrawDev& dd = in.draws[gm.flags >> 16];
                             const float tu = at[0] * u + at[2] * v + at[4] * w;
                             const float tv = at[1] * u + at[3] * v + at[5] * w;
                             const RGBA c = texture_sample(dd.tex, dd.tw, dd.th, tu, tv, lut, oob);
@@ -1086,14 +1227,7 @@ __global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict_
             if (m == 0) break;
         }
     }
-    if (__ballot(oob) && lane == 0) atomicOr(&cnt_w->flags, RZ_FLAG_TEXEL_OOB);
-    if (STATS && lane == 0) {
-        atomicAdd(&cnt_w->rstat[0], (unsigned long long)st_entries);
-        atomicAdd(&cnt_w->rstat[1], (unsigned long long)st_cand);
-        atomicAdd(&cnt_w->rstat[2], (unsigned long long)st_hits);
-        atomicAdd(&cnt_w->rstat[3], (unsigned long long)st_cov);
-        atomicAdd(&cnt_w->rstat[4], (unsigned long long)st_shaded);
-    }
+    if (__ballot(oob) && lane == 0) atomicOr(&b.runs[s].flags, RZ_FLAG_TEXEL_OOB);
     if (onscreen) {
         // buffers.rs:111-125
         uint32_t rs = ((cb0 & 0x00FF0000u) >> 16) + ((cb1 & 0x00FF0000u) >> 16) + ((cb2 & 0x00FF0000u) >> 16) +
@@ -1102,15 +1236,68 @@ __global__ __launch_bounds__(256) void k_raster(const FrameCounters* __restrict_
                       ((cb3 & 0x0000FF00u) >> 8);
         uint32_t bs = (cb0 & 0xFFu) + (cb1 & 0xFFu) + (cb2 & 0xFFu) + (cb3 & 0xFFu);
         size_t o = (size_t)py * W + px;
-        out[o] = (0xFFu << 24) | (rs / 4u) << 16 | (gs / 4u) << 8 | (bs / 4u);
-        if (msaa_color) {
+        sc.out[o] = (0xFFu << 24) | (rs / 4u) << 16 | (gs / 4u) << 8 | (bs / 4u);
+        if (sc.msaa_color) {
             uint4 c4 = make_uint4(cb0, cb1, cb2, cb3);
-            *reinterpret_cast<uint4*>(msaa_color + o * 4) = c4;
+            *reinterpret_cast<uint4*>(sc.msaa_color + o * 4) = c4;
         }
-        if (msaa_depth) {
+        if (sc.msaa_depth) {
             float4 d4 = make_float4(zb0, zb1, zb2, zb3);
-            *reinterpret_cast<float4*>(msaa_depth + o * 4) = d4;
+            *reinterpret_cast<float4*>(sc.msaa_depth + o * 4) = d4;
         }
+    }
+}
+
+template <bool STATS, bool PERSIST>
+__global__ __launch_bounds__(256) void k_raster(JobBufs b, const PrimGeom* __restrict__ geom,
+                                                const float* __restrict__ attrs, const uint4* __restrict__ flist,
+                                                const uint32_t* __restrict__ fine_off,
+                                                const unsigned long long* __restrict__ fpre,
+                                                const uint2* __restrict__ bin_rng,
+                                                const SceneCfg* __restrict__ scenes,
+                                                const DrawDev* __restrict__ draws) {
+    const RasterIn in{geom, attrs, flist, fine_off, fpre, bin_rng, scenes, draws};
+    __shared__ float lut[256];
+    __shared__ uint32_t next_tile;
+    lut[threadIdx.x] = (float)threadIdx.x / 255.0f;  // color.rs:24-27
+    __syncthreads();
+    if (uniform(b.cnt->overflow) != 0) return;
+    uint32_t st_entries = 0, st_cand = 0, st_hits = 0, st_cov = 0, st_shaded = 0;
+    const uint32_t nt = b.head->ntiles;
+    const uint32_t blk = blockIdx.x, xcd = blk & 7u;
+    // one workgroup per tile: workgroups go round-robin to the 8 XCDs, so
+    // each XCD gets a contiguous band of tiles (shared lists stay in its L2);
+    // persistent: the same bands become 8 queues, drained XCD-local first
+    uint32_t tile = PERSIST ? NO_TILE : band_lo(gridDim.x, xcd) + (blk >> 3);
+    if (!PERSIST && tile >= nt) tile = NO_TILE;
+    for (bool first = true;; first = false) {
+        if (PERSIST) {
+            if (threadIdx.x == 0) {
+                uint32_t got = NO_TILE;
+                for (uint32_t k = 0; k < 8 && got == NO_TILE; ++k) {
+                    const uint32_t x = (xcd + k) & 7u;
+                    const uint32_t lo = band_lo(nt, x), n = band_lo(nt, x + 1) - lo;
+                    if (n == 0) continue;
+                    const uint32_t i = atomicAdd(&b.cnt->q[x], 1u);
+                    if (i < n) got = lo + i;
+                }
+                next_tile = got;
+            }
+            __syncthreads();
+            tile = next_tile;
+            __syncthreads();
+        } else if (!first) {
+            tile = NO_TILE;
+        }
+        if (tile == NO_TILE) break;
+        raster_tile<STATS>(b, in, tile, lut, st_entries, st_cand, st_hits, st_cov, st_shaded);
+    }
+    if (STATS && (threadIdx.x & 63) == 0) {
+        atomicAdd(&b.cnt->rstat[0], (unsigned long long)st_entries);
+        atomicAdd(&b.cnt->rstat[1], (unsigned long long)st_cand);
+        atomicAdd(&b.cnt->rstat[2], (unsigned long long)st_hits);
+        atomicAdd(&b.cnt->rstat[3], (unsigned long long)st_cov);
+        atomicAdd(&b.cnt->rstat[4], (unsigned long long)st_shaded);
     }
 }
 
@@ -1132,71 +1319,43 @@ void launch_max_index(hipStream_t s, const uint32_t* idx, uint64_t n, uint32_t* 
     uint64_t blocks = (n + 255) / 256;
     k_max_index<<<(uint32_t)(blocks < 1024 ? blocks : 1024), 256, 0, s>>>(idx, n, out);
 }
-void launch_frame_reset(hipStream_t s, FrameCounters* cnt) { k_frame_reset<<<1, 64, 0, s>>>(cnt); }
-void launch_draw_reset(hipStream_t s, FrameCounters* cnt) { k_draw_reset<<<1, 64, 0, s>>>(cnt); }
-void launch_mvp(hipStream_t s, DrawDev* draws, uint32_t d, const Mat3& m) { k_mvp<<<1, 64, 0, s>>>(draws, d, m); }
 
-void launch_classify(hipStream_t s, const DrawDev* draws, uint32_t d, uint32_t ntri, FrameCounters* cnt,
-                     uint32_t* cls, uint32_t* nsub, uint32_t* clip_tri, uint32_t clip_cap) {
-    if (ntri == 0) return;
-    k_classify<<<(ntri + 255) / 256, 256, 0, s>>>(draws, d, cnt, cls, nsub, clip_tri, clip_cap);
-}
+static uint32_t at_least1(uint32_t v) { return v ? v : 1u; }
 
-void launch_clip(hipStream_t s, const DrawDev* draws, uint32_t d, FrameCounters* cnt, const uint32_t* clip_tri,
-                 uint32_t clip_cap, uint32_t* nsub, float* clip_out, uint32_t* clip_n) {
-    if (clip_cap == 0) return;
-    uint32_t grid = clip_cap < 8192u ? clip_cap : 8192u;
-    k_clip<<<grid, 64, 0, s>>>(draws, d, cnt, clip_tri, clip_cap, nsub, clip_out, clip_n);
-}
-
-void launch_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uint32_t n_host, const uint32_t* n_dev,
-                 uint32_t mul, uint32_t div, uint32_t* parts, uint32_t* base_total, uint32_t* total2) {
-    uint32_t nb = (n_host + SCAN_TILE - 1) / SCAN_TILE;
-    if (nb > 0) k_scan_reduce<<<nb, 256, 0, s>>>(in, n_host, n_dev, mul, div, parts);
-    k_scan_top<<<1, 1024, 0, s>>>(parts, n_host, n_dev, mul, div, base_total, total2);
-    if (nb > 0) k_scan_down<<<nb, 256, 0, s>>>(in, n_host, n_dev, mul, div, parts, out);
-}
-
-void launch_setup(hipStream_t s, const DrawDev* draws, uint32_t d, uint32_t ntri, FrameCounters* cnt,
-                  const uint32_t* cls, const uint32_t* nsub, const uint32_t* sub_off, const float* clip_out,
-                  uint32_t clip_cap, PrimGeom* geom, float* attrs, uint4* pbin, uint32_t prim_cap, uint32_t W,
-                  uint32_t H, BinGeom bg) {
-    if (ntri == 0) return;
-    k_setup<<<(ntri + 255) / 256, 256, 0, s>>>(draws, d, cnt, cls, nsub, sub_off, clip_out, clip_cap, geom, attrs,
-                                               pbin, prim_cap, W, H, bg);
-}
-
-void launch_bin(hipStream_t s, bool scatter, uint32_t grid, FrameCounters* cnt, const uint4* pbin, BinGeom bg,
-                uint32_t* counts, const uint32_t* offsets, uint4* list, uint32_t pair_cap) {
-    if (grid == 0) return;
-    if (scatter)
-        k_bin<true><<<grid, 256, 0, s>>>(cnt, pbin, bg, counts, offsets, list, pair_cap, cnt);
-    else
-        k_bin<false><<<grid, 256, 0, s>>>(cnt, pbin, bg, counts, offsets, list, pair_cap, cnt);
-}
-
-void launch_fine(hipStream_t s, FrameCounters* cnt, const uint32_t* coff, const uint4* clist, uint32_t* chunk_base,
-                 BinGeom bg, uint32_t* fcounts, uint32_t* foff, uint4* flist, uint32_t fcap, uint32_t nch_max,
-                 uint32_t* parts) {
-    k_fine_prep<<<1, 512, 0, s>>>(cnt, coff, bg, chunk_base);
-    const uint32_t grid = nch_max / 4 + 1 < 4096 ? nch_max / 4 + 1 : 4096;
-    k_fine_bin<false><<<grid, 256, 0, s>>>(cnt, coff, clist, chunk_base, bg, fcounts, nullptr, nullptr, 0, cnt);
-    const uint32_t T = (bg.cb / 16) * (bg.cb / 16);
-    launch_scan(s, fcounts, foff, T * nch_max, &cnt->nch, T, 1, parts, nullptr, &cnt->fine_total);
-    k_fine_bin<true><<<grid, 256, 0, s>>>(cnt, coff, clist, chunk_base, bg, nullptr, foff, flist, fcap, cnt);
-}
-
-void launch_raster(hipStream_t s, FrameCounters* cnt, const PrimGeom* geom, const float* attrs,
-                   const DrawDev* draws, const uint32_t* chunk_base, const uint32_t* foff, const uint4* list,
-                   uint32_t fcap, BinGeom bg, uint32_t W, uint32_t H, float clear_depth, uint32_t* out,
-                   uint32_t* msaa_color, float* msaa_depth, bool stats) {
-    uint32_t nt = ((W + 15) / 16) * ((H + 15) / 16);
-    if (stats)
-        k_raster<true><<<nt, 256, 0, s>>>(cnt, geom, attrs, draws, chunk_base, foff, list, fcap, bg, W, H,
-                                          clear_depth, out, msaa_color, msaa_depth, cnt);
-    else
-        k_raster<false><<<nt, 256, 0, s>>>(cnt, geom, attrs, draws, chunk_base, foff, list, fcap, bg, W, H,
-                                           clear_depth, out, msaa_color, msaa_depth, cnt);
+void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_stats, hipEvent_t ev_bin,
+                hipEvent_t ev_raster) {
+    const uint32_t scene_blocks = at_least1((g.scenes + 255) / 256);
+    k_frame_reset<<<scene_blocks, 256, 0, s>>>(b);
+    k_mvp<<<at_least1(g.draws), 256, 0, s>>>(b);
+    k_classify<<<at_least1(g.tris), 256, 0, s>>>(b);
+    k_clip<<<at_least1(g.clip), 64, 0, s>>>(b);
+    launch_scan<uint32_t>(s, b.nsub, b.suboff, b.tri_cap, &b.head->ntri, b.parts, &b.cnt->nprims_raw);
+    k_scene_prep<<<1, 1024, 0, s>>>(b);
+    k_setup<<<at_least1(g.tris), 256, 0, s>>>(b);
+    if (ev_bin) (void)hipEventRecord(ev_bin, s);
+    k_bin<false><<<at_least1(g.bins), 256, 0, s>>>(b);
+    launch_scan<uint32_t>(s, b.bin_counts, b.bin_offsets, b.mat_cap, &b.cnt->mat_len, b.parts,
+                          &b.cnt->pairs_binned);
+    k_bin<true><<<at_least1(g.bins), 256, 0, s>>>(b);
+    k_fine_prep<<<at_least1(std::min<uint32_t>((b.bins_cap + 255) / 256, 1024u)), 256, 0, s>>>(b);
+    // fine_chunks and fmat_len are adjacent: the u64 total fills both
+    launch_scan<unsigned long long>(s, b.fpack, b.fpre, b.bins_cap, &b.head->nbins, b.parts64,
+                                    reinterpret_cast<unsigned long long*>(&b.cnt->fine_chunks));
+    k_fine_bin<false><<<at_least1(g.fine), 256, 0, s>>>(b);
+    launch_scan<uint32_t>(s, b.fine_counts, b.fine_off, b.fmat_cap, &b.cnt->fmat_len, b.parts, &b.cnt->fine_total);
+    k_fine_bin<true><<<at_least1(g.fine), 256, 0, s>>>(b);
+    if (ev_raster) (void)hipEventRecord(ev_raster, s);
+    const uint32_t rgrid = g.raster ? g.raster : at_least1(g.ntiles);
+#define RZ_RASTER(ST, PE) \
+    k_raster<ST, PE><<<rgrid, 256, 0, s>>>(b, b.geom, b.attrs, b.flist, b.fine_off, b.fpre, b.bin_rng, b.scenes, b.draws)
+    if (raster_stats) {
+        if (g.raster) RZ_RASTER(true, true);
+        else RZ_RASTER(true, false);
+    } else {
+        if (g.raster) RZ_RASTER(false, true);
+        else RZ_RASTER(false, false);
+    }
+#undef RZ_RASTER
 }
 
 }  // namespace rzk

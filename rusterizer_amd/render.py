@@ -62,13 +62,19 @@ class DeviceMesh:
 class Context:
     """One rz_ctx: a headless renderer with its own HIP stream."""
 
-    def __init__(self, width: int, height: int):
+    def __init__(self, width: int, height: int, share: "Context | None" = None):
+        """share: an existing context whose meshes and textures this one uses
+        (rz_create_shared); its own stream and buffers are separate."""
         self._L = L.load()
         h = C.c_void_p()
-        rc = self._L.rz_create(width, height, C.byref(h))
+        if share is None:
+            rc = self._L.rz_create(width, height, C.byref(h))
+        else:
+            rc = self._L.rz_create_shared(share.handle, width, height, C.byref(h))
         if rc != 0:
             raise RzError(rc, "rz_create failed")
         self.handle = h
+        self._share = share  # keeps the group's first context alive no longer than needed
         self.width, self.height = width, height
 
     def close(self):
@@ -107,6 +113,9 @@ class Context:
             raise ValueError("texture must be (height, width, 4) RGBA8")
         th, tw = t.shape[:2]
         self._check(self._L.rz_bind_texture(self.handle, index, t.ctypes.data_as(C.POINTER(C.c_uint8)), tw, th))
+
+    def texture_count(self) -> int:
+        return int(self._L.rz_texture_count(self.handle))
 
     def set_coarse_bin(self, pixels: int):
         self._check(self._L.rz_set_coarse_bin(self.handle, pixels))
@@ -162,6 +171,54 @@ class Context:
         keys = ["tris", "clipped", "prims", "bin_entries", "replays", "raster_entries", "raster_candidates",
                 "raster_hits", "raster_covered", "raster_shaded", "overflow_bits"]
         return dict(zip(keys, (int(x) for x in a)))
+
+    # ---- batches
+    def batch_begin(self):
+        self._check(self._L.rz_batch_begin(self.handle))
+
+    def batch_scene(self, width: int, height: int, clear_depth: float, dev_ptr: int) -> int:
+        idx = C.c_uint32()
+        self._check(self._L.rz_batch_scene(self.handle, int(width), int(height), float(clear_depth),
+                                           C.c_void_p(dev_ptr), C.byref(idx)))
+        return idx.value
+
+    def batch_scene_msaa(self, scene: int, color4_ptr: int, depth4_ptr: int):
+        self._check(self._L.rz_batch_scene_msaa(self.handle, int(scene), C.c_void_p(color4_ptr or None),
+                                                C.c_void_p(depth4_ptr or None)))
+
+    def batch_draw(self, scene: int, mesh: DeviceMesh, world, view, proj, texture: int, fs: int,
+                   vs: int = L.RZ_VS_MVP):
+        w, v, p = (_f32(m).reshape(16) for m in (world, view, proj))
+        fp = C.POINTER(C.c_float)
+        self._check(self._L.rz_batch_draw(self.handle, int(scene), mesh.handle, w.ctypes.data_as(fp),
+                                          v.ctypes.data_as(fp), p.ctypes.data_as(fp), int(texture), int(vs),
+                                          int(fs)))
+
+    def batch_submit(self):
+        self._check(self._L.rz_batch_submit(self.handle))
+
+    def batch_query(self) -> bool:
+        rc = self._L.rz_batch_query(self.handle)
+        if rc < 0:
+            self._check(rc)
+        return rc == 1
+
+    def batch_wait(self):
+        self._check(self._L.rz_batch_wait(self.handle))
+
+    def batch_scene_flags(self, scene: int) -> int:
+        f = C.c_uint32()
+        self._check(self._L.rz_batch_scene_flags(self.handle, int(scene), C.byref(f)))
+        return f.value
+
+    def batch_tuning(self, bin_target: int = 256, raster_per_cu: int = 0, use_graph: bool = True):
+        self._check(self._L.rz_batch_tuning(self.handle, int(bin_target), int(raster_per_cu),
+                                            1 if use_graph else 0))
+
+    def get_batch_tuning(self) -> dict:
+        a = (C.c_uint32 * 3)()
+        self._check(self._L.rz_batch_get_tuning(self.handle, a))
+        return dict(bin_target=int(a[0]), raster_per_cu=int(a[1]), use_graph=bool(a[2]))
 
     def profile(self, enable: bool):
         self._check(self._L.rz_profile(self.handle, 1 if enable else 0))

@@ -91,3 +91,69 @@ def assert_parity(gpu: dict, cpu: dict, where: str = ""):
     assert d_msaa <= 1 and d_img <= 1, f"{where}: colour differs by {d_msaa} / {d_img} LSB"
     assert gpu["flags"] == cpu["flags"] & 5, f"{where}: flags gpu={gpu['flags']} cpu={cpu['flags']}"
     return dict(exact_colour=bool(np.array_equal(gpu["col"], cpu["col"]) and np.array_equal(gpu["img"], cpu["img"])))
+
+
+def run_batch(scenes, ctx=None, tuning=None, msaa: bool = True, extra_draws=None, submit_only: bool = False):
+    """Render scenes as one batch (rz_batch_*). scenes: list of Scene (each
+    its own render target). extra_draws: {scene index: [Scene, ...]} further
+    draws recorded into that scene after its own. Returns per-scene dicts
+    like run_gpu (img, col, dep, flags) plus the batch stats."""
+    import torch
+
+    from rusterizer_amd.render import Context
+
+    own = ctx is None
+    if own:
+        ctx = Context(64, 64)
+    extra_draws = extra_draws or {}
+    try:
+        if tuning is not None:
+            ctx.batch_tuning(**tuning)
+        meshes, outs = [], []
+
+        def prep(sc):
+            t = 0
+            if sc.texture is not None:
+                t = ctx.texture_count()
+                ctx.bind_texture(t, sc.texture)
+            m = ctx.mesh(sc.pos, sc.color, sc.uv, sc.idx)
+            meshes.append(m)
+            return m, t
+
+        recs = []
+        for i, sc in enumerate(scenes):
+            draws = [prep(sc)] + [prep(e) for e in extra_draws.get(i, [])]
+            recs.append(draws)
+        ctx.batch_begin()
+        for i, sc in enumerate(scenes):
+            npx = sc.width * sc.height
+            o = dict(img=torch.empty(npx, dtype=torch.int32, device="cuda"))
+            if msaa:
+                o["col"] = torch.empty(npx * 4, dtype=torch.int32, device="cuda")
+                o["dep"] = torch.empty(npx * 4, dtype=torch.float32, device="cuda")
+            outs.append(o)
+            k = ctx.batch_scene(sc.width, sc.height, sc.clear_depth, o["img"].data_ptr())
+            assert k == i
+            if msaa:
+                ctx.batch_scene_msaa(k, o["col"].data_ptr(), o["dep"].data_ptr())
+            for (m, t), dsc in zip(recs[i], [sc] + list(extra_draws.get(i, []))):
+                ctx.batch_draw(k, m, dsc.world, dsc.view, dsc.proj, t, dsc.fs)
+        ctx.batch_submit()
+        if submit_only:
+            return ctx, outs, meshes
+        ctx.batch_wait()
+        res = []
+        for i, (sc, o) in enumerate(zip(scenes, outs)):
+            r = dict(img=o["img"].cpu().numpy().view(np.uint32).reshape(sc.height, sc.width),
+                     flags=ctx.batch_scene_flags(i))
+            if msaa:
+                r["col"] = o["col"].cpu().numpy().view(np.uint32).reshape(sc.height, sc.width, 4)
+                r["dep"] = o["dep"].cpu().numpy().reshape(sc.height, sc.width, 4)
+            res.append(r)
+        stats = ctx.frame_stats()
+        for m in meshes:
+            m.destroy()
+        return res, stats
+    finally:
+        if own and not submit_only:
+            ctx.close()

@@ -1,0 +1,185 @@
+"""Batched rendering (rz_batch_*): many independent scenes of uneven size in
+one job. Every scene must come out exactly as the oracle renders it alone
+(depth bit-exact, coverage exact, colour within 1 LSB), identical to the
+plain one-scene path, and independent of the tuning, of graph replay and of
+other jobs in flight on the device."""
+import numpy as np
+import pytest
+
+from harness import CLEAR_COLOR, assert_parity, run_batch, run_gpu, run_oracle
+from rusterizer_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed_scenes():
+    """Small scenes covering the shader, size and clipping cases."""
+    out = scenes.batch_scenes(n_scenes=8, seed=71, tri_lo=50, tri_hi=6000, resolutions=(16, 64, 200, 256),
+                              tex_size=64)
+    out.append(scenes.soup_scene(3000, 333, 77, seed=72))                       # colour FS, ragged size
+    dbg = scenes.soup_scene(2000, 97, 130, seed=73)
+    dbg.fs = scenes.FS_DEBUG                                                    # debug FS
+    out.append(dbg)
+    out.append(scenes.cube_scene(800, 600))                                     # near plane cuts the cube
+    one = scenes.soup_scene(200, 1, 1, seed=74)                                 # 1x1 target
+    out.append(one)
+    c1 = scenes.soup_scene(4000, 128, 96, seed=75)
+    c1.clear_depth = 1.0                                                        # clear depth 1.0
+    out.append(c1)
+    return out
+
+
+def test_batch_scenes_match_oracle():
+    scs = _mixed_scenes()
+    res, stats = run_batch(scs)
+    assert stats["tris"] == sum(s.ntri for s in scs)
+    for sc, r in zip(scs, res):
+        cpu = run_oracle(sc)
+        assert_parity(r, cpu, where=f"{sc.name} {sc.width}x{sc.height}")
+
+
+def test_batch_multi_draw_scene_and_empty_scene():
+    a = scenes.soup_scene(3000, 200, 150, seed=81)
+    b = scenes.soup_scene(2500, 200, 150, seed=82, textured=True)
+    empty = scenes.soup_scene(0, 40, 30, seed=83)
+    c = scenes.soup_scene(1500, 64, 64, seed=84)
+    res, _ = run_batch([a, empty, c], extra_draws={0: [b]})
+    assert_parity(res[0], run_oracle([a, b]), where="two draws")
+    assert (res[1]["img"] == CLEAR_COLOR).all()
+    assert_parity(res[2], run_oracle(c), where="after empty scene")
+
+
+def test_batch_equals_plain_path():
+    scs = scenes.batch_scenes(n_scenes=16, seed=91, tri_lo=1000, tri_hi=40000, resolutions=(256, 512),
+                              tex_size=128)
+    res, _ = run_batch(scs, msaa=False)
+    for sc, r in zip(scs, res):
+        plain = run_gpu(sc, msaa=False)
+        assert np.array_equal(r["img"], plain["img"]), sc.name
+        assert r["flags"] == plain["flags"], sc.name
+
+
+TUNINGS = [
+    dict(bin_target=256, raster_per_cu=0, use_graph=False),
+    dict(bin_target=16, raster_per_cu=0, use_graph=False),
+    dict(bin_target=4, raster_per_cu=2, use_graph=False),
+    dict(bin_target=64, raster_per_cu=1, use_graph=True),
+    dict(bin_target=256, raster_per_cu=8, use_graph=True),
+]
+
+
+def test_batch_tuning_invariance():
+    scs = scenes.batch_scenes(n_scenes=10, seed=101, tri_lo=500, tri_hi=30000, resolutions=(256, 768),
+                              tex_size=64)
+    ref = None
+    for t in TUNINGS:
+        res, _ = run_batch(scs, tuning=t, msaa=False)
+        imgs = [r["img"] for r in res]
+        if ref is None:
+            ref = imgs
+            continue
+        for sc, a, b in zip(scs, ref, imgs):
+            assert np.array_equal(a, b), (t, sc.name)
+
+
+def test_graph_replays_match_eager():
+    """The captured graph, replayed several times with eager frames and
+    synchronisations in between, gives the eager result every time."""
+    from rusterizer_amd.render import Context
+
+    scs = scenes.batch_scenes(n_scenes=6, seed=111, tri_lo=2000, tri_hi=20000, resolutions=(256, 512),
+                              tex_size=64)
+    eager, _ = run_batch(scs, tuning=dict(bin_target=64, raster_per_cu=0, use_graph=False), msaa=False)
+    ctx = Context(64, 64)
+    try:
+        other = scenes.soup_scene(5000, 300, 200, seed=112)
+        other_ref = run_gpu(other, msaa=False)["img"]
+        for rep in range(4):
+            res, _ = run_batch(scs, ctx=ctx, tuning=dict(bin_target=64, raster_per_cu=4, use_graph=True),
+                               msaa=False)
+            for sc, a, b in zip(scs, eager, res):
+                assert np.array_equal(a["img"], b["img"]), (rep, sc.name)
+            # eager work of another context between replays
+            assert np.array_equal(run_gpu(other, msaa=False)["img"], other_ref)
+    finally:
+        ctx.close()
+
+
+def test_batch_growth_replay():
+    """A batch whose clipped triangles overflow the first estimate is replayed
+    with larger buffers and still matches the oracle."""
+    heavy = scenes.soup_scene(20000, 256, 256, seed=121, clip_frac=0.9)
+    small = scenes.soup_scene(1000, 64, 64, seed=122)
+    res, stats = run_batch([small, heavy])
+    assert stats["clipped"] > 20000 // 8 + 1024  # more than the first clip-slot estimate
+    assert stats["replays"] >= 1
+    assert_parity(res[0], run_oracle(small), where="small")
+    assert_parity(res[1], run_oracle(heavy), where="heavy")
+
+
+def test_shared_contexts_in_flight_together():
+    """Two contexts over the same meshes and textures, each with a batch in
+    flight: every scene equals its result alone."""
+    from rusterizer_amd.render import Context, RzError
+
+    x = scenes.batch_scenes(n_scenes=5, seed=131, tri_lo=5000, tri_hi=60000, resolutions=(512, 768), tex_size=64)
+    y = scenes.batch_scenes(n_scenes=7, seed=132, tri_lo=1000, tri_hi=30000, resolutions=(256, 512), tex_size=64)
+    alone_x, _ = run_batch(x, msaa=False)
+    alone_y, _ = run_batch(y, msaa=False)
+    a = Context(64, 64)
+    b = Context(64, 64, share=a)
+    try:
+        ca, outs_a, meshes_a = run_batch(x, ctx=a, msaa=False, submit_only=True)
+        cb, outs_b, meshes_b = run_batch(y, ctx=b, msaa=False, submit_only=True)
+        with pytest.raises(RzError):
+            meshes_a[0].destroy()  # still used by the batch in flight
+        while not (a.batch_query() and b.batch_query()):
+            pass
+        for sc, o, r in zip(x, outs_a, alone_x):
+            assert np.array_equal(o["img"].cpu().numpy().view(np.uint32).reshape(sc.height, sc.width), r["img"])
+        for sc, o, r in zip(y, outs_b, alone_y):
+            assert np.array_equal(o["img"].cpu().numpy().view(np.uint32).reshape(sc.height, sc.width), r["img"])
+        for m in meshes_a + meshes_b:
+            m.destroy()
+    finally:
+        b.close()
+        a.close()
+
+
+def test_batch_errors():
+    from rusterizer_amd.render import Context, RzError
+
+    ctx = Context(64, 64)
+    try:
+        sc = scenes.soup_scene(100, 32, 32, seed=141)
+        m = ctx.mesh(sc.pos, sc.color, sc.uv, sc.idx)
+        import torch
+
+        out = torch.empty(32 * 32, dtype=torch.int32, device="cuda")
+        with pytest.raises(RzError):
+            ctx.batch_submit()                      # nothing recorded
+        ctx.batch_begin()
+        with pytest.raises(RzError):
+            ctx.batch_begin()                       # already recording
+        with pytest.raises(RzError):
+            ctx.draw(m, sc.fs)                      # plain draw inside a batch
+        with pytest.raises(RzError):
+            ctx.batch_scene(5000, 5000, 1.0, out.data_ptr())  # more than 256 coarse bins
+        with pytest.raises(RzError):
+            ctx.batch_draw(0, m, sc.world, sc.view, sc.proj, 0, sc.fs)  # no scene 0 yet
+        k = ctx.batch_scene(32, 32, 1.0, out.data_ptr())
+        with pytest.raises(RzError):
+            ctx.batch_draw(k, m, sc.world, sc.view, sc.proj, 3, scenes.FS_TEXTURE)  # texture not bound
+        with pytest.raises(RzError):
+            ctx.batch_draw(k, m, sc.world, sc.view, sc.proj, 0, 7)  # unknown shader
+        ctx.batch_draw(k, m, sc.world, sc.view, sc.proj, 0, sc.fs)
+        ctx.batch_submit()
+        ctx.batch_wait()
+        assert ctx.batch_query()
+        ref = run_gpu(sc, msaa=False)["img"]
+        assert np.array_equal(out.cpu().numpy().view(np.uint32).reshape(32, 32), ref)
+        with pytest.raises(RzError):
+            ctx.batch_tuning(bin_target=0)
+        m.destroy()
+    finally:
+        ctx.close()
