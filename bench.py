@@ -1,12 +1,18 @@
-"""Throughput of the rasterizer hot path on one MI355X.
+"""Throughput of the rasterizer on one MI355X. Prints one JSON line.
 
-Workload (default): the synthetic triangle soup -- 1,000,000 triangles,
-1920x1080, 4x MSAA, colour fragment shader, f32 depth LESS, seed 1001.
-One step = one frame: rz_draw of the device-resident mesh + rz_resolve_device
-(vertex stage, clipping, setup, binning, raster, resolve) with inputs already
-in HBM. Prints one JSON line.
+Default workload (batched): 256 independent scenes (seed 1002), each a
+textured triangle soup of log-uniform 1k..200k triangles at a resolution
+drawn from {256, 512, 768, 1024}^2 with its own 512x512 checkerboard, 4x
+MSAA, f32 depth LESS. One step = one batch: all 256 scenes rendered and
+resolved into their device framebuffers by one submission (HIP graph
+replay), inputs resident in HBM. Beside it, on the same scenes:
+  - the plain path: one renderer per scene, rz_draw + rz_resolve_device per
+    scene, one scene per launch sequence;
+  - the CPU oracle (single thread) on a bounded sample of the scenes.
+The 1M-triangle soup frame (1920x1080, colour FS, seed 1001) is timed too
+and reported under "soup".
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch|soup] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -22,6 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = "MSAA samples/s + triangles/s (batched vs plain GPU vs CPU); p99 render latency"
 
 
 def parse():
@@ -29,11 +36,16 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--tris", type=int, default=1_000_000)
+    ap.add_argument("--workload", choices=("batch", "soup"), default="batch")
+    ap.add_argument("--scenes", type=int, default=256)
+    ap.add_argument("--tris", type=int, default=1_000_000, help="soup triangles")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--seed", type=int, default=1001)
+    ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain per-scene leg (batch workload)")
+    ap.add_argument("--no-tune", action="store_true", help="use the default batch tuning")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget")
     ap.add_argument("--cpu-sample-tris", type=int, default=300_000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy (HBM counter calibration)")
@@ -55,43 +67,19 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes": n * 4, "ok": ok}), flush=True)
 
 
-def load_traffic():
+def load_traffic(name: str):
     """HBM bytes per raster launch measured with rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, scaled by a copy-kernel calibration); None if not measured."""
-    p = os.path.join(ROOT, "profiles", "traffic_raster_soup1m.json")
+    p = os.path.join(ROOT, "profiles", name)
     if os.path.exists(p):
         with open(p) as f:
             return json.load(f).get("bytes_per_launch")
     return None
 
 
-def cpu_baseline(sc, n_sample: int):
-    """The oracle (single-threaded C++ restatement of the reference) on the
-    first n_sample triangles of the same soup (about 10 s of CPU work); the
-    frame time is scaled by n_tris / n_sample (triangles are i.i.d., so the
-    sample is representative)."""
-    from oracle.oracle import OracleRenderer
-
-    n_sample = min(n_sample, sc.ntri)
-    idx = sc.idx[: 3 * n_sample]
-    r = OracleRenderer(sc.width, sc.height, clear_depth=sc.clear_depth)
-    r.set_uniforms(sc.world, sc.view, sc.proj)
-    t0 = time.perf_counter()
-    r.render(sc.pos, sc.color, sc.uv, idx, sc.fs, threads=1)
-    r.framebuffer()
-    dt = time.perf_counter() - t0
-    frame_s = dt * sc.ntri / max(n_sample, 1)
-    return {
-        "value": sc.samples / frame_s,
-        "unit": "MSAA samples/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": f"first {n_sample} of {sc.ntri} triangles rendered in {dt:.2f} s on 1 host thread; "
-                  f"frame time scaled by {sc.ntri / max(n_sample, 1):.2f}",
-        "triangles_per_s": sc.ntri / frame_s,
-        "host_cpu": _cpu_model(),
-        "host_nproc": os.cpu_count(),
-    }
+def log(msg: str):
+    """Progress on stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def _cpu_model():
@@ -103,6 +91,251 @@ def _cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+def oracle_frame_seconds(sc, n_tris=None) -> float:
+    """Single-threaded oracle (C++ restatement of the reference) on sc's first
+    n_tris triangles (all by default): seconds for draw + framebuffer()."""
+    from oracle.oracle import OracleRenderer
+
+    n = sc.ntri if n_tris is None else min(n_tris, sc.ntri)
+    r = OracleRenderer(sc.width, sc.height, clear_depth=sc.clear_depth)
+    if sc.texture is not None:
+        r.bind_texture(0, sc.texture)
+    r.set_uniforms(sc.world, sc.view, sc.proj)
+    t0 = time.perf_counter()
+    r.render(sc.pos, sc.color, sc.uv, sc.idx[: 3 * n], sc.fs, threads=1)
+    r.framebuffer()
+    return time.perf_counter() - t0
+
+
+def roofline(alg_bytes: int, raster_ms: float, traffic):
+    achieved = alg_bytes / (raster_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": "k_raster", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes_per_launch": alg_bytes,
+            "raster_ms": raster_ms}
+
+
+# --------------------------------------------------------------------- soup
+def bench_soup(a, steps, warmup, with_cpu):
+    import torch
+
+    from rusterizer_amd import scenes
+    from rusterizer_amd.render import Context
+
+    seed = a.seed if (a.seed is not None and a.workload == "soup") else scenes.SEEDS["soup"]
+    sc = scenes.soup_scene(a.tris, a.width, a.height, seed=seed)
+    ctx = Context(sc.width, sc.height)
+    ctx.set_uniforms(sc.world, sc.view, sc.proj)
+    mesh = ctx.mesh(sc.pos, sc.color, sc.uv, sc.idx)
+    out = torch.empty(sc.width * sc.height, dtype=torch.int32, device="cuda")
+    optr = out.data_ptr()
+
+    def frame():
+        ctx.draw(mesh, sc.fs)
+        ctx.resolve_device(optr)
+
+    warm = []
+    for _ in range(warmup):
+        frame()
+        warm.append(ctx.frame_stats())
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f0 = time.perf_counter()
+        frame()  # returns after the frame finished on the device
+        lat.append(time.perf_counter() - f0)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    prof = ctx.profile_read()
+    stats = ctx.frame_stats()
+    ctx.profile(False)
+    frames = prof["frames"] or steps
+    raster_ms = prof["raster"] / frames
+    res = {
+        "workload": f"triangle soup {sc.ntri} tris, {sc.width}x{sc.height}, 4x MSAA, colour FS, "
+                    f"f32 depth LESS, seed {seed}",
+        "samples_per_s": sc.samples * steps / dt,
+        "triangles_per_s": sc.ntri * steps / dt,
+        "ms_per_frame": dt / steps * 1e3,
+        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
+        "stage_ms": {k: prof[k] / frames for k in ("geometry", "binning", "raster")},
+        "frame_stats": stats,
+        "warmup_replays": [(w["replays"], w["overflow_bits"]) for w in warm],
+        "roofline": roofline(sc.algorithmic_bytes(), raster_ms, load_traffic("traffic_raster_soup1m.json")),
+    }
+    if with_cpu:
+        n = min(a.cpu_sample_tris, sc.ntri)
+        t = oracle_frame_seconds(sc, n)
+        frame_s = t * sc.ntri / max(n, 1)
+        res["cpu_baseline"] = {
+            "value": sc.samples / frame_s, "unit": "MSAA samples/s", "cores": 1, "kind": "port",
+            "sample": f"first {n} of {sc.ntri} triangles rendered in {t:.2f} s on 1 host thread; "
+                      f"frame time scaled by {sc.ntri / max(n, 1):.2f}",
+            "triangles_per_s": sc.ntri / frame_s, "host_cpu": _cpu_model(), "host_nproc": os.cpu_count()}
+    mesh.destroy()
+    ctx.close()
+    return res
+
+
+# -------------------------------------------------------------------- batch
+def bench_batch(a, steps, warmup, with_cpu, with_plain, tune):
+    import torch
+
+    from rusterizer_amd import autotune, batch, scenes
+    from rusterizer_amd.render import Context
+
+    seed = a.seed if a.seed is not None else scenes.SEEDS["batch"]
+    t_gen = time.perf_counter()
+    scs = scenes.batch_scenes(a.scenes, seed=seed)
+    t_gen = time.perf_counter() - t_gen
+    log(f"generated {len(scs)} scenes in {t_gen:.1f} s")
+    samples = sum(s.samples for s in scs)
+    tris = sum(s.ntri for s in scs)
+    alg_bytes = sum(s.algorithmic_bytes() for s in scs)
+    ctx = Context(64, 64)
+    t_up = time.perf_counter()
+    items = batch.upload(ctx, scs)
+    torch.cuda.synchronize()
+    t_up = time.perf_counter() - t_up
+
+    table = batch.item_table(items)  # rz_batch_item records, built once
+
+    def once():
+        batch.run(ctx, items, table)
+
+    once()  # first batch: buffers grow to fit
+    first_stats = ctx.frame_stats()
+    log(f"uploaded in {t_up:.1f} s; first batch done")
+    tuning_log = None
+    if tune:
+        best, results = autotune.tune(ctx, once, reps=2, warmup=1)
+        tuning_log = [dict(t, ms=s * 1e3) for t, s in results]
+        log(f"autotuned: {best}")
+    for _ in range(warmup):
+        once()
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f0 = time.perf_counter()
+        once()
+        lat.append(time.perf_counter() - f0)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stats = ctx.frame_stats()
+    tuning = ctx.get_batch_tuning()
+    # per-stage times of the same batch launched eagerly (graph replays are
+    # timed as a whole): the raster kernel's duration for the roofline
+    ctx.batch_tuning(tuning["bin_target"], tuning["raster_per_cu"], False)
+    once()
+    ctx.profile(True)
+    for _ in range(3):
+        once()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    ctx.batch_tuning(**tuning)
+    eager_ms = sum(prof[k] for k in ("geometry", "binning", "raster")) / prof["frames"]
+    res = {
+        "workload": f"batched: {len(scs)} textured soup scenes, seed {seed}, {tris} tris, resolutions "
+                    f"{{256,512,768,1024}}^2 ({samples // 4} px), 512^2 checkerboard per scene, 4x MSAA",
+        "samples_per_s": samples * steps / dt,
+        "triangles_per_s": tris * steps / dt,
+        "ms_per_batch": dt / steps * 1e3,
+        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
+        "tuning": tuning,
+        "autotune_ms": tuning_log,
+        "stage_ms_eager": {k: prof[k] / prof["frames"] for k in ("geometry", "binning", "raster")},
+        "eager_ms_per_batch": eager_ms,
+        "batch_stats": stats,
+        "first_batch_replays": (first_stats["replays"], first_stats["overflow_bits"]),
+        "setup_s": {"generate": t_gen, "upload": t_up},
+        "roofline": roofline(alg_bytes, prof["raster"] / prof["frames"], load_traffic("traffic_raster_batch256.json")),
+        "samples_per_batch": samples,
+        "tris_per_batch": tris,
+    }
+    log(f"batched: {res['ms_per_batch']:.2f} ms per batch")
+    if with_plain:
+        res["plain"] = bench_plain(scs, items, steps=max(1, min(3, steps)))
+        log(f"plain: {res['plain']['ms_per_pass']:.2f} ms per pass")
+        res["batched_vs_plain"] = res["samples_per_s"] / res["plain"]["samples_per_s"]
+    if with_cpu:
+        res["cpu_baseline"] = cpu_batch(scs, a.cpu_seconds)
+        log("cpu baseline done")
+        res["batched_vs_cpu"] = res["samples_per_s"] / res["cpu_baseline"]["value"]
+    for r in items:
+        r.mesh.destroy()
+    ctx.close()
+    return res
+
+
+def bench_plain(scs, items, steps: int):
+    """The same scenes through the plain API: one renderer per scene (its
+    texture bound at index 0, as the reference's shader reads texture 0),
+    rz_draw + rz_resolve_device per scene, one scene after the other. Checks
+    that every framebuffer equals the batched one."""
+    import torch
+
+    from rusterizer_amd.render import Context
+
+    ctxs, meshes = [], []
+    for sc in scs:
+        c = Context(sc.width, sc.height)
+        if sc.texture is not None:
+            c.bind_texture(0, sc.texture)
+        c.set_clear_depth(sc.clear_depth)
+        meshes.append(c.mesh(sc.pos, sc.color, sc.uv, sc.idx))
+        ctxs.append(c)
+    outs = [torch.empty(sc.width * sc.height, dtype=torch.int32, device="cuda") for sc in scs]
+
+    def one_pass(lat=None):
+        for sc, c, m, o in zip(scs, ctxs, meshes, outs):
+            f0 = time.perf_counter()
+            c.set_uniforms(sc.world, sc.view, sc.proj)
+            c.draw(m, sc.fs)
+            c.resolve_device(o.data_ptr())
+            if lat is not None:
+                lat.append(time.perf_counter() - f0)
+
+    one_pass()  # buffers grow to fit each scene
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one_pass(lat)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    equal = all(bool(torch.equal(o, r.out)) for o, r in zip(outs, items))
+    samples = sum(s.samples for s in scs)
+    tris = sum(s.ntri for s in scs)
+    for m in meshes:
+        m.destroy()
+    for c in ctxs:
+        c.close()
+    return {"samples_per_s": samples * steps / dt, "triangles_per_s": tris * steps / dt,
+            "ms_per_pass": dt / steps * 1e3, "passes": steps,
+            "scene_latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                                 "p99": float(np.percentile(lat, 99) * 1e3)},
+            "images_equal_batched": equal}
+
+
+def cpu_batch(scs, budget_s: float):
+    """The oracle, single-threaded, on the batch's scenes in order until the
+    time budget is spent (at least one whole scene)."""
+    done, secs, samples, tris = 0, 0.0, 0, 0
+    for sc in scs:
+        if secs >= budget_s and done > 0:
+            break
+        secs += oracle_frame_seconds(sc)
+        samples += sc.samples
+        tris += sc.ntri
+        done += 1
+    return {"value": samples / secs, "unit": "MSAA samples/s", "cores": 1, "kind": "port",
+            "sample": f"first {done} of {len(scs)} scenes ({tris} tris, {samples} samples) rendered whole "
+                      f"in {secs:.2f} s on 1 host thread",
+            "triangles_per_s": tris / secs, "host_cpu": _cpu_model(), "host_nproc": os.cpu_count()}
 
 
 def main():
@@ -118,88 +351,35 @@ def main():
     if a.copy_probe:
         copy_probe()
         return
-    from rusterizer_amd import scenes
-    from rusterizer_amd.render import Context
-
-    sc = scenes.soup_scene(a.tris, a.width, a.height, seed=a.seed)
-    ctx = Context(sc.width, sc.height)
-    ctx.set_uniforms(sc.world, sc.view, sc.proj)
-    mesh = ctx.mesh(sc.pos, sc.color, sc.uv, sc.idx)
-    out = torch.empty(sc.width * sc.height, dtype=torch.int32, device="cuda")
-    optr = out.data_ptr()
-
-    def frame():
-        ctx.draw(mesh, sc.fs)
-        ctx.resolve_device(optr)
-
-    warm_stats = []
-    for _ in range(a.warmup):
-        frame()
-        warm_stats.append(ctx.frame_stats())
-    torch.cuda.synchronize()
-    ctx.profile(True)
-    lat = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        f0 = time.perf_counter()
-        frame()  # returns after the frame finished on the device
-        lat.append(time.perf_counter() - f0)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    prof = ctx.profile_read()
-    stats = ctx.frame_stats()
-    ctx.profile(False)
-
-    frames = prof["frames"] or a.steps
-    raster_ms = prof["raster"] / frames
-    alg_bytes = sc.algorithmic_bytes()
-    achieved = alg_bytes / (raster_ms * 1e-3) / 1e9
-    ms_per_step = dt / a.steps * 1e3
-    value = sc.samples * a.steps / dt
+    with_cpu = not a.no_cpu
+    if a.workload == "soup":
+        s = bench_soup(a, a.steps, a.warmup, with_cpu)
+        line = {"metric": METRIC, "value": s["samples_per_s"], "unit": "MSAA samples/s", "n_gpus": 1,
+                "steps": a.steps, "warmup": a.warmup, "ms_per_step": s["ms_per_frame"], "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                "config": {"workload": s["workload"], "triangles": a.tris, "width": a.width, "height": a.height,
+                           "msaa": 4},
+                "triangles_per_s": s["triangles_per_s"], "latency_ms": s["latency_ms"],
+                "roofline": s["roofline"], "cpu_baseline": s.get("cpu_baseline"), "soup": s}
+        print(json.dumps(line), flush=True)
+        return
+    b = bench_batch(a, a.steps, a.warmup, with_cpu, not a.no_plain, not a.no_tune)
+    soup = bench_soup(a, max(5, a.steps // 2), 2, False)
     line = {
-        "metric": "MSAA samples/s",
-        "value": value,
-        "unit": "MSAA samples/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": f"triangle soup {sc.ntri} tris, {sc.width}x{sc.height}, 4x MSAA, colour FS, "
-                        f"f32 depth LESS, seed {a.seed}",
-            "triangles": sc.ntri,
-            "width": sc.width,
-            "height": sc.height,
-            "msaa": 4,
-        },
-        "triangles_per_s": sc.ntri * a.steps / dt,
-        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
-        "stage_ms": {k: prof[k] / frames for k in ("geometry", "binning", "raster")},
-        "frame_stats": stats,
-        "warmup_replays": [(w["replays"], w["overflow_bits"]) for w in warm_stats],
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_raster",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": load_traffic(),
-            "algorithmic_bytes_per_launch": alg_bytes,
-        },
+        "metric": METRIC, "value": b["samples_per_s"], "unit": "MSAA samples/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": b["ms_per_batch"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": b["workload"], "scenes": a.scenes, "msaa": 4,
+                   "samples_per_batch": b["samples_per_batch"], "triangles_per_batch": b["tris_per_batch"]},
+        "triangles_per_s": b["triangles_per_s"],
+        "latency_ms": b["latency_ms"],
+        "batched_vs_plain": b.get("batched_vs_plain"),
+        "batched_vs_cpu": b.get("batched_vs_cpu"),
+        "roofline": b["roofline"],
+        "cpu_baseline": b.get("cpu_baseline"),
+        "batch": b,
+        "soup": soup,
     }
-    if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(sc, a.cpu_sample_tris)
-    else:
-        line["cpu_baseline"] = None
-    mesh.destroy()
-    ctx.close()
     print(json.dumps(line), flush=True)
 
 

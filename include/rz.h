@@ -63,6 +63,10 @@ enum rz_stage { RZ_STAGE_GEOMETRY = 0, RZ_STAGE_BINNING = 1, RZ_STAGE_RASTER = 2
  * bins of 256x256 pixels (e.g. up to 4096x4096 or 3840x2160), else RZ_ERR_LIMIT.
  * Uses the current HIP device. */
 int rz_create(uint32_t width, uint32_t height, rz_ctx **out);
+/* Environment read by rz_create / rz_create_shared (debugging aids):
+ *   RZ_RASTER_STATS=1  count raster work per frame (rz_frame_stats; slower)
+ *   RZ_CLIP_WIDE=1     clip each polygon with a full 64-lane wave instead of
+ *                      16 lanes (same results; exercises the wide clipper) */
 /* A further context on the same device sharing ctx's meshes and textures
  * (no reference counterpart: lets several jobs -- frames or batches -- be in
  * flight at once on their own streams over the same resident data). The
@@ -159,6 +163,21 @@ int rz_batch_scene_msaa(rz_ctx *ctx, uint32_t scene, uint32_t *color4_dev, float
  * rz_bind_texture). */
 int rz_batch_draw(rz_ctx *ctx, uint32_t scene, const rz_mesh *mesh, const float world[16], const float view[16],
                   const float proj[16], uint32_t texture, int vertex_shader, int fragment_shader);
+/* One scene with one draw, for rz_batch_add. */
+typedef struct rz_batch_item {
+    uint32_t width, height;   /* scene size (same limits as rz_create)          */
+    float clear_depth;
+    uint32_t texture;         /* texture index for RZ_FS_TEXTURE                */
+    int32_t vertex_shader, fragment_shader;
+    uint32_t *argb_dev;       /* width*height resolved 0xAARRGGBB words, device */
+    const rz_mesh *mesh;
+    float world[16], view[16], proj[16]; /* row-major uniforms                 */
+} rz_batch_item;
+
+/* Append n scenes, each drawn once, in one call (rz_batch_scene +
+ * rz_batch_draw per item). Either all items are added or none (on error).
+ * *first_scene (may be NULL) receives the index of the first one. */
+int rz_batch_add(rz_ctx *ctx, const rz_batch_item *items, uint32_t n, uint32_t *first_scene);
 /* Queue the batch on the context stream and return. */
 int rz_batch_submit(rz_ctx *ctx);
 /* 1 if the submitted batch has finished (or none is in flight), 0 if it is

@@ -17,6 +17,23 @@ RZ_FS_TEXTURE, RZ_FS_COLOR, RZ_FS_DEBUG = 0, 1, 2
 RZ_FLAG_TEXEL_OOB, RZ_FLAG_CLIP_FEWER = 1, 4
 RZ_STAGE_GEOMETRY, RZ_STAGE_BINNING, RZ_STAGE_RASTER, RZ_NSTAGES = 0, 1, 2, 3
 
+# rz_batch_item of include/rz.h as a numpy record (same layout as the C struct)
+BATCH_ITEM = None
+
+
+def batch_item_dtype():
+    import numpy as np
+
+    global BATCH_ITEM
+    if BATCH_ITEM is None:
+        BATCH_ITEM = np.dtype([("width", "<u4"), ("height", "<u4"), ("clear_depth", "<f4"), ("texture", "<u4"),
+                               ("vertex_shader", "<i4"), ("fragment_shader", "<i4"), ("argb_dev", "<u8"),
+                               ("mesh", "<u8"), ("world", "<f4", (16,)), ("view", "<f4", (16,)),
+                               ("proj", "<f4", (16,))], align=True)
+        assert BATCH_ITEM.itemsize == 232, BATCH_ITEM.itemsize
+    return BATCH_ITEM
+
+
 # every entry point of include/rz.h: name -> (restype, argtypes)
 _vp = C.c_void_p
 _f = C.POINTER(C.c_float)
@@ -45,6 +62,7 @@ SIGNATURES = {
     "rz_batch_scene": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_float, _vp, _u32]),
     "rz_batch_scene_msaa": (C.c_int, [_vp, C.c_uint32, _vp, _vp]),
     "rz_batch_draw": (C.c_int, [_vp, C.c_uint32, _vp, _f, _f, _f, C.c_uint32, C.c_int, C.c_int]),
+    "rz_batch_add": (C.c_int, [_vp, _vp, C.c_uint32, _u32]),
     "rz_batch_submit": (C.c_int, [_vp]),
     "rz_batch_query": (C.c_int, [_vp]),
     "rz_batch_wait": (C.c_int, [_vp]),

@@ -1,0 +1,76 @@
+"""Autotuner of the batched path.
+
+The knobs (rz_batch_tuning): the largest number of coarse bins per scene
+when the bin side is chosen automatically (the side is the smallest power of
+two >= 16 px giving at most that many bins; the raster tile is fixed at
+16x16 px, so this sets how many raster tiles one coarse bin holds), the
+number of persistent raster workgroups per CU (0 = one workgroup per tile),
+and whether the batch replays its captured HIP graph. Every setting renders
+bit-identical images; only the time changes. The tuner times each candidate
+on the caller's real batch and keeps the fastest.
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+import time
+
+BIN_TARGETS = (16, 64, 256)
+RASTER_PER_CU = (0, 4, 8)
+
+
+def candidates(bin_targets=BIN_TARGETS, raster_per_cu=RASTER_PER_CU, graph=(True,)):
+    for bt, rp, g in itertools.product(bin_targets, raster_per_cu, graph):
+        yield dict(bin_target=bt, raster_per_cu=rp, use_graph=g)
+
+
+def signature(scene_sizes, tris: int) -> str:
+    """Key of a batch shape: scene count, resolution histogram, triangle
+    count rounded to a power of two."""
+    hist = {}
+    for w, h in scene_sizes:
+        hist[f"{w}x{h}"] = hist.get(f"{w}x{h}", 0) + 1
+    return json.dumps({"scenes": len(scene_sizes), "res": dict(sorted(hist.items())),
+                       "tris_log2": max(0, int(tris).bit_length())}, sort_keys=True)
+
+
+def tune(ctx, run_batch, cands=None, reps: int = 3, warmup: int = 1):
+    """run_batch(): records, submits and waits one batch on ctx. Returns
+    (best tuning, [(tuning, median seconds), ...]); leaves ctx on the best."""
+    results = []
+    for t in (cands if cands is not None else candidates()):
+        ctx.batch_tuning(**t)
+        for _ in range(warmup):
+            run_batch()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            run_batch()
+            ts.append(time.perf_counter() - t0)
+        ts.sort()
+        results.append((t, ts[len(ts) // 2]))
+    best = min(results, key=lambda r: r[1])[0]
+    ctx.batch_tuning(**best)
+    return best, results
+
+
+class TuningCache:
+    """Best tuning per batch signature, kept in a JSON file."""
+
+    def __init__(self, path: str):
+        self.path = path
+        self.data = {}
+        if os.path.exists(path):
+            with open(path) as f:
+                self.data = json.load(f)
+
+    def get(self, sig: str):
+        return self.data.get(sig)
+
+    def put(self, sig: str, tuning: dict):
+        self.data[sig] = tuning
+        tmp = self.path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(self.data, f, indent=1, sort_keys=True)
+        os.replace(tmp, self.path)

@@ -135,7 +135,7 @@ struct rz_ctx {
     // pipeline buffers
     DBuf<uint32_t> cls, nsub, suboff, parts;
     DBuf<unsigned long long> parts64;
-    DBuf<uint32_t> clip_tri, clip_n;
+    DBuf<uint32_t> clip_tri, clip_redo, clip_n;
     DBuf<float> clip_out;  // MAXV * 10 floats per slot
     DBuf<PrimGeom> geom;
     DBuf<float> attrs;  // ATTR_STRIDE per prim
@@ -155,7 +155,8 @@ struct rz_ctx {
     // graph of the batch launch sequence
     hipGraphExec_t graph = nullptr;
     uint64_t graph_gen = ~0ull;
-    uint32_t graph_dcap = 0, graph_scap = 0, graph_raster = 0;
+    uint32_t graph_dcap = 0, graph_scap = 0, graph_raster = 0, graph_tiles = 0;
+    uint32_t tiles_cap = 0;  // raster grid of the graph: most tiles of any job so far
     bool graph_stats = false;
 
     hipEvent_t done = nullptr;
@@ -166,6 +167,7 @@ struct rz_ctx {
     uint64_t frame_stats[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     std::vector<uint32_t> scene_flags;
     bool raster_stats = false;  // RZ_RASTER_STATS=1 at rz_create
+    bool clip_wide = false;     // RZ_CLIP_WIDE=1 at rz_create: one wave per clipped polygon
 
     bool prof = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -361,6 +363,7 @@ int ensure_buffers(rz_ctx* c, const JobPlan& pl, const Needs& n_in) {
     if ((rc = grow(c, c->attrs, c->geom.cap * ATTR_STRIDE))) return rc;
     if ((rc = grow(c, c->pbin, c->geom.cap))) return rc;
     if ((rc = grow(c, c->clip_tri, n.clip))) return rc;
+    if ((rc = grow(c, c->clip_redo, c->clip_tri.cap))) return rc;
     if ((rc = grow(c, c->clip_n, c->clip_tri.cap))) return rc;
     if ((rc = grow(c, c->clip_out, c->clip_tri.cap * MAXV * 10))) return rc;
     if ((rc = grow(c, c->list, n.pairs))) return rc;
@@ -396,6 +399,7 @@ JobBufs job_bufs(const rz_ctx* c) {
     b.nsub = c->nsub.p;
     b.suboff = c->suboff.p;
     b.clip_tri = c->clip_tri.p;
+    b.clip_redo = c->clip_redo.p;
     b.clip_n = c->clip_n.p;
     b.clip_out = c->clip_out.p;
     b.geom = c->geom.p;
@@ -420,6 +424,7 @@ JobBufs job_bufs(const rz_ctx* c) {
     b.fine_cap = clamp_u32(c->flist.cap);
     b.fmat_cap = clamp_u32(c->fine_counts.cap);
     b.bins_cap = clamp_u32(c->bin_rng.cap);
+    b.clip_wide = c->clip_wide ? 1u : 0u;
     return b;
 }
 
@@ -437,8 +442,7 @@ JobGrid job_grid(const rz_ctx* c, const JobPlan& pl, bool for_graph) {
     const uint64_t fch = c->list.cap / CHUNK + c->bin_rng.cap;
     g.fine = (uint32_t)std::min<uint64_t>((fch + 3) / 4, 4096);
     g.raster = c->raster_per_cu ? c->raster_per_cu * (uint32_t)c->n_cu : 0;
-    if (for_graph && g.raster == 0) g.raster = 8 * (uint32_t)c->n_cu;  // graph: persistent raster
-    g.ntiles = pl.ntiles;
+    g.ntiles = for_graph ? std::max(c->tiles_cap, pl.ntiles) : pl.ntiles;
     g.scenes = for_graph ? c->scap : pl.nscenes;
     return g;
 }
@@ -462,9 +466,10 @@ int enqueue_eager(rz_ctx* c, const JobPlan& pl) {
 // capture (once per buffer generation / table capacity / tuning) and launch
 int enqueue_graph(rz_ctx* c, const JobPlan& pl) {
     hipStream_t s = c->stream;
+    if (pl.ntiles > c->tiles_cap) c->tiles_cap = std::max<uint32_t>(pl.ntiles, c->tiles_cap + c->tiles_cap / 4);
     const JobGrid g = job_grid(c, pl, true);
     if (!c->graph || c->graph_gen != c->gen || c->graph_dcap != c->dcap || c->graph_scap != c->scap ||
-        c->graph_raster != g.raster || c->graph_stats != c->raster_stats) {
+        c->graph_raster != g.raster || c->graph_tiles != g.ntiles || c->graph_stats != c->raster_stats) {
         if (c->graph) {
             HIP_TRY(c, hipStreamSynchronize(s));
             (void)hipGraphExecDestroy(c->graph);
@@ -496,6 +501,7 @@ int enqueue_graph(rz_ctx* c, const JobPlan& pl) {
         c->graph_dcap = c->dcap;
         c->graph_scap = c->scap;
         c->graph_raster = g.raster;
+        c->graph_tiles = g.ntiles;
         c->graph_stats = c->raster_stats;
     }
     if (c->prof) HIP_TRY(c, hipEventRecord(c->ev[0], s));
@@ -708,6 +714,8 @@ int create_impl(rz_pool* pool, uint32_t width, uint32_t height, rz_ctx** out) {
     }
     const char* st = getenv("RZ_RASTER_STATS");
     c->raster_stats = st && st[0] == '1';
+    const char* cw = getenv("RZ_CLIP_WIDE");
+    c->clip_wide = cw && cw[0] == '1';
     if (pool) {
         c->pool = pool;
     } else {
@@ -773,6 +781,7 @@ int rz_destroy(rz_ctx* c) {
     release(c->parts);
     release(c->parts64);
     release(c->clip_tri);
+    release(c->clip_redo);
     release(c->clip_n);
     release(c->clip_out);
     release(c->geom);
@@ -1002,6 +1011,43 @@ int rz_batch_draw(rz_ctx* c, uint32_t scene, const rz_mesh* mesh_c, const float 
     memcpy(d.m.proj, proj, sizeof(d.m.proj));
     c->draws.push_back(d);
     mesh->pending++;
+    return RZ_OK;
+}
+
+int rz_batch_add(rz_ctx* c, const rz_batch_item* items, uint32_t n, uint32_t* first_scene) {
+    if (!c) return RZ_ERR_ARG;
+    if (c->recording != JobKind::Batch || c->in_flight) return fail(c, RZ_ERR_STATE, "no batch being recorded");
+    if (n && !items) return fail(c, RZ_ERR_ARG, "null items");
+    if (c->scenes.size() + n > MAX_SCENES) return fail(c, RZ_ERR_LIMIT, "too many scenes in one batch");
+    if (c->draws.size() + n > MAX_DRAWS) return fail(c, RZ_ERR_LIMIT, "too many draws in one batch");
+    for (uint32_t i = 0; i < n; ++i) {  // validate everything first: all or nothing
+        const rz_batch_item& it = items[i];
+        const std::string at = "item " + std::to_string(i) + ": ";
+        if (!it.argb_dev) return fail(c, RZ_ERR_ARG, at + "null output");
+        if (it.width == 0 || it.height == 0 || it.width > MAX_DIM || it.height > MAX_DIM)
+            return fail(c, RZ_ERR_ARG, at + "bad size");
+        if (((it.width + MAX_COARSE_PX - 1) / MAX_COARSE_PX) * ((it.height + MAX_COARSE_PX - 1) / MAX_COARSE_PX) >
+            MAX_BINS)
+            return fail(c, RZ_ERR_LIMIT, at + "scene needs more than 256 coarse bins");
+        if (!it.mesh || it.mesh->pool != c->pool) return fail(c, RZ_ERR_ARG, at + "bad mesh");
+        int rc = check_shaders(c, it.vertex_shader, it.fragment_shader);
+        if (rc) return fail(c, rc, at + c->err);
+        if (it.fragment_shader == RZ_FS_TEXTURE && it.texture >= c->pool->textures.size())
+            return fail(c, RZ_ERR_ARG, at + "texture index not bound");
+    }
+    if (first_scene) *first_scene = (uint32_t)c->scenes.size();
+    for (uint32_t i = 0; i < n; ++i) {
+        const rz_batch_item& it = items[i];
+        rz_mesh* mesh = const_cast<rz_mesh*>(it.mesh);
+        const uint32_t scene = (uint32_t)c->scenes.size();
+        c->scenes.push_back(PendingScene{it.width, it.height, it.clear_depth, it.argb_dev, nullptr, nullptr});
+        PendingDraw d{mesh, scene, (uint32_t)it.fragment_shader, it.texture, Mat3{}};
+        memcpy(d.m.world, it.world, sizeof(d.m.world));
+        memcpy(d.m.view, it.view, sizeof(d.m.view));
+        memcpy(d.m.proj, it.proj, sizeof(d.m.proj));
+        c->draws.push_back(d);
+        mesh->pending++;
+    }
     return RZ_OK;
 }
 

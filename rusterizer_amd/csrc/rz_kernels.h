@@ -44,6 +44,8 @@ struct FrameCounters {
     uint32_t fine_chunks;   // chunks of the coarse lists (fine binning work items)
     uint32_t fmat_len;      // fine count matrix length (written with fine_chunks as one u64)
     uint32_t q[8];          // raster work queue heads, one per XCD band
+    uint32_t nredo;         // clipped polygons handed from k_clip to k_clip_redo
+    uint32_t pad;
     // raster work counters (stats variant only): list entries scanned,
     // candidates after the bbox / depth / edge culls, hits processed,
     // hits with coverage, hits that shaded
@@ -140,6 +142,7 @@ struct JobBufs {
     uint32_t* nsub;
     uint32_t* suboff;
     uint32_t* clip_tri;
+    uint32_t* clip_redo;  // slots k_clip left to k_clip_redo
     uint32_t* clip_n;
     float* clip_out;
     PrimGeom* geom;
@@ -161,6 +164,7 @@ struct JobBufs {
     unsigned long long* parts64;
     // capacities (elements)
     uint32_t tri_cap, clip_cap, prim_cap, pair_cap, mat_cap, fine_cap, fmat_cap, bins_cap;
+    uint32_t clip_wide;  // clip every polygon with a full wave (k_clip_redo)
 };
 
 // Grid sizes of one launch sequence. Kernels loop over their work with the
@@ -172,8 +176,8 @@ struct JobGrid {
     uint32_t clip;       // k_clip waves
     uint32_t bins;       // k_bin blocks
     uint32_t fine;       // k_fine_bin blocks
-    uint32_t raster;     // raster workgroups; 0 = one per tile (static order)
-    uint32_t ntiles;     // tiles (for the static raster order)
+    uint32_t raster;     // persistent raster workgroups; 0 = one workgroup per tile
+    uint32_t ntiles;     // tiles the one-per-tile raster grid must cover (>= the job's)
     uint32_t scenes;     // scene count (k_frame_reset / k_scene_prep loops)
 };
 

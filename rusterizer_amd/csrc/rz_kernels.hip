@@ -182,6 +182,7 @@ __global__ __launch_bounds__(256) void k_frame_reset(JobBufs b) {
         c->pairs_binned = 0;
         c->fine_chunks = 0;
         c->npairs = 0;
+        c->nredo = 0;
         c->fmat_len = 0;
         c->fine_total = 0;
         for (int i = 0; i < 8; ++i) c->q[i] = 0;
@@ -274,121 +275,158 @@ __global__ __launch_bounds__(256) void k_classify(JobBufs b) {
 }
 
 // ---------------------------------------------------------------------------
-// k_clip: one wave per clipped triangle, one lane per polygon vertex.
-// Sutherland-Hodgman over the planes L, R, B, T, N, F (clipping.rs:114-177):
-// lane i owns the edge (prev = i-1 mod n, cur = i) and emits 0, 1 or 2
-// vertices; a wave prefix count places them, so the output keeps the
-// reference's vertex order exactly. The final polygon (<= MAXV vertices)
-// goes to its clip slot; the fan (clipping.rs:183-190) is formed in k_setup.
+// Clipping: Sutherland-Hodgman over the planes L, R, B, T, N, F
+// (clipping.rs:114-177), one group of G lanes per clipped triangle, one lane
+// per polygon vertex. Lane i of a group owns the edge (prev = i-1 mod n,
+// cur = i) and emits 0, 1 or 2 vertices; a prefix count over the group
+// places them, so the output keeps the reference's vertex order exactly.
+// The final polygon (<= MAXV vertices) goes to its clip slot; the fan
+// (clipping.rs:183-190) is formed in k_setup.
+// k_clip runs 4 groups of 16 lanes per wave. A polygon that would need more
+// than 16 lanes at some plane (never for a triangle with exact arithmetic --
+// each plane adds at most one vertex -- but rounding could) is handed to
+// k_clip_redo, which clips it again from the start with all 64 lanes.
 // ---------------------------------------------------------------------------
 struct ClipV {
     float x, y, z, w;
     float a[6];
 };
 
-__global__ __launch_bounds__(64) void k_clip(JobBufs b) {
-    __shared__ float sh[CLIP_LDS_V * 10];
-    const uint32_t lane = threadIdx.x;
-    const uint32_t nd = b.head->ndraws;
-    const uint32_t nslots = min(uniform(b.cnt->nclip), b.clip_cap);
-    for (uint32_t s = blockIdx.x; s < nslots; s += gridDim.x) {
-        const uint32_t t = uniform(b.clip_tri[s]);
-        const uint32_t d = uniform(find_draw(b.draws, nd, t));
+template <uint32_t G>
+__device__ __forceinline__ void clip_polygon(const JobBufs& b, uint32_t slot, bool have, float* __restrict__ sh) {
+    const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gb = lane - gl;
+    const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << gb);
+    float* q = sh + gb * 10;  // this group's staging area: G vertices
+    uint32_t t = 0, scene = 0;
+    ClipV v;
+    v.x = v.y = v.z = v.w = 0.0f;
+    for (int j = 0; j < 6; ++j) v.a[j] = 0.0f;
+    if (have) {
+        t = b.clip_tri[slot];
+        const uint32_t d = find_draw(b.draws, b.head->ndraws, t);
         const DrawDev& dd = b.draws[d];
-        const uint32_t lt = t - dd.tri_base;
-        ClipV v;
-        if (lane < 3) {
-            uint32_t vi = dd.idx[(size_t)lt * 3 + lane];
+        scene = dd.scene;
+        if (gl < 3) {
+            const uint32_t vi = dd.idx[(size_t)(t - dd.tri_base) * 3 + gl];
             float4 p;
             load_vertex(dd, vi, p, v.a);
             v.x = p.x; v.y = p.y; v.z = p.z; v.w = p.w;
-        } else {
-            v.x = v.y = v.z = v.w = 0.0f;
-            for (int j = 0; j < 6; ++j) v.a[j] = 0.0f;
         }
-        uint32_t n = 3;
-        bool poly_overflow = false;
-        for (int plane = 0; plane < 6; ++plane) {
-            bool act = lane < n;
-            // clipping.rs:29-38
-            float dcur;
-            switch (plane) {
-                case 0: dcur = v.w + v.x; break;
-                case 1: dcur = v.w - v.x; break;
-                case 2: dcur = v.w + v.y; break;
-                case 3: dcur = v.w - v.y; break;
-                case 4: dcur = v.w + v.z; break;
-                default: dcur = v.w - v.z; break;
-            }
-            int prev = (lane == 0) ? (int)n - 1 : (int)lane - 1;
-            ClipV pv;
-            pv.x = __shfl(v.x, prev);
-            pv.y = __shfl(v.y, prev);
-            pv.z = __shfl(v.z, prev);
-            pv.w = __shfl(v.w, prev);
-            for (int j = 0; j < 6; ++j) pv.a[j] = __shfl(v.a[j], prev);
-            float dprev = __shfl(dcur, prev);
-            bool pin = dprev >= 0.0f, cin = dcur >= 0.0f;
-            uint32_t ncnt = act ? ((pin && cin) ? 1u : (pin ? 1u : (cin ? 2u : 0u))) : 0u;
-            uint64_t m1 = __ballot(ncnt >= 1);
-            uint64_t m2 = __ballot(ncnt == 2);
-            uint32_t pos = mbcnt(m1) + mbcnt(m2);
-            uint32_t newn = (uint32_t)__popcll(m1) + (uint32_t)__popcll(m2);
-            ClipV o0 = v;
-            if (act && (pin != cin)) {
-                // clipping.rs:43-51 with p0 = prev, p1 = cur
-                float alpha = dprev / (dprev - dcur);
-                float s1 = 1.0f - alpha;
-                o0.x = pv.x * s1 + v.x * alpha;
-                o0.y = pv.y * s1 + v.y * alpha;
-                o0.z = pv.z * s1 + v.z * alpha;
-                o0.w = pv.w * s1 + v.w * alpha;
-                // clipping.rs:149,161: (cur - prev) * alpha + prev
-                for (int j = 0; j < 6; ++j) o0.a[j] = (v.a[j] - pv.a[j]) * alpha + pv.a[j];
-            }
-            __syncthreads();
-            if (ncnt >= 1 && pos < CLIP_LDS_V) {
-                float* q = sh + pos * 10;
-                q[0] = o0.x; q[1] = o0.y; q[2] = o0.z; q[3] = o0.w;
-                for (int j = 0; j < 6; ++j) q[4 + j] = o0.a[j];
-            }
-            if (ncnt == 2 && pos + 1 < CLIP_LDS_V) {
-                float* q = sh + (pos + 1) * 10;
-                q[0] = v.x; q[1] = v.y; q[2] = v.z; q[3] = v.w;
-                for (int j = 0; j < 6; ++j) q[4 + j] = v.a[j];
-            }
-            __syncthreads();
-            n = newn;
-            if (n > 64) {  // more vertices than lanes: cannot continue
-                poly_overflow = true;
-                break;
-            }
-            if (n == 0) break;
-            if (lane < n) {
-                const float* q = sh + lane * 10;
-                v.x = q[0]; v.y = q[1]; v.z = q[2]; v.w = q[3];
-                for (int j = 0; j < 6; ++j) v.a[j] = q[4 + j];
-            }
+    }
+    uint32_t n = have ? 3u : 0u;  // the same in every lane of the group
+    bool big = false;             // needs more than G lanes
+    for (int plane = 0; plane < 6; ++plane) {
+        const bool live = n > 0 && !big;
+        const bool act = live && gl < n;
+        // clipping.rs:29-38
+        float dcur;
+        switch (plane) {
+            case 0: dcur = v.w + v.x; break;
+            case 1: dcur = v.w - v.x; break;
+            case 2: dcur = v.w + v.y; break;
+            case 3: dcur = v.w - v.y; break;
+            case 4: dcur = v.w + v.z; break;
+            default: dcur = v.w - v.z; break;
         }
-        uint32_t ntris = 0;
-        if (poly_overflow || n > MAXV) {
-            if (lane == 0) atomicOr(&b.cnt->overflow, OVF_POLY);
-        } else if (n >= 3) {
-            ntris = n - 2;
-            if (lane < n) {
-                float* q = b.clip_out + ((size_t)s * MAXV + lane) * 10;
-                q[0] = v.x; q[1] = v.y; q[2] = v.z; q[3] = v.w;
-                for (int j = 0; j < 6; ++j) q[4 + j] = v.a[j];
-            }
-        } else if (n > 0) {
-            if (lane == 0) atomicOr(&b.runs[dd.scene].flags, RZ_FLAG_CLIP_FEWER);
-        }
-        if (lane == 0) {
-            b.nsub[t] = ntris;
-            b.clip_n[s] = n;
+        const int prev = live ? (int)gb + (gl == 0 ? (int)n - 1 : (int)gl - 1) : (int)lane;
+        ClipV pv;
+        pv.x = __shfl(v.x, prev);
+        pv.y = __shfl(v.y, prev);
+        pv.z = __shfl(v.z, prev);
+        pv.w = __shfl(v.w, prev);
+        for (int j = 0; j < 6; ++j) pv.a[j] = __shfl(v.a[j], prev);
+        const float dprev = __shfl(dcur, prev);
+        const bool pin = dprev >= 0.0f, cin = dcur >= 0.0f;
+        const uint32_t ncnt = act ? ((pin && cin) ? 1u : (pin ? 1u : (cin ? 2u : 0u))) : 0u;
+        const uint64_t m1 = __ballot(ncnt >= 1) & gmask;
+        const uint64_t m2 = __ballot(ncnt == 2) & gmask;
+        const uint32_t pos = mbcnt(m1) + mbcnt(m2);
+        const uint32_t newn = (uint32_t)__popcll(m1) + (uint32_t)__popcll(m2);
+        ClipV o0 = v;
+        if (act && (pin != cin)) {
+            // clipping.rs:43-51 with p0 = prev, p1 = cur
+            const float alpha = dprev / (dprev - dcur);
+            const float s1 = 1.0f - alpha;
+            o0.x = pv.x * s1 + v.x * alpha;
+            o0.y = pv.y * s1 + v.y * alpha;
+            o0.z = pv.z * s1 + v.z * alpha;
+            o0.w = pv.w * s1 + v.w * alpha;
+            // clipping.rs:149,161: (cur - prev) * alpha + prev
+            for (int j = 0; j < 6; ++j) o0.a[j] = (v.a[j] - pv.a[j]) * alpha + pv.a[j];
         }
         __syncthreads();
+        if (ncnt >= 1 && pos < G) {
+            float* o = q + pos * 10;
+            o[0] = o0.x; o[1] = o0.y; o[2] = o0.z; o[3] = o0.w;
+            for (int j = 0; j < 6; ++j) o[4 + j] = o0.a[j];
+        }
+        if (ncnt == 2 && pos + 1 < G) {
+            float* o = q + (pos + 1) * 10;
+            o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+            for (int j = 0; j < 6; ++j) o[4 + j] = v.a[j];
+        }
+        __syncthreads();
+        if (live) {
+            n = newn;
+            if (n > G) big = true;
+            if (!big && gl < n) {
+                const float* o = q + gl * 10;
+                v.x = o[0]; v.y = o[1]; v.z = o[2]; v.w = o[3];
+                for (int j = 0; j < 6; ++j) v.a[j] = o[4 + j];
+            }
+        }
     }
+    if (!have) return;
+    if (big) {
+        if (gl == 0) {
+            if (G < 64) {
+                const uint32_t r = atomicAdd(&b.cnt->nredo, 1u);
+                b.clip_redo[r] = slot;  // r < clip slots: each slot is handed over at most once
+            } else {
+                atomicOr(&b.cnt->overflow, OVF_POLY);  // more vertices than lanes: cannot continue
+            }
+        }
+        return;
+    }
+    uint32_t ntris = 0;
+    if (n > MAXV) {
+        if (gl == 0) atomicOr(&b.cnt->overflow, OVF_POLY);
+    } else if (n >= 3) {
+        ntris = n - 2;
+        if (gl < n) {
+            float* o = b.clip_out + ((size_t)slot * MAXV + gl) * 10;
+            o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+            for (int j = 0; j < 6; ++j) o[4 + j] = v.a[j];
+        }
+    } else if (n > 0) {
+        if (gl == 0) atomicOr(&b.runs[scene].flags, RZ_FLAG_CLIP_FEWER);
+    }
+    if (gl == 0) {
+        b.nsub[t] = ntris;
+        b.clip_n[slot] = n;
+    }
+}
+
+// 4 clipped triangles per wave (with clip_wide set, every one goes to k_clip_redo)
+__global__ __launch_bounds__(64) void k_clip(JobBufs b) {
+    __shared__ float sh[64 * 10];
+    const uint32_t nslots = min(uniform(b.cnt->nclip), b.clip_cap);
+    if (b.clip_wide) {
+        for (uint32_t i = blockIdx.x * 64u + threadIdx.x; i < nslots; i += gridDim.x * 64u)
+            b.clip_redo[atomicAdd(&b.cnt->nredo, 1u)] = i;
+        return;
+    }
+    for (uint32_t base = blockIdx.x * 4u; base < nslots; base += gridDim.x * 4u) {
+        const uint32_t slot = base + (threadIdx.x >> 4);
+        clip_polygon<16>(b, slot, slot < nslots, sh);
+    }
+}
+
+// polygons k_clip could not finish: one wave each
+__global__ __launch_bounds__(64) void k_clip_redo(JobBufs b) {
+    __shared__ float sh[64 * 10];
+    const uint32_t nr = uniform(b.cnt->nredo);
+    for (uint32_t i = blockIdx.x; i < nr; i += gridDim.x) clip_polygon<64>(b, uniform(b.clip_redo[i]), true, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -1266,10 +1304,15 @@ __global__ __launch_bounds__(256) void k_raster(JobBufs b, const PrimGeom* __res
     const uint32_t nt = b.head->ntiles;
     const uint32_t blk = blockIdx.x, xcd = blk & 7u;
     // one workgroup per tile: workgroups go round-robin to the 8 XCDs, so
-    // each XCD gets a contiguous band of tiles (shared lists stay in its L2);
+    // workgroup 8k + x takes tile k of band x -- each XCD renders one
+    // contiguous band of tiles (shared lists stay in its L2); any grid of at
+    // least 8 * ceil(nt / 8) workgroups covers every tile, extra ones exit.
     // persistent: the same bands become 8 queues, drained XCD-local first
-    uint32_t tile = PERSIST ? NO_TILE : band_lo(gridDim.x, xcd) + (blk >> 3);
-    if (!PERSIST && tile >= nt) tile = NO_TILE;
+    uint32_t tile = NO_TILE;
+    if (!PERSIST) {
+        const uint32_t k = blk >> 3, lo = band_lo(nt, xcd);
+        if (k < band_lo(nt, xcd + 1) - lo) tile = lo + k;
+    }
     for (bool first = true;; first = false) {
         if (PERSIST) {
             if (threadIdx.x == 0) {
@@ -1328,7 +1371,8 @@ void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_s
     k_frame_reset<<<scene_blocks, 256, 0, s>>>(b);
     k_mvp<<<at_least1(g.draws), 256, 0, s>>>(b);
     k_classify<<<at_least1(g.tris), 256, 0, s>>>(b);
-    k_clip<<<at_least1(g.clip), 64, 0, s>>>(b);
+    k_clip<<<at_least1((g.clip + 3) / 4), 64, 0, s>>>(b);
+    k_clip_redo<<<64, 64, 0, s>>>(b);
     launch_scan<uint32_t>(s, b.nsub, b.suboff, b.tri_cap, &b.head->ntri, b.parts, &b.cnt->nprims_raw);
     k_scene_prep<<<1, 1024, 0, s>>>(b);
     k_setup<<<at_least1(g.tris), 256, 0, s>>>(b);
@@ -1345,7 +1389,7 @@ void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_s
     launch_scan<uint32_t>(s, b.fine_counts, b.fine_off, b.fmat_cap, &b.cnt->fmat_len, b.parts, &b.cnt->fine_total);
     k_fine_bin<true><<<at_least1(g.fine), 256, 0, s>>>(b);
     if (ev_raster) (void)hipEventRecord(ev_raster, s);
-    const uint32_t rgrid = g.raster ? g.raster : at_least1(g.ntiles);
+    const uint32_t rgrid = g.raster ? g.raster : at_least1((g.ntiles + 7) / 8 * 8);
 #define RZ_RASTER(ST, PE) \
     k_raster<ST, PE><<<rgrid, 256, 0, s>>>(b, b.geom, b.attrs, b.flist, b.fine_off, b.fpre, b.bin_rng, b.scenes, b.draws)
     if (raster_stats) {

@@ -194,6 +194,16 @@ class Context:
                                           v.ctypes.data_as(fp), p.ctypes.data_as(fp), int(texture), int(vs),
                                           int(fs)))
 
+    def batch_add(self, items) -> int:
+        """items: numpy array of _lib.batch_item_dtype() records. Returns the
+        index of the first scene added."""
+        import numpy as np
+
+        items = np.ascontiguousarray(items, dtype=L.batch_item_dtype())
+        first = C.c_uint32()
+        self._check(self._L.rz_batch_add(self.handle, C.c_void_p(items.ctypes.data), len(items), C.byref(first)))
+        return first.value
+
     def batch_submit(self):
         self._check(self._L.rz_batch_submit(self.handle))
 

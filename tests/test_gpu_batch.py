@@ -183,3 +183,38 @@ def test_batch_errors():
         m.destroy()
     finally:
         ctx.close()
+
+
+def test_batch_add_array_records_like_single_calls():
+    """rz_batch_add (one call for many scenes) records exactly what
+    rz_batch_scene + rz_batch_draw do, and adds nothing when an item is bad."""
+    import torch
+
+    from rusterizer_amd import _lib as L
+    from rusterizer_amd import batch
+    from rusterizer_amd.render import Context, RzError
+
+    scs = scenes.batch_scenes(n_scenes=6, seed=151, tri_lo=500, tri_hi=20000, resolutions=(256, 512), tex_size=64)
+    ref, _ = run_batch(scs, msaa=False)
+    ctx = Context(64, 64)
+    try:
+        items = batch.upload(ctx, scs)
+        table = batch.item_table(items)
+        batch.run(ctx, items, table)
+        for sc, r, it in zip(scs, ref, items):
+            assert np.array_equal(batch.image(it), r["img"]), sc.name
+        bad = table.copy()
+        bad[3]["texture"] = 10_000  # not bound
+        ctx.batch_begin()
+        with pytest.raises(RzError):
+            ctx.batch_add(bad)
+        first = ctx.batch_add(table[:2])  # nothing of the failed call was added
+        assert first == 0
+        ctx.batch_submit()
+        ctx.batch_wait()
+        assert ctx.frame_stats()["tris"] == scs[0].ntri + scs[1].ntri
+        for r in items:
+            r.mesh.destroy()
+        assert table.dtype == L.batch_item_dtype()
+    finally:
+        ctx.close()

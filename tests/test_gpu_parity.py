@@ -190,3 +190,17 @@ def test_zfight_redraw(dz):
     b.pos = (b.pos + np.array([0.0, 0.0, dz], np.float32)).astype(np.float32)
     b.color = np.ascontiguousarray(b.color[:, ::-1])
     check([a, b])
+
+
+def test_wide_clipper_same_results(monkeypatch):
+    """Clipping with 16 lanes per polygon (default) and with a full wave
+    (RZ_CLIP_WIDE=1, the path for polygons that outgrow 16 lanes) give the
+    same frame, equal to the oracle."""
+    sc = scenes.soup_scene(30_000, 640, 360, seed=1013, clip_frac=0.6)
+    narrow = run_gpu(sc)
+    monkeypatch.setenv("RZ_CLIP_WIDE", "1")
+    wide = run_gpu(sc)
+    assert narrow["stats"]["clipped"] > 10_000
+    assert np.array_equal(narrow["img"], wide["img"])
+    assert np.array_equal(narrow["dep"].view(np.uint32), wide["dep"].view(np.uint32))
+    assert_parity(wide, run_oracle(sc), where="wide clipper")
