@@ -9,10 +9,12 @@ replay), inputs resident in HBM. Beside it, on the same scenes:
   - the plain path: one renderer per scene, rz_draw + rz_resolve_device per
     scene, one scene per launch sequence;
   - the CPU oracle (single thread) on a bounded sample of the scenes.
-The 1M-triangle soup frame (1920x1080, colour FS, seed 1001) is timed too
-and reported under "soup".
+The plain frames of the 1M-triangle soup (1920x1080, colour FS, seed 1001)
+and of the 4M-triangle terrain (3840x2160, 4096^2 texture, seed 1003) are
+timed too and reported under "soup" and "terrain"; --workload soup|terrain
+makes one of them the headline (with its CPU baseline).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch|soup] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch|soup|terrain] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -36,7 +38,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("batch", "soup"), default="batch")
+    ap.add_argument("--workload", choices=("batch", "soup", "terrain"), default="batch")
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--tris", type=int, default=1_000_000, help="soup triangles")
     ap.add_argument("--width", type=int, default=1920)
@@ -116,16 +118,19 @@ def roofline(alg_bytes: int, raster_ms: float, traffic):
             "raster_ms": raster_ms}
 
 
-# --------------------------------------------------------------------- soup
-def bench_soup(a, steps, warmup, with_cpu):
+# ------------------------------------------------------------ plain frames
+def bench_frame(sc, steps, warmup, cpu: str | None, traffic_name: str, cpu_sample_tris: int = 300_000):
+    """One scene drawn and resolved per step through the plain API.
+    cpu: None, "sample" (first cpu_sample_tris triangles, time scaled --
+    for i.i.d. soups) or "full" (the whole frame once)."""
     import torch
 
-    from rusterizer_amd import scenes
     from rusterizer_amd.render import Context
 
-    seed = a.seed if (a.seed is not None and a.workload == "soup") else scenes.SEEDS["soup"]
-    sc = scenes.soup_scene(a.tris, a.width, a.height, seed=seed)
     ctx = Context(sc.width, sc.height)
+    ctx.set_clear_depth(sc.clear_depth)
+    if sc.texture is not None:
+        ctx.bind_texture(0, sc.texture)
     ctx.set_uniforms(sc.world, sc.view, sc.proj)
     mesh = ctx.mesh(sc.pos, sc.color, sc.uv, sc.idx)
     out = torch.empty(sc.width * sc.height, dtype=torch.int32, device="cuda")
@@ -155,19 +160,18 @@ def bench_soup(a, steps, warmup, with_cpu):
     frames = prof["frames"] or steps
     raster_ms = prof["raster"] / frames
     res = {
-        "workload": f"triangle soup {sc.ntri} tris, {sc.width}x{sc.height}, 4x MSAA, colour FS, "
-                    f"f32 depth LESS, seed {seed}",
         "samples_per_s": sc.samples * steps / dt,
         "triangles_per_s": sc.ntri * steps / dt,
         "ms_per_frame": dt / steps * 1e3,
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
         "stage_ms": {k: prof[k] / frames for k in ("geometry", "binning", "raster")},
         "frame_stats": stats,
+        "clipped_share": stats["clipped"] / max(sc.ntri, 1),
         "warmup_replays": [(w["replays"], w["overflow_bits"]) for w in warm],
-        "roofline": roofline(sc.algorithmic_bytes(), raster_ms, load_traffic("traffic_raster_soup1m.json")),
+        "roofline": roofline(sc.algorithmic_bytes(), raster_ms, load_traffic(traffic_name)),
     }
-    if with_cpu:
-        n = min(a.cpu_sample_tris, sc.ntri)
+    if cpu == "sample":
+        n = min(cpu_sample_tris, sc.ntri)
         t = oracle_frame_seconds(sc, n)
         frame_s = t * sc.ntri / max(n, 1)
         res["cpu_baseline"] = {
@@ -175,9 +179,33 @@ def bench_soup(a, steps, warmup, with_cpu):
             "sample": f"first {n} of {sc.ntri} triangles rendered in {t:.2f} s on 1 host thread; "
                       f"frame time scaled by {sc.ntri / max(n, 1):.2f}",
             "triangles_per_s": sc.ntri / frame_s, "host_cpu": _cpu_model(), "host_nproc": os.cpu_count()}
+    elif cpu == "full":
+        t = oracle_frame_seconds(sc)
+        res["cpu_baseline"] = {
+            "value": sc.samples / t, "unit": "MSAA samples/s", "cores": 1, "kind": "port",
+            "sample": f"the whole frame ({sc.ntri} triangles) rendered in {t:.2f} s on 1 host thread",
+            "triangles_per_s": sc.ntri / t, "host_cpu": _cpu_model(), "host_nproc": os.cpu_count()}
     mesh.destroy()
     ctx.close()
     return res
+
+
+def soup_scene_of(a):
+    from rusterizer_amd import scenes
+
+    seed = a.seed if (a.seed is not None and a.workload == "soup") else scenes.SEEDS["soup"]
+    sc = scenes.soup_scene(a.tris, a.width, a.height, seed=seed)
+    return sc, (f"triangle soup {sc.ntri} tris, {sc.width}x{sc.height}, 4x MSAA, colour FS, "
+                f"f32 depth LESS, seed {seed}")
+
+
+def terrain_scene_of(a):
+    from rusterizer_amd import scenes
+
+    seed = a.seed if (a.seed is not None and a.workload == "terrain") else scenes.SEEDS["terrain"]
+    sc = scenes.terrain_scene(seed=seed)
+    return sc, (f"terrain {sc.ntri} tris (1415x1415 quads, fBm heights), 3840x2160, 4096^2 checkerboard, "
+                f"4x MSAA, bilinear texture FS, f32 depth LESS, seed {seed}")
 
 
 # -------------------------------------------------------------------- batch
@@ -352,19 +380,31 @@ def main():
         copy_probe()
         return
     with_cpu = not a.no_cpu
-    if a.workload == "soup":
-        s = bench_soup(a, a.steps, a.warmup, with_cpu)
+    if a.workload in ("soup", "terrain"):
+        if a.workload == "soup":
+            sc, desc = soup_scene_of(a)
+            s = bench_frame(sc, a.steps, a.warmup, "sample" if with_cpu else None, "traffic_raster_soup1m.json",
+                            a.cpu_sample_tris)
+            cfg = {"workload": desc, "triangles": sc.ntri, "width": sc.width, "height": sc.height, "msaa": 4}
+        else:
+            sc, desc = terrain_scene_of(a)
+            s = bench_frame(sc, a.steps, a.warmup, "full" if with_cpu else None, "traffic_raster_terrain.json")
+            cfg = {"workload": desc, "triangles": sc.ntri, "width": sc.width, "height": sc.height, "msaa": 4,
+                   "texture": 4096}
         line = {"metric": METRIC, "value": s["samples_per_s"], "unit": "MSAA samples/s", "n_gpus": 1,
                 "steps": a.steps, "warmup": a.warmup, "ms_per_step": s["ms_per_frame"], "higher_is_better": True,
-                "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-                "config": {"workload": s["workload"], "triangles": a.tris, "width": a.width, "height": a.height,
-                           "msaa": 4},
+                "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": cfg,
                 "triangles_per_s": s["triangles_per_s"], "latency_ms": s["latency_ms"],
-                "roofline": s["roofline"], "cpu_baseline": s.get("cpu_baseline"), "soup": s}
+                "roofline": s["roofline"], "cpu_baseline": s.get("cpu_baseline"), a.workload: s}
         print(json.dumps(line), flush=True)
         return
     b = bench_batch(a, a.steps, a.warmup, with_cpu, not a.no_plain, not a.no_tune)
-    soup = bench_soup(a, max(5, a.steps // 2), 2, False)
+    sc, desc = soup_scene_of(a)
+    soup = dict(workload=desc, **bench_frame(sc, max(5, a.steps // 2), 2, None, "traffic_raster_soup1m.json"))
+    log(f"soup: {soup['ms_per_frame']:.2f} ms per frame")
+    sc, desc = terrain_scene_of(a)
+    terrain = dict(workload=desc, **bench_frame(sc, max(5, a.steps // 2), 2, None, "traffic_raster_terrain.json"))
+    log(f"terrain: {terrain['ms_per_frame']:.2f} ms per frame")
     line = {
         "metric": METRIC, "value": b["samples_per_s"], "unit": "MSAA samples/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": b["ms_per_batch"], "higher_is_better": True,
@@ -379,6 +419,7 @@ def main():
         "cpu_baseline": b.get("cpu_baseline"),
         "batch": b,
         "soup": soup,
+        "terrain": terrain,
     }
     print(json.dumps(line), flush=True)
 

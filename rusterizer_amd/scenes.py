@@ -193,10 +193,20 @@ def _fbm(rng, n, octaves=6):
 
 
 def terrain_scene(quads: int = 1415, width: int = 3840, height: int = 2160, seed: int = SEEDS["terrain"],
-                  tex_size: int = 4096, cam_height: float = 0.12, near: float = 0.05) -> Scene:
+                  tex_size: int = 4096, detail: float = 0.038, height_scale: float = 0.1,
+                  cam_height: float = 4.0) -> Scene:
+    """A (quads x quads)-quad grid over [-1, 1] x [0, 2] (x, z) with seeded
+    heights: 6-octave value-noise fBm plus a per-vertex detail octave of
+    relative amplitude `detail`, all scaled by `height_scale`. uv spans the
+    grid once; a tex_size^2 checkerboard (64 x 64 checks). The camera looks
+    straight down from cam_height above the grid centre with a lens whose
+    half-width at the ground is 1 unit (the grid fills the screen width), and
+    its near plane lies at the terrain's mean level: the terrain above it is
+    cut away, and the triangles the near plane slices through take the
+    clipping path (about 10% of the full-size grid's 4,004,450 triangles)."""
     rng = np.random.default_rng(seed)
     n = quads + 1
-    h = _fbm(rng, n) * 0.08
+    h = (_fbm(rng, n) * 0.08 + rng.uniform(-1.0, 1.0, (n, n)) * detail) * height_scale
     gx, gz = np.meshgrid(np.linspace(-1.0, 1.0, n), np.linspace(0.0, 2.0, n))
     pos = np.stack([gx, h, gz], axis=-1).reshape(-1, 3).astype(np.float32)
     uu, vv = np.meshgrid(np.linspace(0.0, 1.0, n), np.linspace(0.0, 1.0, n))
@@ -206,7 +216,24 @@ def terrain_scene(quads: int = 1415, width: int = 3840, height: int = 2160, seed
     b, c, d = a + 1, a + n + 1, a + n
     # two triangles per quad, wound to face the camera above the terrain
     idx = np.stack([a, c, b, a, d, c], axis=-1).reshape(-1).astype(np.uint32)
-    view = hm.Camera(pos=(0.0, cam_height, -0.15), up=(0.0, 1.0, 0.0), dir=(0.0, -0.35, 1.0)).get_view_matrix()
-    proj = hm.project(F(near), F(10.0), F(height) / F(width), F(math.pi / 2))
+    view = hm.Camera(pos=(0.0, cam_height, 1.0), up=(0.0, 0.0, 1.0), dir=(0.0, -1.0, 0.0)).get_view_matrix()
+    fov = F(2.0 * math.atan(1.0 / cam_height))
+    proj = hm.project(F(cam_height), F(2.0 * cam_height), F(height) / F(width), fov)
     tex = meshes.checkerboard(tex_size, 64)
     return Scene("terrain", width, height, pos, None, uv, idx, hm.identity(), view, proj, FS_TEXTURE, tex)
+
+
+def classify_fractions(sc: Scene) -> dict:
+    """Shares of the scene's triangles by the reference's pre-clip
+    classification (clipping.rs:62-112), from numpy f32 clip coordinates:
+    degenerate, outside (trivial reject), inside (trivial accept), clipped."""
+    M = np.asarray(hm.mat_mul(hm.mat_mul(sc.proj, sc.view), sc.world), np.float32).reshape(4, 4)
+    p = np.concatenate([sc.pos, np.ones((len(sc.pos), 1), np.float32)], 1) @ M.T
+    t = p[sc.idx.reshape(-1, 3)]
+    x, y, z, w = t[..., 0], t[..., 1], t[..., 2], t[..., 3]
+    a2 = (x[:, 1] - x[:, 0]) * (y[:, 2] - y[:, 0]) - (x[:, 2] - x[:, 0]) * (y[:, 1] - y[:, 0])
+    deg = np.abs(a2) < 1e-6
+    inside = ((x >= -w) & (y >= -w) & (z >= -w) & (x <= w) & (y <= w) & (z <= w)).all(1)
+    out = ((x < -w).all(1) | (y < -w).all(1) | (z < -w).all(1) | (x > w).all(1) | (y > w).all(1) | (z > w).all(1))
+    return {"degenerate": float(deg.mean()), "outside": float((out & ~deg).mean()),
+            "inside": float((inside & ~deg).mean()), "clipped": float((~deg & ~out & ~inside).mean())}

@@ -204,3 +204,18 @@ def test_wide_clipper_same_results(monkeypatch):
     assert np.array_equal(narrow["img"], wide["img"])
     assert np.array_equal(narrow["dep"].view(np.uint32), wide["dep"].view(np.uint32))
     assert_parity(wide, run_oracle(sc), where="wide clipper")
+
+
+def test_terrain_small():
+    """The terrain generator at reduced size: near-plane slicing of a
+    detailed height field, textured."""
+    sc = scenes.terrain_scene(quads=160, width=640, height=360, tex_size=256)
+    g, _ = check(sc)
+    assert g["stats"]["clipped"] > 0.05 * sc.ntri
+    assert (g["img"] != 0xFF191919).mean() > 0.2
+
+
+def test_terrain_full_size():
+    """The 4M-triangle 3840x2160 terrain with its 4096^2 texture, full frame."""
+    sc = scenes.terrain_scene()
+    check(sc)

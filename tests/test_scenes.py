@@ -1,0 +1,42 @@
+"""Properties of the synthetic workloads (CPU): sizes, seeds, the share of
+triangles on the clipping path, front-facing winding."""
+import numpy as np
+
+from rusterizer_amd import scenes
+
+
+def test_terrain_full_size_shape_and_clip_share():
+    sc = scenes.terrain_scene()
+    assert sc.ntri == 4_004_450 and len(sc.pos) == 2_005_056
+    assert sc.texture.shape == (4096, 4096, 4) and (sc.width, sc.height) == (3840, 2160)
+    assert sc.algorithmic_bytes() == 188_440_984
+    f = scenes.classify_fractions(sc)
+    assert 0.08 <= f["clipped"] <= 0.12, f
+    assert f["degenerate"] == 0.0
+
+
+def test_terrain_is_seeded():
+    a = scenes.terrain_scene(quads=64, width=320, height=180, tex_size=64)
+    b = scenes.terrain_scene(quads=64, width=320, height=180, tex_size=64)
+    c = scenes.terrain_scene(quads=64, width=320, height=180, tex_size=64, seed=7)
+    assert np.array_equal(a.pos, b.pos) and not np.array_equal(a.pos, c.pos)
+
+
+def test_soup_clip_share_and_winding():
+    sc = scenes.soup_scene(50_000, 1920, 1080, seed=scenes.SEEDS["soup"])
+    f = scenes.classify_fractions(sc)
+    # 10% are built to cross the near plane; a few more cross the screen edges
+    assert 0.10 <= f["clipped"] <= 0.16, f
+
+
+def test_batch_scenes_distribution():
+    scs = scenes.batch_scenes(256)
+    sizes = {s.width for s in scs}
+    assert sizes == {256, 512, 768, 1024}
+    assert all(s.width == s.height for s in scs)
+    tris = np.array([s.ntri for s in scs])
+    assert tris.min() >= 1000 and tris.max() <= 200_000
+    assert 8e6 < tris.sum() < 11e6  # ~9.6M expected for log-uniform 1k..200k
+    assert all(s.texture.shape == (512, 512, 4) for s in scs)
+    pal = {s.texture[0, 0].tobytes() for s in scs}
+    assert len(pal) > 200  # per-scene palettes
