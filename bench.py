@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-scene leg (batch workload)")
     ap.add_argument("--no-tune", action="store_true", help="use the default batch tuning")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson request rate (1/s)")
+    ap.add_argument("--serve-requests", type=int, default=4000)
     ap.add_argument("--cpu-sample-tris", type=int, default=300_000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy (HBM counter calibration)")
@@ -209,17 +212,21 @@ def terrain_scene_of(a):
 
 
 # -------------------------------------------------------------------- batch
-def bench_batch(a, steps, warmup, with_cpu, with_plain, tune):
+def batch_scenes_of(a):
+    from rusterizer_amd import scenes
+
+    seed = a.seed if (a.seed is not None and a.workload == "batch") else scenes.SEEDS["batch"]
+    t = time.perf_counter()
+    scs = scenes.batch_scenes(a.scenes, seed=seed)
+    return scs, seed, time.perf_counter() - t
+
+
+def bench_batch(a, steps, warmup, with_cpu, with_plain, tune, scs, seed, t_gen):
     import torch
 
-    from rusterizer_amd import autotune, batch, scenes
+    from rusterizer_amd import autotune, batch
     from rusterizer_amd.render import Context
 
-    seed = a.seed if a.seed is not None else scenes.SEEDS["batch"]
-    t_gen = time.perf_counter()
-    scs = scenes.batch_scenes(a.scenes, seed=seed)
-    t_gen = time.perf_counter() - t_gen
-    log(f"generated {len(scs)} scenes in {t_gen:.1f} s")
     samples = sum(s.samples for s in scs)
     tris = sum(s.ntri for s in scs)
     alg_bytes = sum(s.algorithmic_bytes() for s in scs)
@@ -297,6 +304,51 @@ def bench_batch(a, steps, warmup, with_cpu, with_plain, tune):
         r.mesh.destroy()
     ctx.close()
     return res
+
+
+SERVE_WEIGHTS = {"batch": 0.90, "soup": 0.07, "terrain": 0.03}
+
+
+def bench_serve(kinds: dict, tuning: dict | None, rate: float, n: int, seed: int, slots: int = 3,
+                budget: float = 200e9):
+    """Poisson request stream over resident templates of the three kinds,
+    continuous batching over `slots` contexts; then D2H of one image of each
+    kind, timed separately."""
+    import torch
+
+    from rusterizer_amd.render import Context
+    from rusterizer_amd.serve import Catalog, Server, make_requests
+
+    root = Context(64, 64)
+    cat = Catalog(root, kinds)
+    srv = Server(cat, slots=slots, budget_bytes=budget, max_batch=64, tuning=tuning)
+    try:
+        warm = make_requests(cat, min(400, n), rate, SERVE_WEIGHTS, seed + 100)
+        srv.run(warm)  # buffers grow, graphs are captured
+        reqs = make_requests(cat, n, rate, SERVE_WEIGHTS, seed)
+        rep = srv.run(reqs, keep_outputs=False)
+        d2h = {}
+        for kind, ids in cat.by_kind.items():
+            t = cat.templates[ids[0]]
+            dev = torch.empty(t.pixels, dtype=torch.int32, device="cuda")
+            host = torch.empty(t.pixels, dtype=torch.int32, pin_memory=True)
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(10):
+                t0 = time.perf_counter()
+                host.copy_(dev)
+                ts.append(time.perf_counter() - t0)
+            ts.sort()
+            d2h[kind] = {"bytes": t.pixels * 4, "ms_p50": ts[5] * 1e3, "GB_per_s": t.pixels * 4 / ts[5] / 1e9}
+        rep["d2h_per_image"] = d2h
+        rep["weights"] = SERVE_WEIGHTS
+        rep["seed"] = seed
+        rep["rate"] = rate
+        return rep
+    finally:
+        srv.close()
+        cat.destroy()
+        root.close()
 
 
 def bench_plain(scs, items, steps: int):
@@ -398,13 +450,23 @@ def main():
                 "roofline": s["roofline"], "cpu_baseline": s.get("cpu_baseline"), a.workload: s}
         print(json.dumps(line), flush=True)
         return
-    b = bench_batch(a, a.steps, a.warmup, with_cpu, not a.no_plain, not a.no_tune)
-    sc, desc = soup_scene_of(a)
-    soup = dict(workload=desc, **bench_frame(sc, max(5, a.steps // 2), 2, None, "traffic_raster_soup1m.json"))
+    scs, bseed, t_gen = batch_scenes_of(a)
+    log(f"generated {len(scs)} scenes in {t_gen:.1f} s")
+    b = bench_batch(a, a.steps, a.warmup, with_cpu, not a.no_plain, not a.no_tune, scs, bseed, t_gen)
+    soup_sc, desc = soup_scene_of(a)
+    soup = dict(workload=desc, **bench_frame(soup_sc, max(5, a.steps // 2), 2, None, "traffic_raster_soup1m.json"))
     log(f"soup: {soup['ms_per_frame']:.2f} ms per frame")
-    sc, desc = terrain_scene_of(a)
-    terrain = dict(workload=desc, **bench_frame(sc, max(5, a.steps // 2), 2, None, "traffic_raster_terrain.json"))
+    terr_sc, desc = terrain_scene_of(a)
+    terrain = dict(workload=desc, **bench_frame(terr_sc, max(5, a.steps // 2), 2, None,
+                                                "traffic_raster_terrain.json"))
     log(f"terrain: {terrain['ms_per_frame']:.2f} ms per frame")
+    serve = None
+    if not a.no_serve:
+        from rusterizer_amd import scenes as S
+
+        serve = bench_serve({"batch": scs, "soup": [soup_sc], "terrain": [terr_sc]}, b["tuning"], a.serve_rate,
+                            a.serve_requests, S.SEEDS["serve"])
+        log(f"serve: p50 {serve['latency_ms']['p50']:.2f} ms, p99 {serve['latency_ms']['p99']:.2f} ms")
     line = {
         "metric": METRIC, "value": b["samples_per_s"], "unit": "MSAA samples/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": b["ms_per_batch"], "higher_is_better": True,
@@ -413,6 +475,7 @@ def main():
                    "samples_per_batch": b["samples_per_batch"], "triangles_per_batch": b["tris_per_batch"]},
         "triangles_per_s": b["triangles_per_s"],
         "latency_ms": b["latency_ms"],
+        "serving_latency_ms": serve["latency_ms"] if serve else None,
         "batched_vs_plain": b.get("batched_vs_plain"),
         "batched_vs_cpu": b.get("batched_vs_cpu"),
         "roofline": b["roofline"],
@@ -420,6 +483,7 @@ def main():
         "batch": b,
         "soup": soup,
         "terrain": terrain,
+        "serve": serve,
     }
     print(json.dumps(line), flush=True)
 

@@ -1,0 +1,109 @@
+"""Serving (continuous batching): every request's image is the same as when
+its scene is rendered alone, while requests of different kinds and sizes
+share batches and batches overlap on the device; admission respects the
+HBM budget."""
+import numpy as np
+import pytest
+
+from rusterizer_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _catalog():
+    from rusterizer_amd.render import Context
+    from rusterizer_amd.serve import Catalog
+
+    root = Context(64, 64)
+    kinds = {
+        "batch": scenes.batch_scenes(n_scenes=12, seed=201, tri_lo=500, tri_hi=20000, resolutions=(128, 256, 384),
+                                     tex_size=64),
+        "soup": [scenes.soup_scene(40_000, 640, 360, seed=202)],
+        "terrain": [scenes.terrain_scene(quads=150, width=480, height=270, tex_size=256)],
+    }
+    return root, Catalog(root, kinds)
+
+
+def _alone(cat):
+    """Each template rendered alone (a batch of one scene) on the root context."""
+    import torch
+
+    from rusterizer_amd import _lib as L
+
+    out = {}
+    for i, t in enumerate(cat.templates):
+        img = torch.empty(t.pixels, dtype=torch.int32, device="cuda")
+        row = np.array([t.row], dtype=L.batch_item_dtype())
+        row[0]["argb_dev"] = img.data_ptr()
+        cat.root.batch_begin()
+        cat.root.batch_add(row)
+        cat.root.batch_submit()
+        cat.root.batch_wait()
+        out[i] = img
+    return out
+
+
+def test_serving_mix_equals_alone():
+    import torch
+
+    from rusterizer_amd.serve import Server, make_requests
+
+    root, cat = _catalog()
+    try:
+        alone = _alone(cat)
+        reqs = make_requests(cat, 400, rate=40_000.0, weights={"batch": 0.8, "soup": 0.1, "terrain": 0.1}, seed=7)
+        srv = Server(cat, slots=3, max_batch=12)
+        try:
+            rep = srv.run(reqs, keep_outputs=True)
+        finally:
+            for r in reqs:
+                if r.out is not None:
+                    assert torch.equal(r.out, alone[r.tmpl]), (r.rid, cat.templates[r.tmpl].kind)
+            srv.close()
+        assert rep["requests"] == 400 and rep["rejected"] == 0
+        assert set(rep["kinds"]) == {"batch", "soup", "terrain"}
+        assert rep["batches"] >= 400 // 12
+        assert rep["max_batches_in_flight"] >= 2  # batches overlapped on the device
+        mixed = {}
+        for r in reqs:
+            mixed.setdefault(r.batch, set()).add(cat.templates[r.tmpl].kind)
+        assert any(len(k) >= 2 for k in mixed.values())  # kinds shared batches
+    finally:
+        cat.destroy()
+        root.close()
+
+
+def test_serving_budget_admission():
+    import torch
+
+    from rusterizer_amd.serve import Server, make_requests
+
+    root, cat = _catalog()
+    try:
+        alone = _alone(cat)
+        big = max(cat.templates, key=lambda t: t.scene.ntri)
+        srv0 = Server(cat, slots=1)
+        need_big = srv0.est_bytes(big)
+        srv0.close()
+        budget = cat.resident_bytes + 1.5 * need_big
+        reqs = make_requests(cat, 120, rate=40_000.0, weights={"batch": 0.6, "soup": 0.2, "terrain": 0.2}, seed=8)
+        srv = Server(cat, slots=3, budget_bytes=budget, max_batch=12)
+        try:
+            rep = srv.run(reqs, keep_outputs=True)
+            for r in reqs:
+                assert torch.equal(r.out, alone[r.tmpl]), r.rid
+        finally:
+            srv.close()
+        assert rep["peak_admitted_bytes"] <= budget
+        assert rep["requests"] == 120
+        # a request larger than the whole budget is rejected, not waited on forever
+        reqs2 = make_requests(cat, 10, rate=40_000.0, weights={"terrain": 1.0}, seed=9)
+        srv = Server(cat, slots=2, budget_bytes=cat.resident_bytes + 1000)
+        try:
+            rep2 = srv.run(reqs2)
+        finally:
+            srv.close()
+        assert rep2["rejected"] == 10 and rep2["requests"] == 0
+    finally:
+        cat.destroy()
+        root.close()
