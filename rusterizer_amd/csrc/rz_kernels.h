@@ -27,6 +27,10 @@ constexpr uint32_t ATTR_STRIDE = 12;  // floats of attributes per prim
 constexpr int SCAN_ITEMS = 8;
 constexpr uint32_t SCAN_TILE = 256 * SCAN_ITEMS;
 constexpr uint32_t NO_TILE = 0xFFFFFFFFu;
+constexpr uint32_t NO_PRIM = 0xFFFFFFFFu;
+
+// PrimGeom.flags bit: texture coordinates not proven in bounds (see uv_in_bounds)
+constexpr uint32_t PRIM_UV_CHECK = 8;
 
 // overflow bits: the job must be replayed with larger buffers
 constexpr uint32_t OVF_PRIMS = 1, OVF_CLIP = 2, OVF_PAIRS = 4, OVF_POLY = 8, OVF_FINE = 16, OVF_MATRIX = 32;
@@ -118,7 +122,7 @@ struct PrimGeom {
     float inv_area;                      // 1 / (2 * area)
     float z0, z1, z2;                    // screen-space depth
     float w0, w1, w2;                    // clip-space w
-    uint32_t flags;                      // tie-rule bit per edge (0..2) | fs << 8 | draw << 16
+    uint32_t flags;                      // tie-rule bit per edge (0..2) | PRIM_UV_CHECK | fs << 8 | draw << 16
     uint32_t bbx, bby;                   // clamped pixel bbox: lo | hi << 16
     float zlo;                           // lower bound of every interpolated depth
     float pz0, pzx, pzy;                 // depth lower-bound plane: pz0 + pzx*(x-bx0) + pzy*(y-by0)

@@ -626,6 +626,29 @@ __device__ void depth_plane_bound(PrimGeom& o, uint32_t x0, uint32_t x1, uint32_
     o.pzy = ok ? (float)py : 0.0f;
 }
 
+// True when no fragment of the prim can address a texel past the texture,
+// whatever its size: all w > 0 and every uv component finite and at most
+// 1 - 2^-19. Proof sketch: a fragment is shaded at a covered sample (edge
+// values >= 0) or, when all 4 samples pass, at the pixel centre, whose edge
+// values are > 0 (the centre is the mean of the 4 samples and each sample's
+// edge value errs by at most 4u*M, M <= 1.42*|n|*32768 < |n|/16, while the
+// samples' spread along any edge normal is >= 0.28*|n|). So the perspective
+// weights f_k = e_k / w_k are >= 0, u + v <= 1 + 2^-22, u + v + w <= 1 + 2^-22
+// and tu <= max(a) * (1 + 2^-21); then tu * (tw - 1) < tw - 1, every texel
+// coordinate is <= tw - 1 (th - 1 for v) and the index x + y*tw < tw*th.
+// Negative and NaN coordinates clamp to texel 0. Prims that fail the test
+// have their fragments checked exactly in the raster.
+__device__ __forceinline__ bool uv_in_bounds(const ClipV* tv, const float* w) {
+    const float lim = 1.0f - 0x1p-19f;
+    bool ok = (w[0] > 0.0f) & (w[1] > 0.0f) & (w[2] > 0.0f);
+    for (int k = 0; k < 3; ++k)
+        for (int j = 4; j < 6; ++j) {
+            const float a = tv[k].a[j];
+            ok &= (fabsf(a) <= FLT_MAX_F) & (a <= lim);
+        }
+    return ok;
+}
+
 __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t H, uint32_t draw, uint32_t fs,
                                            uint32_t cb, PrimGeom* __restrict__ g, float* __restrict__ attr,
                                            uint4* __restrict__ pbin, uint32_t& pairs) {
@@ -667,6 +690,7 @@ __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t
     o.bby = y0 | (y1 << 16);
     o.flags = (tie_rule(o.nx0, o.ny0) ? 1u : 0u) | (tie_rule(o.nx1, o.ny1) ? 2u : 0u) |
               (tie_rule(o.nx2, o.ny2) ? 4u : 0u) | (fs << 8) | (draw << 16);
+    if (fs == RZ_FS_TEXTURE && !uv_in_bounds(tv, w)) o.flags |= PRIM_UV_CHECK;
     // Depth of a covered sample is ((b0*z0 + b1*z1) + b2*z2) with clamped
     // b_i in [0, 1] and 1 - 2^-23 <= b0+b1+b2 <= 2; three roundings later it
     // is >= zlo below, so a sample can pass LESS only where zbuf > zlo.
@@ -1072,6 +1096,85 @@ __device__ __forceinline__ uint32_t band_lo(uint32_t nt, uint32_t x) {
     return x * q + min(x, r);
 }
 
+// mod.rs:70-99: perspective-correct barycentrics of a fragment, at the
+// pixel centre when all four samples passed the depth test, else at the
+// first passing sample
+__device__ __forceinline__ void frag_bary(const PrimGeom& g, float fx, float fy, uint32_t pass, float& u, float& v,
+                                          float& w) {
+    float ix = fx + 0.5f, iy = fy + 0.5f;
+    if (pass != 0xFu) {
+        const int k = __builtin_ctz(pass);
+        ix = fx + sample_x(k);
+        iy = fy + sample_y(k);
+    }
+    const float ef0 = edge(g.nx0, g.ny0, g.ax0, g.ay0, ix, iy);
+    const float ef1 = edge(g.nx1, g.ny1, g.ax1, g.ay1, ix, iy);
+    const float ef2 = edge(g.nx2, g.ny2, g.ax2, g.ay2, ix, iy);
+    const float f_u = ef1 / g.w0;
+    const float f_v = ef2 / g.w1;
+    const float f_w = ef0 / g.w2;
+    const float sum = f_u + f_v + f_w;
+    u = clamp_bary(f_u / sum);
+    v = clamp_bary(f_v / sum);
+    w = clamp_bary(1.0f - u - v);
+}
+
+// texture.rs:65-83 coordinates of (tu, tv): true if one of the four texel
+// reads would fall past the texture (the read_texel condition)
+__device__ __forceinline__ bool texel_oob(uint32_t tw, uint32_t th, float tu, float tv) {
+    const float x = tu * (float)(tw - 1);
+    const float y = tv * (float)(th - 1);
+    const uint32_t xf0 = sat_u32(floorf(x)), xc = sat_u32(ceilf(x));
+    const uint32_t yf0 = sat_u32(floorf(y)), yc = sat_u32(ceilf(y));
+    const uint64_t n = (uint64_t)tw * th;
+    auto past = [&](uint32_t X, uint32_t Y) { return Y >= th || (uint64_t)X >= n - (uint64_t)Y * tw; };
+    return past(xf0, yf0) | past(xc, yf0) | past(xf0, yc) | past(xc, yc);
+}
+
+// FragCoords.depths[0] (mod.rs:225-262): the prim's depth at sample 0 when
+// the prim covers that sample, else 0.0
+__device__ __forceinline__ float sample0_depth(const PrimGeom& g, float fx, float fy) {
+    const float sx = fx + sample_x(0), sy = fy + sample_y(0);
+    const float e0 = edge(g.nx0, g.ny0, g.ax0, g.ay0, sx, sy);
+    const float e1 = edge(g.nx1, g.ny1, g.ax1, g.ay1, sx, sy);
+    const float e2 = edge(g.nx2, g.ny2, g.ax2, g.ay2, sx, sy);
+    const bool t0 = g.flags & 1u, t1 = g.flags & 2u, t2 = g.flags & 4u;
+    const bool in = ((e0 > 0.0f) | (t0 & !(e0 < 0.0f))) & ((e1 > 0.0f) | (t1 & !(e1 < 0.0f))) &
+                    ((e2 > 0.0f) | (t2 & !(e2 < 0.0f)));
+    const float b0 = clamp_bary(e1 * g.inv_area);
+    const float b1 = clamp_bary(e2 * g.inv_area);
+    const float b2 = clamp_bary(1.0f - b0 - b1);
+    const float z = b0 * g.z0 + b1 * g.z1 + b2 * g.z2;
+    return in ? z : 0.0f;
+}
+
+// main.rs:67-77 fragment shaders, then mod.rs:380-397 (to ARGB): the colour
+// one fragment writes to its passing samples
+__device__ __forceinline__ uint32_t shade_fragment(const PrimGeom& g, const float* __restrict__ at,
+                                                   const DrawDev* __restrict__ draws, float fx, float fy,
+                                                   uint32_t pass, const float* lut, bool& oob) {
+    float u, v, w;
+    frag_bary(g, fx, fy, pass, u, v, w);
+    const uint32_t fs = (g.flags >> 8) & 3u;
+    float cr, cg, cbl, ca;
+    if (fs == RZ_FS_COLOR) {
+        cr = at[0] * u + at[4] * v + at[8] * w;
+        cg = at[1] * u + at[5] * v + at[9] * w;
+        cbl = at[2] * u + at[6] * v + at[10] * w;
+        ca = at[3] * u + at[7] * v + at[11] * w;
+    } else if (fs == RZ_FS_TEXTURE) {
+        const DrawDev& dd = draws[g.flags >> 16];
+        const float tu = at[0] * u + at[2] * v + at[4] * w;
+        const float tv = at[1] * u + at[3] * v + at[5] * w;
+        const RGBA c = texture_sample(dd.tex, dd.tw, dd.th, tu, tv, lut, oob);
+        cr = c.r; cg = c.g; cbl = c.b; ca = c.a;
+    } else {  // Debug: Color::grayscale(depths[0]) (color.rs:65-72)
+        const float d = sample0_depth(g, fx, fy);
+        cr = d; cg = d; cbl = d; ca = 1.0f;
+    }
+    return to_argb(cr, cg, cbl, ca);
+}
+
 // Read-only inputs of the raster, passed as separate __restrict__ kernel
 // arguments: the no-alias guarantee is what lets the compiler fetch prim
 // records with scalar loads while the kernel writes its outputs.
@@ -1106,8 +1209,12 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
     const float qxl = (float)qx0 + 0.125f, qxh = (float)qx0 + 7.875f;
     const float qyl = (float)qy0 + 0.125f, qyh = (float)qy0 + 7.875f;
 
+    // per sample: stored depth and the prim that wrote it last (NO_PRIM:
+    // cleared); per sample nibble of pm: that prim's passing samples when it
+    // wrote, which fixes where it is shaded
     float zb0 = clear_depth, zb1 = clear_depth, zb2 = clear_depth, zb3 = clear_depth;
-    uint32_t cb0 = CLEAR_COLOR, cb1 = CLEAR_COLOR, cb2 = CLEAR_COLOR, cb3 = CLEAR_COLOR;
+    uint32_t pr0 = NO_PRIM, pr1 = NO_PRIM, pr2 = NO_PRIM, pr3 = NO_PRIM;
+    uint32_t pm = 0;
     bool oob = false;
     // qzmax: largest depth stored in the quadrant's on-screen samples
     float qzmax = __int_as_float(
@@ -1213,46 +1320,27 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
                         pass |= (c & (z < zbuf[k])) ? (1u << k) : 0u;
                     }
                     if (pass != 0) {
-                        // mod.rs:70-99: perspective-correct barycentrics at the pixel
-                        // centre or at the first depth-passing sample
-                        float ix = fx + 0.5f, iy = fy + 0.5f;
-                        if (pass != 0xFu) {
-                            const int k = __builtin_ctz(pass);
-                            ix = fx + sample_x(k);
-                            iy = fy + sample_y(k);
-                        }
-                        const float ef0 = edge(gm.nx0, gm.ny0, gm.ax0, gm.ay0, ix, iy);
-                        const float ef1 = edge(gm.nx1, gm.ny1, gm.ax1, gm.ay1, ix, iy);
-                        const float ef2 = edge(gm.nx2, gm.ny2, gm.ax2, gm.ay2, ix, iy);
-                        const float f_u = ef1 / gm.w0;
-                        const float f_v = ef2 / gm.w1;
-                        const float f_w = ef0 / gm.w2;
-                        const float sum = f_u + f_v + f_w;
-                        const float u = clamp_bary(f_u / sum);
-                        const float v = clamp_bary(f_v / sum);
-                        const float w = clamp_bary(1.0f - u - v);
-                        // main.rs:67-77 fragment shaders; Debug is Color::grayscale(depths[0])
-                        const uint32_t fs = (gm.flags >> 8) & 3u;
-                        const float* at = in.attrs + (size_t)prim * ATTR_STRIDE;
-                        float cr = sd[0], cg = sd[0], cbl = sd[0], ca = 1.0f;
-                        if (fs == RZ_FS_COLOR) {
-                            cr = at[0] * u + at[4] * v + at[8] * w;
-                            cg = at[1] * u + at[5] * v + at[9] * w;
-                            cbl = at[2] * u + at[6] * v + at[10] * w;
-                            ca = at[3] * u + at[7] * v + at[11] * w;
-                        } else if (fs == RZ_FS_TEXTURE) {
+                        if (gm.flags & PRIM_UV_CHECK) {
+                            // texture coordinates not proven in bounds: check this
+                            // fragment's texel reads exactly (the flag covers every
+                            // fragment, also ones overwritten later)
+                            float u, v, w;
+                            frag_bary(gm, fx, fy, pass, u, v, w);
+                            const float* at = in.attrs + (size_t)prim * ATTR_STRIDE;
                             const DrawDev& dd = in.draws[gm.flags >> 16];
-                            const float tu = at[0] * u + at[2] * v + at[4] * w;
-                            const float tv = at[1] * u + at[3] * v + at[5] * w;
-                            const RGBA c = texture_sample(dd.tex, dd.tw, dd.th, tu, tv, lut, oob);
-                            cr = c.r; cg = c.g; cbl = c.b; ca = c.a;
+                            oob |= texel_oob(dd.tw, dd.th, at[0] * u + at[2] * v + at[4] * w,
+                                             at[1] * u + at[3] * v + at[5] * w);
                         }
-                        // mod.rs:380-397
-                        const uint32_t argb = to_argb(cr, cg, cbl, ca);
-                        if (pass & 1u) { cb0 = argb; zb0 = sd[0]; }
-                        if (pass & 2u) { cb1 = argb; zb1 = sd[1]; }
-                        if (pass & 4u) { cb2 = argb; zb2 = sd[2]; }
-                        if (pass & 8u) { cb3 = argb; zb3 = sd[3]; }
+                        // mod.rs:380-397 writes the fragment's colour and depth to its
+                        // passing samples; the colour is computed after the list, once
+                        // per surviving (prim, pixel), from the same inputs
+                        if (pass & 1u) { pr0 = prim; zb0 = sd[0]; }
+                        if (pass & 2u) { pr1 = prim; zb1 = sd[1]; }
+                        if (pass & 4u) { pr2 = prim; zb2 = sd[2]; }
+                        if (pass & 8u) { pr3 = prim; zb3 = sd[3]; }
+                        const uint32_t sel = ((pass & 1u) ? 0xFu : 0u) | ((pass & 2u) ? 0xF0u : 0u) |
+                                             ((pass & 4u) ? 0xF00u : 0u) | ((pass & 8u) ? 0xF000u : 0u);
+                        pm = (pm & ~sel) | ((pass * 0x1111u) & sel);
                         lane_zbmax = fmaxf(fmaxf(zb0, zb1), fmaxf(zb2, zb3));
                         wrote = true;
                     }
@@ -1263,6 +1351,27 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
                 qzmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(wave_max(lane_zbmax))));
             }
             if (m == 0) break;
+        }
+    }
+    // deferred shading: each distinct surviving prim of the pixel, once
+    uint32_t cb0 = CLEAR_COLOR, cb1 = CLEAR_COLOR, cb2 = CLEAR_COLOR, cb3 = CLEAR_COLOR;
+    uint32_t todo = (pr0 != NO_PRIM ? 1u : 0u) | (pr1 != NO_PRIM ? 2u : 0u) | (pr2 != NO_PRIM ? 4u : 0u) |
+                    (pr3 != NO_PRIM ? 8u : 0u);
+    while (__ballot(todo != 0) != 0) {
+        if (todo != 0) {
+            const uint32_t k = __builtin_ctz(todo);
+            const uint32_t P = k == 0 ? pr0 : (k == 1 ? pr1 : (k == 2 ? pr2 : pr3));
+            const uint32_t pass = (pm >> (4 * k)) & 0xFu;
+            const PrimGeom gp = geom[P];
+            const uint32_t argb =
+                shade_fragment(gp, in.attrs + (size_t)P * ATTR_STRIDE, in.draws, fx, fy, pass, lut, oob);
+            const uint32_t match = (pr0 == P ? 1u : 0u) | (pr1 == P ? 2u : 0u) | (pr2 == P ? 4u : 0u) |
+                                   (pr3 == P ? 8u : 0u);
+            if (match & 1u) cb0 = argb;
+            if (match & 2u) cb1 = argb;
+            if (match & 4u) cb2 = argb;
+            if (match & 8u) cb3 = argb;
+            todo &= ~match;
         }
     }
     if (__ballot(oob) && lane == 0) atomicOr(&b.runs[s].flags, RZ_FLAG_TEXEL_OOB);

@@ -219,3 +219,30 @@ def test_terrain_full_size():
     """The 4M-triangle 3840x2160 terrain with its 4096^2 texture, full frame."""
     sc = scenes.terrain_scene()
     check(sc)
+
+
+def _quad(z, half, uv_max, tex):
+    pos = np.array([[-half, -half, z], [half, -half, z], [half, half, z], [-half, half, z]], np.float32)
+    uv = np.array([[0, 0], [uv_max, 0], [uv_max, uv_max], [0, uv_max]], np.float32)
+    idx = np.array([0, 2, 1, 0, 3, 2], np.uint32)  # front-facing for this camera
+    col = np.ones((4, 4), np.float32)
+    return scenes.Scene(f"quad{z}", 160, 120, pos, col, uv, idx, hm.identity(), hm.Camera().get_view_matrix(),
+                        hm.project(1.0, 200.0, F(120 / 160), hm.FRAC_PI_2), scenes.FS_TEXTURE, tex)
+
+
+def test_oob_flag_counts_overwritten_fragments():
+    """The reference shades every depth-passing fragment in order, so a
+    texel read past the texture by a fragment that a later one overwrites
+    still happens (and would panic there): the flag must be raised although
+    no such texel reaches the image. A fragment that fails the depth test is
+    never shaded and must not raise it."""
+    tex = meshes.checkerboard(64, 8)
+    far_oob = _quad(3.0, 1.0, 1.5, tex)   # uv up to 1.5: reads past the texture
+    near_ok = _quad(2.0, 1.6, 0.5, tex)   # in front, covers it entirely
+    far_alone = run_oracle(far_oob)
+    assert (far_alone["img"] != 0xFF191919).sum() > 100 and far_alone["flags"] & 1
+    g, c = check([far_oob, near_ok])
+    assert c["flags"] & 1 and g["flags"] & 1
+    g2, c2 = check([near_ok, far_oob])    # the far quad now fails LESS everywhere
+    assert not (c2["flags"] & 1) and not (g2["flags"] & 1)
+    assert np.array_equal(g["img"], g2["img"])
