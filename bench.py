@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-scene leg (batch workload)")
     ap.add_argument("--no-tune", action="store_true", help="use the default batch tuning")
+    ap.add_argument("--tuning", default=None,
+                    help="fixed batch tuning bin_target,raster_per_cu,use_graph (e.g. 64,0,1); skips the autotuner")
+    ap.add_argument("--no-extra", action="store_true", help="batch workload only: skip the soup/terrain/serve legs")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson request rate (1/s)")
@@ -241,6 +244,10 @@ def bench_batch(a, steps, warmup, with_cpu, with_plain, tune, scs, seed, t_gen):
     def once():
         batch.run(ctx, items, table)
 
+    if a.tuning:
+        bt, rp, g = (int(v) for v in a.tuning.split(","))
+        ctx.batch_tuning(bt, rp, bool(g))
+        tune = False
     once()  # first batch: buffers grow to fit
     first_stats = ctx.frame_stats()
     log(f"uploaded in {t_up:.1f} s; first batch done")
@@ -287,7 +294,7 @@ def bench_batch(a, steps, warmup, with_cpu, with_plain, tune, scs, seed, t_gen):
         "batch_stats": stats,
         "first_batch_replays": (first_stats["replays"], first_stats["overflow_bits"]),
         "setup_s": {"generate": t_gen, "upload": t_up},
-        "roofline": roofline(alg_bytes, prof["raster"] / prof["frames"], load_traffic("traffic_raster_batch256.json")),
+        "roofline": roofline(alg_bytes, prof["raster"] / prof["frames"], load_traffic("traffic_raster_batch.json")),
         "samples_per_batch": samples,
         "tris_per_batch": tris,
     }
@@ -453,14 +460,18 @@ def main():
     scs, bseed, t_gen = batch_scenes_of(a)
     log(f"generated {len(scs)} scenes in {t_gen:.1f} s")
     b = bench_batch(a, a.steps, a.warmup, with_cpu, not a.no_plain, not a.no_tune, scs, bseed, t_gen)
-    soup_sc, desc = soup_scene_of(a)
-    soup = dict(workload=desc, **bench_frame(soup_sc, max(5, a.steps // 2), 2, None, "traffic_raster_soup1m.json"))
-    log(f"soup: {soup['ms_per_frame']:.2f} ms per frame")
-    terr_sc, desc = terrain_scene_of(a)
-    terrain = dict(workload=desc, **bench_frame(terr_sc, max(5, a.steps // 2), 2, None,
-                                                "traffic_raster_terrain.json"))
-    log(f"terrain: {terrain['ms_per_frame']:.2f} ms per frame")
-    serve = None
+    soup = terrain = serve = None
+    if a.no_extra:
+        a.no_serve = True
+    else:
+        soup_sc, desc = soup_scene_of(a)
+        soup = dict(workload=desc, **bench_frame(soup_sc, max(5, a.steps // 2), 2, None,
+                                                 "traffic_raster_soup1m.json"))
+        log(f"soup: {soup['ms_per_frame']:.2f} ms per frame")
+        terr_sc, desc = terrain_scene_of(a)
+        terrain = dict(workload=desc, **bench_frame(terr_sc, max(5, a.steps // 2), 2, None,
+                                                    "traffic_raster_terrain.json"))
+        log(f"terrain: {terrain['ms_per_frame']:.2f} ms per frame")
     if not a.no_serve:
         from rusterizer_amd import scenes as S
 
