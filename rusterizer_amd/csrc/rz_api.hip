@@ -302,8 +302,10 @@ int build_job(rz_ctx* c, JobPlan& pl) {
             DrawDev& h = dd[d];
             memset(&h, 0, sizeof(h));
             h.pos = pd.mesh->pos;
-            h.col = pd.mesh->col;
-            h.uv = pd.mesh->uv;
+            // only the stream the fragment shader reads is fetched (the other
+            // attribute channels never reach an output)
+            h.col = pd.fs == RZ_FS_COLOR ? pd.mesh->col : nullptr;
+            h.uv = pd.fs == RZ_FS_TEXTURE ? pd.mesh->uv : nullptr;
             h.idx = pd.mesh->idx;
             h.ntri = pd.mesh->ntri;
             h.fs = pd.fs;
@@ -436,7 +438,7 @@ JobGrid job_grid(const rz_ctx* c, const JobPlan& pl, bool for_graph) {
     const uint32_t nt = for_graph ? (uint32_t)c->cls.cap : pl.ntri;
     g.draws = (nd + 255) / 256;
     g.tris = std::min<uint32_t>((nt + 255) / 256, 8192);
-    g.clip = (uint32_t)std::min<size_t>(c->clip_tri.cap, 8192);
+    g.clip = (uint32_t)std::min<size_t>(c->clip_tri.cap, 131072);  // slots; k_clip takes CLIP_GROUPS per wave
     const uint64_t chunks = c->geom.cap / CHUNK + (for_graph ? c->scap : pl.nscenes);
     g.bins = (uint32_t)std::min<uint64_t>((chunks + 3) / 4, 4096);
     const uint64_t fch = c->list.cap / CHUNK + c->bin_rng.cap;

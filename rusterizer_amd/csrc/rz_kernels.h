@@ -19,7 +19,7 @@ namespace rzk {
 constexpr uint32_t CLEAR_COLOR = 0xFF191919u;  // rasterizer/buffers.rs:5
 constexpr float FLT_MAX_F = 3.40282347e+38f;   // f32::MAX
 constexpr uint32_t MAXV = 16;         // vertices kept per clipped polygon
-constexpr uint32_t CLIP_LDS_V = 66;   // polygon vertices a clip wave can stage
+constexpr uint32_t CLIP_GROUPS = 8;   // clipped triangles per k_clip wave (8 lanes each)
 constexpr uint32_t CHUNK = 256;       // prims per binning chunk (one wave)
 constexpr uint32_t MAX_BINS = 256;    // coarse bins per scene (4 per lane)
 constexpr uint32_t TRI_ALIGN = 256;   // draws start at multiples of this in the triangle index space
@@ -122,7 +122,7 @@ struct PrimGeom {
     float inv_area;                      // 1 / (2 * area)
     float z0, z1, z2;                    // screen-space depth
     float w0, w1, w2;                    // clip-space w
-    uint32_t flags;                      // tie-rule bit per edge (0..2) | PRIM_UV_CHECK | fs << 8 | draw << 16
+    uint32_t flags;                      // tie-rule bit per edge (0..2) | PRIM_* bits | fs << 8 | draw << 16
     uint32_t bbx, bby;                   // clamped pixel bbox: lo | hi << 16
     float zlo;                           // lower bound of every interpolated depth
     float pz0, pzx, pzy;                 // depth lower-bound plane: pz0 + pzx*(x-bx0) + pzy*(y-by0)

@@ -282,10 +282,11 @@ __global__ __launch_bounds__(256) void k_classify(JobBufs b) {
 // places them, so the output keeps the reference's vertex order exactly.
 // The final polygon (<= MAXV vertices) goes to its clip slot; the fan
 // (clipping.rs:183-190) is formed in k_setup.
-// k_clip runs 4 groups of 16 lanes per wave. A polygon that would need more
-// than 16 lanes at some plane (never for a triangle with exact arithmetic --
-// each plane adds at most one vertex -- but rounding could) is handed to
-// k_clip_redo, which clips it again from the start with all 64 lanes.
+// k_clip runs CLIP_GROUPS groups of 64 / CLIP_GROUPS lanes per wave. A
+// polygon that would need more lanes than its group has at some plane (a
+// triangle clipped by all six planes can reach 9 vertices; rounding can add
+// more) is handed to k_clip_redo, which clips it again from the start with
+// all 64 lanes.
 // ---------------------------------------------------------------------------
 struct ClipV {
     float x, y, z, w;
@@ -407,7 +408,7 @@ __device__ __forceinline__ void clip_polygon(const JobBufs& b, uint32_t slot, bo
     }
 }
 
-// 4 clipped triangles per wave (with clip_wide set, every one goes to k_clip_redo)
+// CLIP_GROUPS clipped triangles per wave (with clip_wide set, every one goes to k_clip_redo)
 __global__ __launch_bounds__(64) void k_clip(JobBufs b) {
     __shared__ float sh[64 * 10];
     const uint32_t nslots = min(uniform(b.cnt->nclip), b.clip_cap);
@@ -416,9 +417,10 @@ __global__ __launch_bounds__(64) void k_clip(JobBufs b) {
             b.clip_redo[atomicAdd(&b.cnt->nredo, 1u)] = i;
         return;
     }
-    for (uint32_t base = blockIdx.x * 4u; base < nslots; base += gridDim.x * 4u) {
-        const uint32_t slot = base + (threadIdx.x >> 4);
-        clip_polygon<16>(b, slot, slot < nslots, sh);
+    constexpr uint32_t G = 64 / CLIP_GROUPS;
+    for (uint32_t base = blockIdx.x * CLIP_GROUPS; base < nslots; base += gridDim.x * CLIP_GROUPS) {
+        const uint32_t slot = base + threadIdx.x / G;
+        clip_polygon<G>(b, slot, slot < nslots, sh);
     }
 }
 
@@ -650,7 +652,7 @@ __device__ __forceinline__ bool uv_in_bounds(const ClipV* tv, const float* w) {
 }
 
 __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t H, uint32_t draw, uint32_t fs,
-                                           uint32_t cb, PrimGeom* __restrict__ g, float* __restrict__ attr,
+                                           uint32_t cb, PrimGeom* g, float* __restrict__ attr,
                                            uint4* __restrict__ pbin, uint32_t& pairs) {
     const float zmin = 0.0f, zmax = 1.0f;
     float sx[3], sy[3], sz[3], w[3];
@@ -716,18 +718,30 @@ __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t
     }
 }
 
+// The prims of a workgroup's 256 triangles are one contiguous range; when it
+// fits SETUP_STAGE records they are assembled in LDS and written with
+// contiguous 16-byte stores (whole cache lines) instead of one scattered
+// 112-byte record per thread.
+constexpr uint32_t SETUP_STAGE = 352;
+
 __global__ __launch_bounds__(256) void k_setup(JobBufs b) {
+    __shared__ PrimGeom stage[SETUP_STAGE];
     const uint32_t NT = b.head->ntri, nd = b.head->ndraws;
     const uint32_t prim_cap = b.prim_cap;
     for (uint32_t blk = blockIdx.x; blk * 256u < NT; blk += gridDim.x) {
-        const uint32_t t = blk * 256u + threadIdx.x;
-        const uint32_t d = uniform(find_draw(b.draws, nd, blk * 256u));
+        const uint32_t t0 = blk * 256u;
+        const uint32_t t = t0 + threadIdx.x;
+        const uint32_t d = uniform(find_draw(b.draws, nd, t0));
         const DrawDev& dd = b.draws[d];
         const uint32_t scene = dd.scene;
         const SceneCfg& sc = b.scenes[scene];
         const SceneRun& run = b.runs[scene];
         const uint32_t W = sc.W, H = sc.H, cb = sc.cb, fs = dd.fs;
         const uint32_t lt = t - dd.tri_base;
+        // the block's prim range (its 256 triangles belong to draw d: draws are 256-aligned)
+        const uint32_t blk_first = uniform(run.prim_base + (b.suboff[t0] - run.prim_off));
+        const uint32_t blk_end = uniform(run.prim_base + (b.suboff[t0 + 255u] + b.nsub[t0 + 255u] - run.prim_off));
+        const bool staged = blk_end - blk_first <= SETUP_STAGE && blk_end <= prim_cap;
         uint32_t pairs = 0;
         bool ovf = false;
         if (lt < dd.ntri) {
@@ -742,8 +756,8 @@ __global__ __launch_bounds__(256) void k_setup(JobBufs b) {
                     tv[k].x = p.x; tv[k].y = p.y; tv[k].z = p.z; tv[k].w = p.w;
                 }
                 if (base < prim_cap)
-                    setup_prim(tv, W, H, d, fs, cb, b.geom + base, b.attrs + (size_t)base * ATTR_STRIDE,
-                               b.pbin + base, pairs);
+                    setup_prim(tv, W, H, d, fs, cb, staged ? stage + (base - blk_first) : b.geom + base,
+                               b.attrs + (size_t)base * ATTR_STRIDE, b.pbin + base, pairs);
                 else
                     ovf = true;
             } else if (c >= 2) {
@@ -761,7 +775,8 @@ __global__ __launch_bounds__(256) void k_setup(JobBufs b) {
                             for (int j = 0; j < 6; ++j) tv[q].a[j] = p[4 + j];
                         }
                         if (base + k < prim_cap)
-                            setup_prim(tv, W, H, d, fs, cb, b.geom + base + k,
+                            setup_prim(tv, W, H, d, fs, cb,
+                                       staged ? stage + (base + k - blk_first) : b.geom + base + k,
                                        b.attrs + (size_t)(base + k) * ATTR_STRIDE, b.pbin + base + k, pairs);
                         else
                             ovf = true;
@@ -769,6 +784,14 @@ __global__ __launch_bounds__(256) void k_setup(JobBufs b) {
                 }
             }
         }
+        __syncthreads();
+        if (staged) {
+            const uint4* src = reinterpret_cast<const uint4*>(stage);
+            uint4* dst = reinterpret_cast<uint4*>(b.geom + blk_first);
+            const uint32_t n16 = (blk_end - blk_first) * (uint32_t)(sizeof(PrimGeom) / sizeof(uint4));
+            for (uint32_t i = threadIdx.x; i < n16; i += 256u) dst[i] = src[i];
+        }
+        __syncthreads();  // the stage is reused by the next block
         if (__ballot(ovf) && lane_id() == 0) atomicOr(&b.cnt->overflow, OVF_PRIMS);
         // wave sum of coarse-bin entries
         uint32_t s = pairs;
@@ -1480,7 +1503,7 @@ void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_s
     k_frame_reset<<<scene_blocks, 256, 0, s>>>(b);
     k_mvp<<<at_least1(g.draws), 256, 0, s>>>(b);
     k_classify<<<at_least1(g.tris), 256, 0, s>>>(b);
-    k_clip<<<at_least1((g.clip + 3) / 4), 64, 0, s>>>(b);
+    k_clip<<<at_least1((g.clip + CLIP_GROUPS - 1) / CLIP_GROUPS), 64, 0, s>>>(b);
     k_clip_redo<<<64, 64, 0, s>>>(b);
     launch_scan<uint32_t>(s, b.nsub, b.suboff, b.tri_cap, &b.head->ntri, b.parts, &b.cnt->nprims_raw);
     k_scene_prep<<<1, 1024, 0, s>>>(b);

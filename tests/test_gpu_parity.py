@@ -246,3 +246,46 @@ def test_oob_flag_counts_overwritten_fragments():
     g2, c2 = check([near_ok, far_oob])    # the far quad now fails LESS everywhere
     assert not (c2["flags"] & 1) and not (g2["flags"] & 1)
     assert np.array_equal(g["img"], g2["img"])
+
+
+def _lattice_scene(W=64, flip=False, step=4.0, ox=0.625, oy=0.125):
+    """Triangles tiling a region with every edge (vertical, horizontal and
+    diagonal) running exactly through sample positions: identity matrices,
+    so screen = (W (x+1)/2, W (1-y)/2) exactly."""
+    n = int((W - 2) // step)
+    pts, idx, cols = [], [], []
+    rng = np.random.default_rng(5)
+    for j in range(n):
+        for i in range(n):
+            x0, y0 = ox + step * i, oy + step * j
+            quad = [(x0, y0), (x0 + step, y0), (x0 + step, y0 + step), (x0, y0 + step)]
+            for tri in ((0, 1, 2), (0, 2, 3)):
+                t = [quad[k] for k in tri]
+                if flip:
+                    t = [t[0], t[2], t[1]]
+                base = len(pts)
+                for sx, sy in t:
+                    pts.append([sx / (W / 2) - 1.0, 1.0 - sy / (W / 2), 0.25 + 0.5 * rng.random()])
+                c = rng.random(4).astype(np.float32)
+                c[3] = 1.0
+                cols += [c, c, c]
+                idx += [base, base + 1, base + 2]
+    pos = np.array(pts, np.float32)
+    sc = scenes.Scene("lattice", W, W, pos, np.array(cols, np.float32), None, np.array(idx, np.uint32),
+                      hm.identity(), hm.identity(), hm.identity(), scenes.FS_COLOR)
+    return sc
+
+
+def test_tie_rule_on_shared_edges():
+    """Edges through sample points: every such sample goes to exactly one of
+    the two triangles sharing the edge, by the tie rule (mod.rs:148-170)."""
+    for flip in (False, True):
+        sc = _lattice_scene(flip=flip)
+        c = run_oracle(sc)
+        if (c["img"] != 0xFF191919).mean() < 0.5:
+            continue  # this winding faces away
+        g, _ = check(sc)
+        covered = (c["col"] != 0xFF191919)
+        assert covered.mean() > 0.5
+        return
+    raise AssertionError("no winding of the lattice was front-facing")
