@@ -29,8 +29,9 @@ constexpr uint32_t SCAN_TILE = 256 * SCAN_ITEMS;
 constexpr uint32_t NO_TILE = 0xFFFFFFFFu;
 constexpr uint32_t NO_PRIM = 0xFFFFFFFFu;
 
-// PrimGeom.flags bit: texture coordinates not proven in bounds (see uv_in_bounds)
-constexpr uint32_t PRIM_UV_CHECK = 8;
+// PrimGeom.flags bits: texture coordinates not proven in bounds (see
+// uv_in_bounds); barycentric arguments finite at covered samples (see finite_bary)
+constexpr uint32_t PRIM_UV_CHECK = 8, PRIM_FINITE_BARY = 16;
 
 // overflow bits: the job must be replayed with larger buffers
 constexpr uint32_t OVF_PRIMS = 1, OVF_CLIP = 2, OVF_PAIRS = 4, OVF_POLY = 8, OVF_FINE = 16, OVF_MATRIX = 32;

@@ -651,6 +651,17 @@ __device__ __forceinline__ bool uv_in_bounds(const ClipV* tv, const float* w) {
     return ok;
 }
 
+// True when the edge values at every sample position are finite and e *
+// inv_area is finite at every covered sample: 0 < inv_area < inf and the
+// points and normals finite and below 2^24 (a covered sample's edge value is
+// at most about 2A).
+__device__ __forceinline__ bool finite_bary(const PrimGeom& o) {
+    bool ok = (o.inv_area > 0.0f) & (o.inv_area <= FLT_MAX_F);
+    const float v[12] = {o.ax0, o.ay0, o.ax1, o.ay1, o.ax2, o.ay2, o.nx0, o.ny0, o.nx1, o.ny1, o.nx2, o.ny2};
+    for (int k = 0; k < 12; ++k) ok &= fabsf(v[k]) <= 0x1p24f;
+    return ok;
+}
+
 __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t H, uint32_t draw, uint32_t fs,
                                            uint32_t cb, PrimGeom* g, float* __restrict__ attr,
                                            uint4* __restrict__ pbin, uint32_t& pairs) {
@@ -693,6 +704,7 @@ __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t
     o.flags = (tie_rule(o.nx0, o.ny0) ? 1u : 0u) | (tie_rule(o.nx1, o.ny1) ? 2u : 0u) |
               (tie_rule(o.nx2, o.ny2) ? 4u : 0u) | (fs << 8) | (draw << 16);
     if (fs == RZ_FS_TEXTURE && !uv_in_bounds(tv, w)) o.flags |= PRIM_UV_CHECK;
+    if (finite_bary(o)) o.flags |= PRIM_FINITE_BARY;
     // Depth of a covered sample is ((b0*z0 + b1*z1) + b2*z2) with clamped
     // b_i in [0, 1] and 1 - 2^-23 <= b0+b1+b2 <= 2; three roundings later it
     // is >= zlo below, so a sample can pass LESS only where zbuf > zlo.
@@ -1332,15 +1344,31 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
                     float sd[4];
                     uint32_t pass = 0;
                     const float zbuf[4] = {zb0, zb1, zb2, zb3};
+                    if (gm.flags & PRIM_FINITE_BARY) {
+                        // covered samples of this prim have finite barycentric
+                        // arguments, where med3(x, 0, 1) is f32::clamp (a zero's
+                        // sign may differ, which no depth or colour depends on)
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        const float b0 = clamp_bary(e1s[k] * gm.inv_area);
-                        const float b1 = clamp_bary(e2s[k] * gm.inv_area);
-                        const float b2 = clamp_bary(1.0f - b0 - b1);
-                        const float z = b0 * gm.z0 + b1 * gm.z1 + b2 * gm.z2;
-                        const bool c = (cov >> k) & 1u;
-                        sd[k] = c ? z : 0.0f;
-                        pass |= (c & (z < zbuf[k])) ? (1u << k) : 0u;
+                        for (int k = 0; k < 4; ++k) {
+                            const float b0 = __builtin_amdgcn_fmed3f(e1s[k] * gm.inv_area, 0.0f, 1.0f);
+                            const float b1 = __builtin_amdgcn_fmed3f(e2s[k] * gm.inv_area, 0.0f, 1.0f);
+                            const float b2 = __builtin_amdgcn_fmed3f(1.0f - b0 - b1, 0.0f, 1.0f);
+                            const float z = b0 * gm.z0 + b1 * gm.z1 + b2 * gm.z2;
+                            const bool c = (cov >> k) & 1u;
+                            sd[k] = c ? z : 0.0f;
+                            pass |= (c & (z < zbuf[k])) ? (1u << k) : 0u;
+                        }
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const float b0 = clamp_bary(e1s[k] * gm.inv_area);
+                            const float b1 = clamp_bary(e2s[k] * gm.inv_area);
+                            const float b2 = clamp_bary(1.0f - b0 - b1);
+                            const float z = b0 * gm.z0 + b1 * gm.z1 + b2 * gm.z2;
+                            const bool c = (cov >> k) & 1u;
+                            sd[k] = c ? z : 0.0f;
+                            pass |= (c & (z < zbuf[k])) ? (1u << k) : 0u;
+                        }
                     }
                     if (pass != 0) {
                         if (gm.flags & PRIM_UV_CHECK) {
