@@ -440,9 +440,9 @@ JobGrid job_grid(const rz_ctx* c, const JobPlan& pl, bool for_graph) {
     g.tris = std::min<uint32_t>((nt + 255) / 256, 8192);
     g.clip = (uint32_t)std::min<size_t>(c->clip_tri.cap, 131072);  // slots; k_clip takes CLIP_GROUPS per wave
     const uint64_t chunks = c->geom.cap / CHUNK + (for_graph ? c->scap : pl.nscenes);
-    g.bins = (uint32_t)std::min<uint64_t>((chunks + 3) / 4, 4096);
+    g.bins = (uint32_t)std::min<uint64_t>(chunks, 8192);  // one workgroup per chunk
     const uint64_t fch = c->list.cap / CHUNK + c->bin_rng.cap;
-    g.fine = (uint32_t)std::min<uint64_t>((fch + 3) / 4, 4096);
+    g.fine = (uint32_t)std::min<uint64_t>(fch, 8192);
     g.raster = c->raster_per_cu ? c->raster_per_cu * (uint32_t)c->n_cu : 0;
     g.ntiles = for_graph ? std::max(c->tiles_cap, pl.ntiles) : pl.ntiles;
     g.scenes = for_graph ? c->scap : pl.nscenes;

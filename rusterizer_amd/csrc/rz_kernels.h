@@ -21,7 +21,7 @@ constexpr float FLT_MAX_F = 3.40282347e+38f;   // f32::MAX
 constexpr uint32_t MAXV = 16;         // vertices kept per clipped polygon
 constexpr uint32_t CLIP_GROUPS = 8;   // clipped triangles per k_clip wave (8 lanes each)
 constexpr uint32_t CHUNK = 256;       // prims per binning chunk (one wave)
-constexpr uint32_t MAX_BINS = 256;    // coarse bins per scene (4 per lane)
+constexpr uint32_t MAX_BINS = 256;    // coarse bins per scene (one binning thread each)
 constexpr uint32_t TRI_ALIGN = 256;   // draws start at multiples of this in the triangle index space
 constexpr uint32_t ATTR_STRIDE = 12;  // floats of attributes per prim
 constexpr int SCAN_ITEMS = 8;

@@ -813,84 +813,78 @@ __global__ __launch_bounds__(256) void k_setup(JobBufs b) {
 }
 
 // ---------------------------------------------------------------------------
-// k_bin: ordered coarse binning, one wave per chunk of CHUNK prims of one
-// scene. With r = the scene's SceneRun and lc the chunk's index in the scene:
+// k_bin: ordered coarse binning, one workgroup per chunk of CHUNK prims of one
+// scene (thread t holds slot t). With r = the scene's SceneRun and lc the
+// chunk's index in the scene:
 // COUNT:   counts[r.mat_base + bin * r.nch + lc] = prims of the chunk overlapping bin
 // SCATTER: writes {prim, bbox, zlo} to list position offsets[same index]
 //          + rank, rank = number of earlier prims of the chunk in that bin.
-// Both loop bins in the same order and rank lanes with ballots, so each bin's
-// list holds its prims in submission order.
+// The chunk's bin membership is a bit matrix in LDS, bits[w][bin] bit k =
+// slot 32w + k overlaps bin: each thread sets the bits of its own bins, so the
+// work follows the bins a prim covers, not the bins of the scene. A bin's
+// count is the popcount of its column; a slot's rank in a bin is the popcount
+// of the column's bits below it, which keeps each bin's list in submission
+// order.
 // ---------------------------------------------------------------------------
+constexpr uint32_t BIN_WORDS = CHUNK / 32;
+static_assert(MAX_BINS == 256 && CHUNK == 256, "one thread per bin and per chunk slot");
+
 template <bool SCATTER>
 __global__ __launch_bounds__(256) void k_bin(JobBufs b) {
+    __shared__ uint32_t bits[BIN_WORDS * MAX_BINS];
+    __shared__ uint32_t wbase[SCATTER ? BIN_WORDS * MAX_BINS : 1];
     if (uniform(b.cnt->overflow) != 0) return;
     const uint32_t NC = uniform(b.cnt->nchunks), ns = b.head->nscenes;
-    const uint32_t lane = lane_id();
-    const uint32_t wave = uniform((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t t = threadIdx.x, w = t >> 5, below = (1u << (t & 31)) - 1u;
     bool ovf = false;
-    for (uint32_t chunk = wave; chunk < NC; chunk += nwaves) {
+    for (uint32_t chunk = blockIdx.x; chunk < NC; chunk += gridDim.x) {
         const uint32_t s = uniform(find_scene_chunk(b.runs, ns, chunk));
         const SceneCfg& sc = b.scenes[s];
         const SceneRun& r = b.runs[s];
-        const uint32_t cb = sc.cb, nbx = sc.nbx, nby = sc.nby, nb = nbx * nby;
-        const uint32_t lc = chunk - r.chunk_base, nch = r.nch;
-        const uint32_t pend = r.prim_base + r.nprims;
-        const size_t mb = (size_t)r.mat_base + lc;
-        // per-bin running rank, held by lane (bin & 63) in register (bin >> 6)
-        uint32_t run0 = 0, run1 = 0, run2 = 0, run3 = 0;
-        if (SCATTER) {
-            if (lane < nb) run0 = b.bin_offsets[mb + (size_t)lane * nch];
-            if (lane + 64 < nb) run1 = b.bin_offsets[mb + (size_t)(lane + 64) * nch];
-            if (lane + 128 < nb) run2 = b.bin_offsets[mb + (size_t)(lane + 128) * nch];
-            if (lane + 192 < nb) run3 = b.bin_offsets[mb + (size_t)(lane + 192) * nch];
-        }
-        for (uint32_t sub = 0; sub < CHUNK / 64; ++sub) {
-            const uint32_t p = chunk * CHUNK + sub * 64 + lane;
-            uint4 pb = make_uint4(0u, 0u, 0u, 0u);
-            if (p < pend) pb = b.pbin[p];
-            const uint32_t x0 = pb.y & 0xFFFFu, x1 = pb.y >> 16, y0 = pb.z & 0xFFFFu, y1 = pb.z >> 16;
-            const bool ne = (x0 < x1) && (y0 < y1);
-            const uint32_t cx0 = x0 / cb, cx1 = ne ? (x1 - 1) / cb : 0u;
-            const uint32_t cy0 = y0 / cb, cy1 = ne ? (y1 - 1) / cb : 0u;
-            for (uint32_t by = 0; by < nby; ++by) {
-                const bool rowhit = ne && by >= cy0 && by <= cy1;
-                if (__ballot(rowhit) == 0) continue;
-                for (uint32_t bx = 0; bx < nbx; ++bx) {
-                    const bool hit = rowhit && bx >= cx0 && bx <= cx1;
-                    const uint64_t m = __ballot(hit);
-                    if (m == 0) continue;
-                    const uint32_t bi = by * nbx + bx;
-                    const uint32_t j = bi >> 6, bl = bi & 63;
-                    const uint32_t c = (uint32_t)__popcll(m);
-                    if (SCATTER) {
-                        const uint32_t rv = j == 0 ? run0 : (j == 1 ? run1 : (j == 2 ? run2 : run3));
-                        const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)rv, (int)bl);
-                        if (hit) {
-                            const uint32_t pos = r0 + mbcnt(m);
-                            if (pos < b.pair_cap)
-                                b.list[pos] = make_uint4(p, pb.y, pb.z, pb.w);
-                            else
-                                ovf = true;
-                        }
-                    }
-                    if (lane == bl) {
-                        if (j == 0) run0 += c;
-                        else if (j == 1) run1 += c;
-                        else if (j == 2) run2 += c;
-                        else run3 += c;
-                    }
-                }
+        const uint32_t cb = sc.cb, nbx = sc.nbx, nb = nbx * sc.nby;
+        const uint32_t nch = r.nch;
+        const size_t mb = (size_t)r.mat_base + (chunk - r.chunk_base) + (size_t)t * nch;
+        const uint32_t p = chunk * CHUNK + t;
+        uint4 pb = make_uint4(0u, 0u, 0u, 0u);
+        if (p < r.prim_base + r.nprims) pb = b.pbin[p];
+        uint32_t run = 0;
+        if (SCATTER && t < nb) run = b.bin_offsets[mb];
+        if (t < nb)
+            for (uint32_t k = 0; k < BIN_WORDS; ++k) bits[k * MAX_BINS + t] = 0u;
+        const uint32_t x0 = pb.y & 0xFFFFu, x1 = pb.y >> 16, y0 = pb.z & 0xFFFFu, y1 = pb.z >> 16;
+        const bool ne = (x0 < x1) && (y0 < y1);
+        const uint32_t cx0 = x0 / cb, cx1 = ne ? (x1 - 1) / cb : 0u;
+        const uint32_t cy0 = y0 / cb, cy1 = ne ? (y1 - 1) / cb : 0u;
+        uint32_t* row = bits + w * MAX_BINS;
+        __syncthreads();
+        if (ne)
+            for (uint32_t by = cy0; by <= cy1; ++by)
+                for (uint32_t bx = cx0; bx <= cx1; ++bx) atomicOr(&row[by * nbx + bx], 1u << (t & 31));
+        __syncthreads();
+        if (t < nb) {
+            for (uint32_t k = 0; k < BIN_WORDS; ++k) {
+                const uint32_t c = (uint32_t)__popc(bits[k * MAX_BINS + t]);
+                if (SCATTER) wbase[k * MAX_BINS + t] = run;
+                run += c;
             }
+            if (!SCATTER) b.bin_counts[mb] = run;
         }
-        if (!SCATTER) {
-            if (lane < nb) b.bin_counts[mb + (size_t)lane * nch] = run0;
-            if (lane + 64 < nb) b.bin_counts[mb + (size_t)(lane + 64) * nch] = run1;
-            if (lane + 128 < nb) b.bin_counts[mb + (size_t)(lane + 128) * nch] = run2;
-            if (lane + 192 < nb) b.bin_counts[mb + (size_t)(lane + 192) * nch] = run3;
+        if (SCATTER) {
+            __syncthreads();
+            const uint4 en = make_uint4(p, pb.y, pb.z, pb.w);
+            const uint32_t* wb = wbase + w * MAX_BINS;
+            if (ne)
+                for (uint32_t by = cy0; by <= cy1; ++by)
+                    for (uint32_t bx = cx0; bx <= cx1; ++bx) {
+                        const uint32_t bi = by * nbx + bx;
+                        const uint32_t pos = wb[bi] + (uint32_t)__popc(row[bi] & below);
+                        if (pos < b.pair_cap) b.list[pos] = en;
+                        else ovf = true;
+                    }
         }
+        __syncthreads();  // bits and wbase are reused by the next chunk
     }
-    if (SCATTER && __ballot(ovf) && lane == 0) atomicOr(&b.cnt->overflow, OVF_PAIRS);
+    if (SCATTER && __ballot(ovf) && lane_id() == 0) atomicOr(&b.cnt->overflow, OVF_PAIRS);
 }
 
 // ---------------------------------------------------------------------------
@@ -932,6 +926,11 @@ __global__ __launch_bounds__(256) void k_fine_prep(JobBufs b) {
 
 template <bool SCATTER>
 __global__ __launch_bounds__(256) void k_fine_bin(JobBufs b) {
+    // one workgroup per fine-binning chunk of CHUNK list entries of one coarse
+    // bin, thread t holding entry t; the same LDS bit matrix as k_bin, with
+    // the bin's local tiles in place of the scene's bins
+    __shared__ uint32_t bits[BIN_WORDS * MAX_BINS];
+    __shared__ uint32_t wbase[SCATTER ? BIN_WORDS * MAX_BINS : 1];
     if (uniform(b.cnt->overflow) != 0) return;
     const uint32_t NB = b.head->nbins, ns = b.head->nscenes;
     const uint32_t NCH = uniform(b.cnt->fine_chunks);
@@ -939,77 +938,64 @@ __global__ __launch_bounds__(256) void k_fine_bin(JobBufs b) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&b.cnt->overflow, OVF_FINE);
         return;
     }
-    const uint32_t lane = lane_id();
-    const uint32_t wave = uniform((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t t = threadIdx.x, w = t >> 5, below = (1u << (t & 31)) - 1u;
     bool ovf = false;
-    for (uint32_t gc = wave; gc < NCH; gc += nwaves) {
+    for (uint32_t gc = blockIdx.x; gc < NCH; gc += gridDim.x) {
         const uint32_t g = uniform(find_bin_fchunk(b.fpre, NB, gc));
         const uint32_t s = uniform(find_scene_bin(b.scenes, ns, g));
         const SceneCfg& sc = b.scenes[s];
         const uint32_t cb = sc.cb, TS = cb / 16, T = TS * TS;
-        if (T > 256) continue;  // the host never configures more tiles per coarse bin
+        if (T > MAX_BINS) continue;  // the host never configures more tiles per coarse bin
         const unsigned long long fp = b.fpre[g];
         const uint2 rng = b.bin_rng[g];
         const uint32_t c = gc - lo32(fp);
         const uint32_t nchb = (rng.y - rng.x + CHUNK - 1) / CHUNK;
         const uint32_t bi = g - sc.bin_base;
         const uint32_t binx0 = (bi % sc.nbx) * cb, biny0 = (bi / sc.nbx) * cb;
-        const size_t mbase = (size_t)hi32(fp) + c;
-        uint32_t run0 = 0, run1 = 0, run2 = 0, run3 = 0;
-        if (SCATTER) {
-            if (lane < T) run0 = b.fine_off[mbase + (size_t)lane * nchb];
-            if (lane + 64 < T) run1 = b.fine_off[mbase + (size_t)(lane + 64) * nchb];
-            if (lane + 128 < T) run2 = b.fine_off[mbase + (size_t)(lane + 128) * nchb];
-            if (lane + 192 < T) run3 = b.fine_off[mbase + (size_t)(lane + 192) * nchb];
-        }
-        for (uint32_t sub = 0; sub < CHUNK / 64; ++sub) {
-            const uint32_t e = rng.x + c * CHUNK + sub * 64 + lane;
-            const bool valid = e < rng.y;
-            uint4 en = make_uint4(0u, 0u, 0u, 0u);
-            if (valid) en = b.list[e];
-            const uint32_t x0 = en.y & 0xFFFFu, x1 = en.y >> 16, y0 = en.z & 0xFFFFu, y1 = en.z >> 16;
-            // local tile range inside this coarse bin (the entry overlaps it)
-            const uint32_t lx0 = (max(x0, binx0) - binx0) >> 4;
-            const uint32_t lx1 = valid ? (min(x1, binx0 + cb) - 1 - binx0) >> 4 : 0u;
-            const uint32_t ly0 = (max(y0, biny0) - biny0) >> 4;
-            const uint32_t ly1 = valid ? (min(y1, biny0 + cb) - 1 - biny0) >> 4 : 0u;
-            for (uint32_t ty = 0; ty < TS; ++ty) {
-                const bool rowhit = valid & (ty >= ly0) & (ty <= ly1);
-                if (__ballot(rowhit) == 0) continue;
-                for (uint32_t tx = 0; tx < TS; ++tx) {
-                    const bool hit = rowhit & (tx >= lx0) & (tx <= lx1);
-                    const uint64_t m = __ballot(hit);
-                    if (m == 0) continue;
-                    const uint32_t lt = ty * TS + tx;
-                    const uint32_t j = lt >> 6, bl = lt & 63;
-                    const uint32_t cnt_m = (uint32_t)__popcll(m);
-                    if (SCATTER) {
-                        const uint32_t rv = j == 0 ? run0 : (j == 1 ? run1 : (j == 2 ? run2 : run3));
-                        const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)rv, (int)bl);
-                        if (hit) {
-                            const uint32_t pos = r0 + mbcnt(m);
-                            if (pos < b.fine_cap) b.flist[pos] = en;
-                            else ovf = true;
-                        }
-                    }
-                    if (lane == bl) {
-                        if (j == 0) run0 += cnt_m;
-                        else if (j == 1) run1 += cnt_m;
-                        else if (j == 2) run2 += cnt_m;
-                        else run3 += cnt_m;
-                    }
-                }
+        const size_t mt = (size_t)hi32(fp) + c + (size_t)t * nchb;
+        const uint32_t e = rng.x + c * CHUNK + t;
+        const bool valid = e < rng.y;
+        uint4 en = make_uint4(0u, 0u, 0u, 0u);
+        if (valid) en = b.list[e];
+        uint32_t run = 0;
+        if (SCATTER && t < T) run = b.fine_off[mt];
+        if (t < T)
+            for (uint32_t k = 0; k < BIN_WORDS; ++k) bits[k * MAX_BINS + t] = 0u;
+        const uint32_t x0 = en.y & 0xFFFFu, x1 = en.y >> 16, y0 = en.z & 0xFFFFu, y1 = en.z >> 16;
+        // local tile range inside this coarse bin (the entry overlaps it)
+        const uint32_t lx0 = (max(x0, binx0) - binx0) >> 4;
+        const uint32_t lx1 = valid ? (min(x1, binx0 + cb) - 1 - binx0) >> 4 : 0u;
+        const uint32_t ly0 = (max(y0, biny0) - biny0) >> 4;
+        const uint32_t ly1 = valid ? (min(y1, biny0 + cb) - 1 - biny0) >> 4 : 0u;
+        uint32_t* row = bits + w * MAX_BINS;
+        __syncthreads();
+        if (valid)
+            for (uint32_t ty = ly0; ty <= ly1; ++ty)
+                for (uint32_t tx = lx0; tx <= lx1; ++tx) atomicOr(&row[ty * TS + tx], 1u << (t & 31));
+        __syncthreads();
+        if (t < T) {
+            for (uint32_t k = 0; k < BIN_WORDS; ++k) {
+                const uint32_t n = (uint32_t)__popc(bits[k * MAX_BINS + t]);
+                if (SCATTER) wbase[k * MAX_BINS + t] = run;
+                run += n;
             }
+            if (!SCATTER) b.fine_counts[mt] = run;
         }
-        if (!SCATTER) {
-            if (lane < T) b.fine_counts[mbase + (size_t)lane * nchb] = run0;
-            if (lane + 64 < T) b.fine_counts[mbase + (size_t)(lane + 64) * nchb] = run1;
-            if (lane + 128 < T) b.fine_counts[mbase + (size_t)(lane + 128) * nchb] = run2;
-            if (lane + 192 < T) b.fine_counts[mbase + (size_t)(lane + 192) * nchb] = run3;
+        if (SCATTER) {
+            __syncthreads();
+            const uint32_t* wb = wbase + w * MAX_BINS;
+            if (valid)
+                for (uint32_t ty = ly0; ty <= ly1; ++ty)
+                    for (uint32_t tx = lx0; tx <= lx1; ++tx) {
+                        const uint32_t lt = ty * TS + tx;
+                        const uint32_t pos = wb[lt] + (uint32_t)__popc(row[lt] & below);
+                        if (pos < b.fine_cap) b.flist[pos] = en;
+                        else ovf = true;
+                    }
         }
+        __syncthreads();  // bits and wbase are reused by the next chunk
     }
-    if (SCATTER && __ballot(ovf) && lane == 0) atomicOr(&b.cnt->overflow, OVF_FINE);
+    if (SCATTER && __ballot(ovf) && lane_id() == 0) atomicOr(&b.cnt->overflow, OVF_FINE);
 }
 
 // ---------------------------------------------------------------------------
