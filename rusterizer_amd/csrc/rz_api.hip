@@ -145,7 +145,7 @@ struct rz_ctx {
     DBuf<uint2> bin_rng;
     DBuf<unsigned long long> fpack, fpre;
     DBuf<uint32_t> fine_counts, fine_off;
-    DBuf<uint4> flist;
+    DBuf<uint2> flist;
     // plain-frame outputs
     DBuf<uint32_t> out;
     DBuf<uint32_t> msaa_color;

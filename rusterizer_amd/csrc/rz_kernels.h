@@ -163,7 +163,7 @@ struct JobBufs {
     unsigned long long* fpre;    // its exclusive scan
     uint32_t* fine_counts;
     uint32_t* fine_off;
-    uint4* flist;
+    uint2* flist;
     // scan partials
     uint32_t* parts;
     unsigned long long* parts64;

@@ -924,6 +924,24 @@ __global__ __launch_bounds__(256) void k_fine_prep(JobBufs b) {
     }
 }
 
+// A fine-list entry is 8 bytes: {prim, key}, key = the prim's depth lower
+// bound zlo rounded toward -inf to a multiple of 16 ulps, with the low 4 bits
+// holding which 8x8 quadrants of the tile the prim's bbox overlaps (bit
+// 2*qy + qx). The raster only compares the key against depths it stores, to
+// skip prims, so a rounded-down bound skips no prim it would have kept.
+__device__ __forceinline__ uint32_t zlo_key(uint32_t z) {
+    // negative floats: round the magnitude up; NaN stays NaN
+    return (z & 0x80000000u) ? ((min(z, 0xFFFFFFF0u) + 0xFu) & ~0xFu) : (z & ~0xFu);
+}
+
+// quadrants of the 16x16 tile at (ox, oy) that the bbox [x0,x1) x [y0,y1)
+// overlaps, given that it overlaps the tile
+__device__ __forceinline__ uint32_t quad_mask(uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1, uint32_t ox,
+                                              uint32_t oy) {
+    const uint32_t cx = (x0 < ox + 8 ? 1u : 0u) | (x1 > ox + 8 ? 2u : 0u);
+    return ((y0 < oy + 8) ? cx : 0u) | ((y1 > oy + 8) ? cx << 2 : 0u);
+}
+
 template <bool SCATTER>
 __global__ __launch_bounds__(256) void k_fine_bin(JobBufs b) {
     // one workgroup per fine-binning chunk of CHUNK list entries of one coarse
@@ -984,12 +1002,14 @@ __global__ __launch_bounds__(256) void k_fine_bin(JobBufs b) {
         if (SCATTER) {
             __syncthreads();
             const uint32_t* wb = wbase + w * MAX_BINS;
+            const uint32_t key = zlo_key(en.w);
             if (valid)
                 for (uint32_t ty = ly0; ty <= ly1; ++ty)
                     for (uint32_t tx = lx0; tx <= lx1; ++tx) {
                         const uint32_t lt = ty * TS + tx;
                         const uint32_t pos = wb[lt] + (uint32_t)__popc(row[lt] & below);
-                        if (pos < b.fine_cap) b.flist[pos] = en;
+                        const uint32_t qm = quad_mask(x0, x1, y0, y1, binx0 + tx * 16, biny0 + ty * 16);
+                        if (pos < b.fine_cap) b.flist[pos] = make_uint2(en.x, key | qm);
                         else ovf = true;
                     }
         }
@@ -1202,7 +1222,7 @@ __device__ __forceinline__ uint32_t shade_fragment(const PrimGeom& g, const floa
 struct RasterIn {
     const PrimGeom* __restrict__ geom;
     const float* __restrict__ attrs;
-    const uint4* __restrict__ flist;
+    const uint2* __restrict__ flist;
     const uint32_t* __restrict__ fine_off;
     const unsigned long long* __restrict__ fpre;
     const uint2* __restrict__ bin_rng;
@@ -1264,10 +1284,9 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
         bool cand = false;
         if (STATS) st_entries += min(64u, lend - base);
         if (i < lend) {
-            const uint4 en = in.flist[i];
+            const uint2 en = in.flist[i];
             myprim = en.x;
-            const uint32_t x0 = en.y & 0xFFFFu, x1 = en.y >> 16, y0 = en.z & 0xFFFFu, y1 = en.z >> 16;
-            cand = (x0 < qx0 + 8) & (x1 > qx0) & (y0 < qy0 + 8) & (y1 > qy0) & (__uint_as_float(en.w) < qzmax);
+            cand = ((en.y >> wave) & 1u) && __uint_as_float(en.y & ~0xFu) < qzmax;
         }
         if (cand) {
             const float4* gp = reinterpret_cast<const float4*>(geom + myprim);
@@ -1434,7 +1453,7 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
 
 template <bool STATS, bool PERSIST>
 __global__ __launch_bounds__(256) void k_raster(JobBufs b, const PrimGeom* __restrict__ geom,
-                                                const float* __restrict__ attrs, const uint4* __restrict__ flist,
+                                                const float* __restrict__ attrs, const uint2* __restrict__ flist,
                                                 const uint32_t* __restrict__ fine_off,
                                                 const unsigned long long* __restrict__ fpre,
                                                 const uint2* __restrict__ bin_rng,
