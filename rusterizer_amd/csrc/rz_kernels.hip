@@ -1281,6 +1281,7 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
     for (uint32_t base = lbeg; base < lend; base += 64) {
         const uint32_t i = base + lane;
         uint32_t myprim = 0;
+        float lane_zlb = 0.0f;
         bool cand = false;
         if (STATS) st_entries += min(64u, lend - base);
         if (i < lend) {
@@ -1291,36 +1292,48 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
         if (cand) {
             const float4* gp = reinterpret_cast<const float4*>(geom + myprim);
             const float4 g0 = gp[0], g1 = gp[1], g2 = gp[2];  // a0 a1 | a2 n0 | n1 n2
+            const float4 g5 = gp[5], g6 = gp[6];              // bbx bby zlo pz0 | pzx pzy
+            // depth lower bound over the quadrant's samples inside the bbox
+            const uint32_t bbx = __float_as_uint(g5.x), bby = __float_as_uint(g5.y);
+            const float fbx0 = (float)(bbx & 0xFFFFu), fby0 = (float)(bby & 0xFFFFu);
+            const float xl = fmaxf(qxl, fbx0 + 0.125f), xh = fminf(qxh, (float)(bbx >> 16) - 0.125f);
+            const float yl = fmaxf(qyl, fby0 + 0.125f), yh = fminf(qyh, (float)(bby >> 16) - 0.125f);
+            const float zq = g5.w + g6.x * ((g6.x >= 0.0f ? xl : xh) - fbx0) + g6.y * ((g6.y >= 0.0f ? yl : yh) - fby0);
+            lane_zlb = fmaxf(zq, g5.z);
             cand = edge_may_cover(g1.z, g1.w, g0.x, g0.y, qxl, qxh, qyl, qyh) &&
                    edge_may_cover(g2.x, g2.y, g0.z, g0.w, qxl, qxh, qyl, qyh) &&
-                   edge_may_cover(g2.z, g2.w, g1.x, g1.y, qxl, qxh, qyl, qyh);
+                   edge_may_cover(g2.z, g2.w, g1.x, g1.y, qxl, qxh, qyl, qyh) && !(lane_zlb >= qzmax);
         }
         uint64_t m = __ballot(cand);
         if (STATS) st_cand += (uint32_t)__popcll(m);
         if (m == 0) continue;
-        uint32_t nprim = (uint32_t)__builtin_amdgcn_readlane((int)myprim, (int)(__ffsll((unsigned long long)m) - 1));
+        uint32_t first = (uint32_t)__ffsll((unsigned long long)m) - 1;
+        uint32_t nprim = (uint32_t)__builtin_amdgcn_readlane((int)myprim, (int)first);
+        float nzlb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lane_zlb), (int)first));
         PrimGeom nxt = geom[nprim];
         for (;;) {
             m &= m - 1;
             const PrimGeom gm = nxt;
             const uint32_t prim = nprim;
+            const float zlb = nzlb;  // the prim's depth lower bound over the quadrant
             if (m) {  // start loading the next hit's record now
-                nprim = (uint32_t)__builtin_amdgcn_readlane((int)myprim, (int)(__ffsll((unsigned long long)m) - 1));
+                first = (uint32_t)__ffsll((unsigned long long)m) - 1;
+                nprim = (uint32_t)__builtin_amdgcn_readlane((int)myprim, (int)first);
+                nzlb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lane_zlb), (int)first));
                 nxt = geom[nprim];
             }
             bool wrote = false;
             const uint32_t bx0 = gm.bbx & 0xFFFFu, bx1 = gm.bbx >> 16, by0 = gm.bby & 0xFFFFu, by1 = gm.bby >> 16;
-            // depth lower bound over the quadrant's samples inside the bbox
-            const float fbx0 = (float)bx0, fby0 = (float)by0;
-            const float xl = fmaxf(qxl, fbx0 + 0.125f), xh = fminf(qxh, (float)bx1 - 0.125f);
-            const float yl = fmaxf(qyl, fby0 + 0.125f), yh = fminf(qyh, (float)by1 - 0.125f);
-            const float zq = gm.pz0 + gm.pzx * ((gm.pzx >= 0.0f ? xl : xh) - fbx0) +
-                             gm.pzy * ((gm.pzy >= 0.0f ? yl : yh) - fby0);
-            const float zlb = fmaxf(zq, gm.zlo);
-            if (!(zq >= qzmax) && !(gm.zlo >= qzmax)) {
+            // qzmax may have dropped since the candidates were chosen
+            if (!(zlb >= qzmax)) {
                 const bool inb = (px >= bx0) & (px < bx1) & (py >= by0) & (py < by1);
                 // mod.rs:134-146 + 148-170: coverage of the 4 samples
-                const bool t0 = gm.flags & 1u, t1 = gm.flags & 2u, t2 = gm.flags & 4u;
+                // the tie-rule test (e > 0) | (t & !(e < 0)) is a class test of
+                // e: +denormal, +normal, +inf without the rule; also NaN and
+                // both zeros with it (class bits: 0 sNaN, 1 qNaN, 5 -0, 6 +0,
+                // 7 +denormal, 8 +normal, 9 +inf)
+                const int cl0 = (gm.flags & 1u) ? 0x3E3 : 0x380, cl1 = (gm.flags & 2u) ? 0x3E3 : 0x380,
+                          cl2 = (gm.flags & 4u) ? 0x3E3 : 0x380;
                 float e1s[4], e2s[4];
                 uint32_t cov = 0;
 #pragma unroll
@@ -1329,9 +1342,8 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
                     const float e0 = edge(gm.nx0, gm.ny0, gm.ax0, gm.ay0, sx, sy);
                     const float e1 = edge(gm.nx1, gm.ny1, gm.ax1, gm.ay1, sx, sy);
                     const float e2 = edge(gm.nx2, gm.ny2, gm.ax2, gm.ay2, sx, sy);
-                    // bitwise on purpose: straight-line compares, no branches
-                    const bool in = ((e0 > 0.0f) | (t0 & !(e0 < 0.0f))) & ((e1 > 0.0f) | (t1 & !(e1 < 0.0f))) &
-                                    ((e2 > 0.0f) | (t2 & !(e2 < 0.0f)));
+                    const bool in = __builtin_amdgcn_classf(e0, cl0) & __builtin_amdgcn_classf(e1, cl1) &
+                                    __builtin_amdgcn_classf(e2, cl2);
                     cov |= (in ? 1u : 0u) << k;
                     e1s[k] = e1;
                     e2s[k] = e2;
