@@ -638,8 +638,9 @@ __device__ void depth_plane_bound(PrimGeom& o, uint32_t x0, uint32_t x1, uint32_
 // weights f_k = e_k / w_k are >= 0, u + v <= 1 + 2^-22, u + v + w <= 1 + 2^-22
 // and tu <= max(a) * (1 + 2^-21); then tu * (tw - 1) < tw - 1, every texel
 // coordinate is <= tw - 1 (th - 1 for v) and the index x + y*tw < tw*th.
-// Negative and NaN coordinates clamp to texel 0. Prims that fail the test
-// have their fragments checked exactly in the raster.
+// Negative and NaN coordinates clamp to texel 0. Prims that fail the test,
+// and all prims of a texture whose texel index does not fit the raster's
+// 32-bit fast path, have their fragments checked exactly in the raster.
 __device__ __forceinline__ bool uv_in_bounds(const ClipV* tv, const float* w) {
     const float lim = 1.0f - 0x1p-19f;
     bool ok = (w[0] > 0.0f) & (w[1] > 0.0f) & (w[2] > 0.0f);
@@ -663,6 +664,7 @@ __device__ __forceinline__ bool finite_bary(const PrimGeom& o) {
 }
 
 __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t H, uint32_t draw, uint32_t fs,
+                                           bool tex_idx32,
                                            uint32_t cb, PrimGeom* g, float* __restrict__ attr,
                                            uint4* __restrict__ pbin, uint32_t& pairs) {
     const float zmin = 0.0f, zmax = 1.0f;
@@ -703,7 +705,7 @@ __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t
     o.bby = y0 | (y1 << 16);
     o.flags = (tie_rule(o.nx0, o.ny0) ? 1u : 0u) | (tie_rule(o.nx1, o.ny1) ? 2u : 0u) |
               (tie_rule(o.nx2, o.ny2) ? 4u : 0u) | (fs << 8) | (draw << 16);
-    if (fs == RZ_FS_TEXTURE && !uv_in_bounds(tv, w)) o.flags |= PRIM_UV_CHECK;
+    if (fs == RZ_FS_TEXTURE && !(tex_idx32 && uv_in_bounds(tv, w))) o.flags |= PRIM_UV_CHECK;
     if (finite_bary(o)) o.flags |= PRIM_FINITE_BARY;
     // Depth of a covered sample is ((b0*z0 + b1*z1) + b2*z2) with clamped
     // b_i in [0, 1] and 1 - 2^-23 <= b0+b1+b2 <= 2; three roundings later it
@@ -749,6 +751,8 @@ __global__ __launch_bounds__(256) void k_setup(JobBufs b) {
         const SceneCfg& sc = b.scenes[scene];
         const SceneRun& run = b.runs[scene];
         const uint32_t W = sc.W, H = sc.H, cb = sc.cb, fs = dd.fs;
+        // texel indices of this draw's texture fit 32-bit (24-bit factor) arithmetic
+        const bool tex_idx32 = dd.tw < (1u << 24) && dd.th < (1u << 24) && (uint64_t)dd.tw * dd.th <= 0xFFFFFFFFull;
         const uint32_t lt = t - dd.tri_base;
         // the block's prim range (its 256 triangles belong to draw d: draws are 256-aligned)
         const uint32_t blk_first = uniform(run.prim_base + (b.suboff[t0] - run.prim_off));
@@ -768,7 +772,7 @@ __global__ __launch_bounds__(256) void k_setup(JobBufs b) {
                     tv[k].x = p.x; tv[k].y = p.y; tv[k].z = p.z; tv[k].w = p.w;
                 }
                 if (base < prim_cap)
-                    setup_prim(tv, W, H, d, fs, cb, staged ? stage + (base - blk_first) : b.geom + base,
+                    setup_prim(tv, W, H, d, fs, tex_idx32, cb, staged ? stage + (base - blk_first) : b.geom + base,
                                b.attrs + (size_t)base * ATTR_STRIDE, b.pbin + base, pairs);
                 else
                     ovf = true;
@@ -787,7 +791,7 @@ __global__ __launch_bounds__(256) void k_setup(JobBufs b) {
                             for (int j = 0; j < 6; ++j) tv[q].a[j] = p[4 + j];
                         }
                         if (base + k < prim_cap)
-                            setup_prim(tv, W, H, d, fs, cb,
+                            setup_prim(tv, W, H, d, fs, tex_idx32, cb,
                                        staged ? stage + (base + k - blk_first) : b.geom + base + k,
                                        b.attrs + (size_t)(base + k) * ATTR_STRIDE, b.pbin + base + k, pairs);
                         else
@@ -1033,9 +1037,10 @@ __global__ __launch_bounds__(256) void k_fine_bin(JobBufs b) {
 //
 // Work that provably cannot change a sample is skipped (exact culls):
 //  - list entries whose bbox misses the quadrant;
-//  - hierarchical depth: the quadrant keeps qzmax = max of its stored
-//    depths; a prim whose depth lower bound zlo is >= qzmax fails the strict
-//    LESS test on every sample of the quadrant;
+//  - hierarchical depth: the quadrant keeps qzmax >= max of its stored
+//    depths (refreshed after each batch of list entries), each pixel its own
+//    maximum; a prim whose depth lower bound is >= that maximum fails the
+//    strict LESS test on every sample there;
 //  - edge reject: if an edge function is negative, with a margin that covers
 //    the rounding of the reference's evaluation, at the quadrant corner where
 //    it is largest, no sample of the quadrant is inside that edge;
@@ -1068,16 +1073,27 @@ struct RGBA {
 };
 
 // texture.rs:65-83 (channels via color.rs:22-29, byte / 255.0 from a table)
+// checked: bounds-checked texel reads (texture.rs:47-63); otherwise the prim's
+// coordinates are proven in bounds and the index x + y*tw fits 32 bits.
 __device__ __forceinline__ RGBA texture_sample(const uint32_t* __restrict__ tex, uint32_t tw, uint32_t th,
-                                               float u, float v, const float* lut, bool& oob) {
+                                               float u, float v, const float* lut, bool checked, bool& oob) {
     float x = u * (float)(tw - 1);
     float y = v * (float)(th - 1);
     uint32_t xf0 = sat_u32(floorf(x)), xc = sat_u32(ceilf(x));
     uint32_t yf0 = sat_u32(floorf(y)), yc = sat_u32(ceilf(y));
-    uint32_t tl = read_texel(tex, tw, th, xf0, yf0, oob);
-    uint32_t tr = read_texel(tex, tw, th, xc, yf0, oob);
-    uint32_t bl = read_texel(tex, tw, th, xf0, yc, oob);
-    uint32_t br = read_texel(tex, tw, th, xc, yc, oob);
+    uint32_t tl, tr, bl, br;
+    if (checked) {
+        tl = read_texel(tex, tw, th, xf0, yf0, oob);
+        tr = read_texel(tex, tw, th, xc, yf0, oob);
+        bl = read_texel(tex, tw, th, xf0, yc, oob);
+        br = read_texel(tex, tw, th, xc, yc, oob);
+    } else {
+        const uint32_t r0 = __umul24(yf0, tw), r1 = __umul24(yc, tw);
+        tl = tex[r0 + xf0];
+        tr = tex[r0 + xc];
+        bl = tex[r1 + xf0];
+        br = tex[r1 + xc];
+    }
     float x_f = x - truncf(x);
     float y_f = y - truncf(y);
     float ix = 1.0f - x_f, iy = 1.0f - y_f;
@@ -1207,7 +1223,7 @@ __device__ __forceinline__ uint32_t shade_fragment(const PrimGeom& g, const floa
         const DrawDev& dd = draws[g.flags >> 16];
         const float tu = at[0] * u + at[2] * v + at[4] * w;
         const float tv = at[1] * u + at[3] * v + at[5] * w;
-        const RGBA c = texture_sample(dd.tex, dd.tw, dd.th, tu, tv, lut, oob);
+        const RGBA c = texture_sample(dd.tex, dd.tw, dd.th, tu, tv, lut, (g.flags & PRIM_UV_CHECK) != 0, oob);
         cr = c.r; cg = c.g; cbl = c.b; ca = c.a;
     } else {  // Debug: Color::grayscale(depths[0]) (color.rs:65-72)
         const float d = sample0_depth(g, fx, fy);
@@ -1311,6 +1327,7 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
         uint32_t nprim = (uint32_t)__builtin_amdgcn_readlane((int)myprim, (int)first);
         float nzlb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lane_zlb), (int)first));
         PrimGeom nxt = geom[nprim];
+        bool dirty = false;
         for (;;) {
             m &= m - 1;
             const PrimGeom gm = nxt;
@@ -1325,8 +1342,11 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
             bool wrote = false;
             const uint32_t bx0 = gm.bbx & 0xFFFFu, bx1 = gm.bbx >> 16, by0 = gm.bby & 0xFFFFu, by1 = gm.bby >> 16;
             // qzmax may have dropped since the candidates were chosen
-            if (!(zlb >= qzmax)) {
-                const bool inb = (px >= bx0) & (px < bx1) & (py >= by0) & (py < by1);
+            // live lanes: pixel inside the bbox, and some stored depth of the
+            // pixel above the bound (else no covered sample can pass LESS)
+            const bool live = !(zlb >= qzmax) && (px >= bx0) & (px < bx1) & (py >= by0) & (py < by1) &
+                                                     !(zlb >= lane_zbmax);
+            if (__ballot(live) != 0) {
                 // mod.rs:134-146 + 148-170: coverage of the 4 samples
                 // the tie-rule test (e > 0) | (t & !(e < 0)) is a class test of
                 // e: +denormal, +normal, +inf without the rule; also NaN and
@@ -1348,10 +1368,7 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
                     e1s[k] = e1;
                     e2s[k] = e2;
                 }
-                if (!inb) cov = 0;
-                // a covered lane can pass LESS only if some stored depth exceeds the
-                // bound; when no lane can, the wave skips the depth math
-                if (zlb >= lane_zbmax) cov = 0;
+                if (!live) cov = 0;
                 if (STATS) {
                     st_hits += 1;
                     st_cov += __ballot(cov != 0) ? 1u : 0u;
@@ -1371,20 +1388,23 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
                             const float b1 = __builtin_amdgcn_fmed3f(e2s[k] * gm.inv_area, 0.0f, 1.0f);
                             const float b2 = __builtin_amdgcn_fmed3f(1.0f - b0 - b1, 0.0f, 1.0f);
                             const float z = b0 * gm.z0 + b1 * gm.z1 + b2 * gm.z2;
-                            const bool c = (cov >> k) & 1u;
-                            sd[k] = c ? z : 0.0f;
-                            pass |= (c & (z < zbuf[k])) ? (1u << k) : 0u;
+                            sd[k] = z;  // written only where the sample passes
+                            pass |= (((cov >> k) & 1u) && z < zbuf[k]) ? (1u << k) : 0u;
                         }
                     } else {
 #pragma unroll
                         for (int k = 0; k < 4; ++k) {
-                            const float b0 = clamp_bary(e1s[k] * gm.inv_area);
-                            const float b1 = clamp_bary(e2s[k] * gm.inv_area);
+                            // opaque copies: keeps the compiler from sharing these
+                            // products with the other branch, where each folds into
+                            // its clamp
+                            float e1 = e1s[k], e2 = e2s[k];
+                            asm volatile("" : "+v"(e1), "+v"(e2));
+                            const float b0 = clamp_bary(e1 * gm.inv_area);
+                            const float b1 = clamp_bary(e2 * gm.inv_area);
                             const float b2 = clamp_bary(1.0f - b0 - b1);
                             const float z = b0 * gm.z0 + b1 * gm.z1 + b2 * gm.z2;
-                            const bool c = (cov >> k) & 1u;
-                            sd[k] = c ? z : 0.0f;
-                            pass |= (c & (z < zbuf[k])) ? (1u << k) : 0u;
+                            sd[k] = z;  // written only where the sample passes
+                            pass |= (((cov >> k) & 1u) && z < zbuf[k]) ? (1u << k) : 0u;
                         }
                     }
                     if (pass != 0) {
@@ -1415,10 +1435,13 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
                 }
             }
             if (STATS) st_shaded += __ballot(wrote) ? 1u : 0u;
-            if (__ballot(wrote)) {
-                qzmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(wave_max(lane_zbmax))));
-            }
+            dirty |= wrote;
             if (m == 0) break;
+        }
+        // stored depths only fall, so a stale qzmax is still an upper bound;
+        // it is refreshed once per batch of entries
+        if (__ballot(dirty)) {
+            qzmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(wave_max(lane_zbmax))));
         }
     }
     // deferred shading: each distinct surviving prim of the pixel, once
