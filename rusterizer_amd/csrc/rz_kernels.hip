@@ -1246,8 +1246,18 @@ struct RasterIn {
     const DrawDev* __restrict__ draws;
 };
 
+// LDS of the deferred shading stage: per wave, its shading tasks {prim,
+// owner lane | pass mask << 6}, their colours, and its task count (a wave's
+// 64 pixels hold at most 256 distinct prims)
+struct ShadeLds {
+    uint2 task[4][256];
+    uint32_t argb[4][256];
+    uint32_t count[4];
+};
+
 template <bool STATS>
 __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in, uint32_t gtile, const float* lut,
+                                            ShadeLds& sh,
                                             uint32_t& st_entries, uint32_t& st_cand, uint32_t& st_hits,
                                             uint32_t& st_cov, uint32_t& st_shaded) {
     const uint32_t ns = b.head->nscenes;
@@ -1444,27 +1454,59 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
             qzmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(wave_max(lane_zbmax))));
         }
     }
-    // deferred shading: each distinct surviving prim of the pixel, once
-    uint32_t cb0 = CLEAR_COLOR, cb1 = CLEAR_COLOR, cb2 = CLEAR_COLOR, cb3 = CLEAR_COLOR;
-    uint32_t todo = (pr0 != NO_PRIM ? 1u : 0u) | (pr1 != NO_PRIM ? 2u : 0u) | (pr2 != NO_PRIM ? 4u : 0u) |
-                    (pr3 != NO_PRIM ? 8u : 0u);
-    while (__ballot(todo != 0) != 0) {
-        if (todo != 0) {
-            const uint32_t k = __builtin_ctz(todo);
-            const uint32_t P = k == 0 ? pr0 : (k == 1 ? pr1 : (k == 2 ? pr2 : pr3));
-            const uint32_t pass = (pm >> (4 * k)) & 0xFu;
-            const PrimGeom gp = geom[P];
-            const uint32_t argb =
-                shade_fragment(gp, in.attrs + (size_t)P * ATTR_STRIDE, in.draws, fx, fy, pass, lut, oob);
-            const uint32_t match = (pr0 == P ? 1u : 0u) | (pr1 == P ? 2u : 0u) | (pr2 == P ? 4u : 0u) |
-                                   (pr3 == P ? 8u : 0u);
-            if (match & 1u) cb0 = argb;
-            if (match & 2u) cb1 = argb;
-            if (match & 4u) cb2 = argb;
-            if (match & 8u) cb3 = argb;
-            todo &= ~match;
+    if (onscreen && sc.msaa_depth) {
+        const float4 d4 = make_float4(zb0, zb1, zb2, zb3);
+        *reinterpret_cast<float4*>(sc.msaa_depth + ((size_t)py * W + px) * 4) = d4;
+    }
+    // deferred shading: one task per distinct surviving prim of a pixel (its
+    // first sample heads it); the workgroup's tasks are shaded 64 at a time
+    // by all four waves, then each pixel gathers its samples' colours
+    const bool h1 = pr1 != pr0, h2 = (pr2 != pr0) & (pr2 != pr1), h3 = (pr3 != pr0) & (pr3 != pr1) & (pr3 != pr2);
+    const bool v0 = pr0 != NO_PRIM, v1 = pr1 != NO_PRIM, v2 = pr2 != NO_PRIM, v3 = pr3 != NO_PRIM;
+    const bool hd0 = v0, hd1 = v1 & h1, hd2 = v2 & h2, hd3 = v3 & h3;
+    const uint32_t ntask = (uint32_t)hd0 + (uint32_t)hd1 + (uint32_t)hd2 + (uint32_t)hd3;
+    // exclusive prefix of ntask (<= 4) over the wave: three bit planes
+    const uint64_t np0 = __ballot(ntask & 1u), np1 = __ballot(ntask & 2u), np2 = __ballot(ntask & 4u);
+    const uint32_t s0 = mbcnt(np0) + 2u * mbcnt(np1) + 4u * mbcnt(np2);
+    const uint32_t s1 = s0 + (uint32_t)hd0, s2 = s1 + (uint32_t)hd1, s3 = s2 + (uint32_t)hd2;
+    uint2* const wtask = sh.task[wave];
+    if (hd0) wtask[s0] = make_uint2(pr0, lane | (pm & 0xFu) << 6);
+    if (hd1) wtask[s1] = make_uint2(pr1, lane | ((pm >> 4) & 0xFu) << 6);
+    if (hd2) wtask[s2] = make_uint2(pr2, lane | ((pm >> 8) & 0xFu) << 6);
+    if (hd3) wtask[s3] = make_uint2(pr3, lane | ((pm >> 12) & 0xFu) << 6);
+    if (lane == 0)
+        sh.count[wave] = (uint32_t)__popcll(np0) + 2u * (uint32_t)__popcll(np1) + 4u * (uint32_t)__popcll(np2);
+    // each sample's task: that of its prim's first sample (8-bit slots)
+    const uint32_t slots = s0 | (h1 ? s1 : s0) << 8 | (h2 ? s2 : (pr2 == pr0 ? s0 : s1)) << 16 |
+                           (h3 ? s3 : (pr3 == pr0 ? s0 : (pr3 == pr1 ? s1 : s2))) << 24;
+    const uint32_t vmask = (uint32_t)v0 | (uint32_t)v1 << 1 | (uint32_t)v2 << 2 | (uint32_t)v3 << 3;
+    __syncthreads();
+    {
+        const uint32_t p1 = uniform(sh.count[0]), p2 = p1 + uniform(sh.count[1]);
+        const uint32_t p3 = p2 + uniform(sh.count[2]), tot = p3 + uniform(sh.count[3]);
+        for (uint32_t gb = wave * 64u; gb < tot; gb += 256u) {
+            const uint32_t g = gb + lane;
+            if (g < tot) {
+                // task g of the workgroup: wave r's task g - (its first index)
+                const uint32_t r = (uint32_t)(g >= p1) + (uint32_t)(g >= p2) + (uint32_t)(g >= p3);
+                const uint32_t i = g - (r == 0 ? 0u : (r == 1 ? p1 : (r == 2 ? p2 : p3)));
+                const uint2 t = sh.task[r][i];
+                const uint32_t P = t.x, ol = t.y & 63u, pass = t.y >> 6;
+                // the owner's pixel
+                const float ofx = (float)(tx * 16 + (r & 1u) * 8 + (ol & 7u));
+                const float ofy = (float)(ty * 16 + (r >> 1) * 8 + (ol >> 3));
+                const PrimGeom gp = geom[P];
+                sh.argb[r][i] =
+                    shade_fragment(gp, in.attrs + (size_t)P * ATTR_STRIDE, in.draws, ofx, ofy, pass, lut, oob);
+            }
         }
     }
+    __syncthreads();
+    const uint32_t* const wargb = sh.argb[wave];
+    const uint32_t cb0 = (vmask & 1u) ? wargb[slots & 0xFFu] : CLEAR_COLOR;
+    const uint32_t cb1 = (vmask & 2u) ? wargb[(slots >> 8) & 0xFFu] : CLEAR_COLOR;
+    const uint32_t cb2 = (vmask & 4u) ? wargb[(slots >> 16) & 0xFFu] : CLEAR_COLOR;
+    const uint32_t cb3 = (vmask & 8u) ? wargb[slots >> 24] : CLEAR_COLOR;
     if (__ballot(oob) && lane == 0) atomicOr(&b.runs[s].flags, RZ_FLAG_TEXEL_OOB);
     if (onscreen) {
         // buffers.rs:111-125
@@ -1478,10 +1520,6 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
         if (sc.msaa_color) {
             uint4 c4 = make_uint4(cb0, cb1, cb2, cb3);
             *reinterpret_cast<uint4*>(sc.msaa_color + o * 4) = c4;
-        }
-        if (sc.msaa_depth) {
-            float4 d4 = make_float4(zb0, zb1, zb2, zb3);
-            *reinterpret_cast<float4*>(sc.msaa_depth + o * 4) = d4;
         }
     }
 }
@@ -1497,6 +1535,7 @@ __global__ __launch_bounds__(256) void k_raster(JobBufs b, const PrimGeom* __res
     const RasterIn in{geom, attrs, flist, fine_off, fpre, bin_rng, scenes, draws};
     __shared__ float lut[256];
     __shared__ uint32_t next_tile;
+    __shared__ ShadeLds sh;
     lut[threadIdx.x] = (float)threadIdx.x / 255.0f;  // color.rs:24-27
     __syncthreads();
     if (uniform(b.cnt->overflow) != 0) return;
@@ -1533,7 +1572,7 @@ __global__ __launch_bounds__(256) void k_raster(JobBufs b, const PrimGeom* __res
             tile = NO_TILE;
         }
         if (tile == NO_TILE) break;
-        raster_tile<STATS>(b, in, tile, lut, st_entries, st_cand, st_hits, st_cov, st_shaded);
+        raster_tile<STATS>(b, in, tile, lut, sh, st_entries, st_cand, st_hits, st_cov, st_shaded);
     }
     if (STATS && (threadIdx.x & 63) == 0) {
         atomicAdd(&b.cnt->rstat[0], (unsigned long long)st_entries);
