@@ -45,7 +45,6 @@ struct FrameCounters {
     uint32_t mat_len;       // coarse count matrix length (sum over scenes of bins * chunks)
     uint32_t pairs_binned;  // entries placed by the coarse bin scan
     uint32_t fine_total;    // entries placed by the fine bin scan
-    unsigned long long npairs;  // coarse-bin entries counted by k_setup
     uint32_t fine_chunks;   // chunks of the coarse lists (fine binning work items)
     uint32_t fmat_len;      // fine count matrix length (written with fine_chunks as one u64)
     uint32_t q[8];          // raster work queue heads, one per XCD band

@@ -181,7 +181,6 @@ __global__ __launch_bounds__(256) void k_frame_reset(JobBufs b) {
         c->mat_len = 0;
         c->pairs_binned = 0;
         c->fine_chunks = 0;
-        c->npairs = 0;
         c->nredo = 0;
         c->fmat_len = 0;
         c->fine_total = 0;
@@ -664,9 +663,8 @@ __device__ __forceinline__ bool finite_bary(const PrimGeom& o) {
 }
 
 __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t H, uint32_t draw, uint32_t fs,
-                                           bool tex_idx32,
-                                           uint32_t cb, PrimGeom* g, float* __restrict__ attr,
-                                           uint4* __restrict__ pbin, uint32_t& pairs) {
+                                           bool tex_idx32, PrimGeom* g, float* __restrict__ attr,
+                                           uint4* __restrict__ pbin) {
     const float zmin = 0.0f, zmax = 1.0f;
     float sx[3], sy[3], sz[3], w[3];
     for (int k = 0; k < 3; ++k) {
@@ -717,10 +715,6 @@ __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t
     o.pad1 = 0;
     *g = o;
     *pbin = make_uint4(0u, o.bbx, o.bby, __float_as_uint(o.zlo));
-    if (x0 < x1 && y0 < y1) {
-        uint32_t cx0 = x0 / cb, cx1 = (x1 - 1) / cb, cy0 = y0 / cb, cy1 = (y1 - 1) / cb;
-        pairs += (cx1 - cx0 + 1) * (cy1 - cy0 + 1);
-    }
     if (fs == RZ_FS_COLOR) {
         for (int k = 0; k < 3; ++k)
             for (int j = 0; j < 4; ++j) attr[k * 4 + j] = tv[k].a[j];
@@ -750,7 +744,7 @@ __global__ __launch_bounds__(256) void k_setup(JobBufs b) {
         const uint32_t scene = dd.scene;
         const SceneCfg& sc = b.scenes[scene];
         const SceneRun& run = b.runs[scene];
-        const uint32_t W = sc.W, H = sc.H, cb = sc.cb, fs = dd.fs;
+        const uint32_t W = sc.W, H = sc.H, fs = dd.fs;
         // texel indices of this draw's texture fit 32-bit (24-bit factor) arithmetic
         const bool tex_idx32 = dd.tw < (1u << 24) && dd.th < (1u << 24) && (uint64_t)dd.tw * dd.th <= 0xFFFFFFFFull;
         const uint32_t lt = t - dd.tri_base;
@@ -758,7 +752,6 @@ __global__ __launch_bounds__(256) void k_setup(JobBufs b) {
         const uint32_t blk_first = uniform(run.prim_base + (b.suboff[t0] - run.prim_off));
         const uint32_t blk_end = uniform(run.prim_base + (b.suboff[t0 + 255u] + b.nsub[t0 + 255u] - run.prim_off));
         const bool staged = blk_end - blk_first <= SETUP_STAGE && blk_end <= prim_cap;
-        uint32_t pairs = 0;
         bool ovf = false;
         if (lt < dd.ntri) {
             const uint32_t c = b.cls[t];
@@ -772,8 +765,8 @@ __global__ __launch_bounds__(256) void k_setup(JobBufs b) {
                     tv[k].x = p.x; tv[k].y = p.y; tv[k].z = p.z; tv[k].w = p.w;
                 }
                 if (base < prim_cap)
-                    setup_prim(tv, W, H, d, fs, tex_idx32, cb, staged ? stage + (base - blk_first) : b.geom + base,
-                               b.attrs + (size_t)base * ATTR_STRIDE, b.pbin + base, pairs);
+                    setup_prim(tv, W, H, d, fs, tex_idx32, staged ? stage + (base - blk_first) : b.geom + base,
+                               b.attrs + (size_t)base * ATTR_STRIDE, b.pbin + base);
                 else
                     ovf = true;
             } else if (c >= 2) {
@@ -791,9 +784,9 @@ __global__ __launch_bounds__(256) void k_setup(JobBufs b) {
                             for (int j = 0; j < 6; ++j) tv[q].a[j] = p[4 + j];
                         }
                         if (base + k < prim_cap)
-                            setup_prim(tv, W, H, d, fs, tex_idx32, cb,
+                            setup_prim(tv, W, H, d, fs, tex_idx32,
                                        staged ? stage + (base + k - blk_first) : b.geom + base + k,
-                                       b.attrs + (size_t)(base + k) * ATTR_STRIDE, b.pbin + base + k, pairs);
+                                       b.attrs + (size_t)(base + k) * ATTR_STRIDE, b.pbin + base + k);
                         else
                             ovf = true;
                     }
@@ -809,10 +802,6 @@ __global__ __launch_bounds__(256) void k_setup(JobBufs b) {
         }
         __syncthreads();  // the stage is reused by the next block
         if (__ballot(ovf) && lane_id() == 0) atomicOr(&b.cnt->overflow, OVF_PRIMS);
-        // wave sum of coarse-bin entries
-        uint32_t s = pairs;
-        for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
-        if (lane_id() == 0 && s) atomicAdd(&b.cnt->npairs, (unsigned long long)s);
     }
 }
 
