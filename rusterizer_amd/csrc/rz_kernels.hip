@@ -193,8 +193,8 @@ __global__ __launch_bounds__(256) void k_frame_reset(JobBufs b) {
 }
 
 // ---------------------------------------------------------------------------
-// k_classify: one thread per input triangle, one workgroup per TRI_ALIGN
-// triangles of one draw.
+// k_classify: one thread per input triangle; a workgroup takes chunks of
+// TRI_ALIGN triangles (each of one draw) in a grid-stride loop.
 // render.rs:104-108 (VS), render.rs:75-96 (assembly), clipping.rs:62-112
 // (degenerate cull, trivial reject / accept). Triangles that need the
 // Sutherland-Hodgman clipper get a slot in the clip list.
@@ -203,6 +203,8 @@ __global__ __launch_bounds__(256) void k_frame_reset(JobBufs b) {
 __global__ __launch_bounds__(256) void k_classify(JobBufs b) {
     __shared__ uint32_t wcount[4], wbase[4];
     const uint32_t NT = b.head->ntri, nd = b.head->ndraws;
+    const uint32_t wave = threadIdx.x >> 6;
+    uint32_t wneed = 0;  // the wave's clipped triangles over all its chunks
     for (uint32_t blk = blockIdx.x; blk * 256u < NT; blk += gridDim.x) {
         const uint32_t t = blk * 256u + threadIdx.x;
         const uint32_t d = uniform(find_draw(b.draws, nd, blk * 256u));
@@ -240,36 +242,45 @@ __global__ __launch_bounds__(256) void k_classify(JobBufs b) {
                 c = out_any ? 0u : (in_all ? 1u : 2u);
             }
         }
-        // one clip-list reservation per workgroup: wave counts -> LDS -> one atomic
+        // pass 1 marks clipped triangles with 2; their slots come below
         const bool need = live && c == 2;
-        const uint64_t m = __ballot(need);
-        const uint32_t wave = threadIdx.x >> 6;
-        if (lane_id() == 0) wcount[wave] = (uint32_t)__popcll(m);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t tot = wcount[0] + wcount[1] + wcount[2] + wcount[3];
-            uint32_t base = tot ? atomicAdd(&b.cnt->nclip, tot) : 0u;
-            for (int w = 0; w < 4; ++w) {
-                wbase[w] = base;
-                base += wcount[w];
-            }
-        }
-        __syncthreads();
-        const uint32_t slot = wbase[wave] + mbcnt(m);
+        wneed += (uint32_t)__popcll(__ballot(need));
         if (t < NT) {
-            if (need) {
-                b.cls[t] = 2u + slot;
-                b.nsub[t] = 0;  // written by k_clip
-                if (slot < b.clip_cap)
-                    b.clip_tri[slot] = t;
-                else
-                    atomicOr(&b.cnt->overflow, OVF_CLIP);
-            } else {
-                b.cls[t] = c;
-                b.nsub[t] = c;  // Inside -> 1 rasterizer triangle, Outside / padding -> 0
-            }
+            b.cls[t] = need ? 2u : c;
+            b.nsub[t] = need ? 0u : c;  // Inside -> 1 rasterizer triangle, Outside / padding -> 0;
+                                        // clipped: written by k_clip
         }
-        __syncthreads();  // wcount / wbase are reused by the next iteration
+    }
+    // one clip-list reservation per workgroup for all its chunks (a global
+    // atomic per chunk would serialize thousands of workgroups on one counter)
+    if (lane_id() == 0) wcount[wave] = wneed;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t base = 0;
+        const uint32_t tot = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        if (tot) base = atomicAdd(&b.cnt->nclip, tot);
+        for (int w = 0; w < 4; ++w) {
+            wbase[w] = base;
+            base += wcount[w];
+        }
+    }
+    __syncthreads();
+    if (wneed == 0) return;
+    // pass 2: each wave hands out its slots in triangle order
+    uint32_t next = wbase[wave];
+    for (uint32_t blk = blockIdx.x; blk * 256u < NT; blk += gridDim.x) {
+        const uint32_t t = blk * 256u + threadIdx.x;
+        const bool need = t < NT && b.cls[t] == 2u;
+        const uint64_t m = __ballot(need);
+        if (need) {
+            const uint32_t slot = next + mbcnt(m);
+            b.cls[t] = 2u + slot;
+            if (slot < b.clip_cap)
+                b.clip_tri[slot] = t;
+            else
+                atomicOr(&b.cnt->overflow, OVF_CLIP);
+        }
+        next += (uint32_t)__popcll(m);
     }
 }
 
@@ -1598,7 +1609,7 @@ void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_s
     const uint32_t scene_blocks = at_least1((g.scenes + 255) / 256);
     k_frame_reset<<<scene_blocks, 256, 0, s>>>(b);
     k_mvp<<<at_least1(g.draws), 256, 0, s>>>(b);
-    k_classify<<<at_least1(g.tris), 256, 0, s>>>(b);
+    k_classify<<<at_least1(std::min<uint32_t>(g.tris, 2048u)), 256, 0, s>>>(b);
     k_clip<<<at_least1((g.clip + CLIP_GROUPS - 1) / CLIP_GROUPS), 64, 0, s>>>(b);
     k_clip_redo<<<64, 64, 0, s>>>(b);
     launch_scan<uint32_t>(s, b.nsub, b.suboff, b.tri_cap, &b.head->ntri, b.parts, &b.cnt->nprims_raw);
