@@ -726,14 +726,17 @@ __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t
     o.pad1 = 0;
     *g = o;
     *pbin = make_uint4(0u, o.bbx, o.bby, __float_as_uint(o.zlo));
+    // vector stores of the attributes the shader reads (the record is 48-byte aligned)
     if (fs == RZ_FS_COLOR) {
-        for (int k = 0; k < 3; ++k)
-            for (int j = 0; j < 4; ++j) attr[k * 4 + j] = tv[k].a[j];
+        float4* a4 = reinterpret_cast<float4*>(attr);
+        a4[0] = make_float4(tv[0].a[0], tv[0].a[1], tv[0].a[2], tv[0].a[3]);
+        a4[1] = make_float4(tv[1].a[0], tv[1].a[1], tv[1].a[2], tv[1].a[3]);
+        a4[2] = make_float4(tv[2].a[0], tv[2].a[1], tv[2].a[2], tv[2].a[3]);
     } else if (fs == RZ_FS_TEXTURE) {
-        for (int k = 0; k < 3; ++k) {
-            attr[k * 2] = tv[k].a[4];
-            attr[k * 2 + 1] = tv[k].a[5];
-        }
+        float2* a2 = reinterpret_cast<float2*>(attr);
+        a2[0] = make_float2(tv[0].a[4], tv[0].a[5]);
+        a2[1] = make_float2(tv[1].a[4], tv[1].a[5]);
+        a2[2] = make_float2(tv[2].a[4], tv[2].a[5]);
     }
 }
 
@@ -743,7 +746,7 @@ __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t
 // 112-byte record per thread.
 constexpr uint32_t SETUP_STAGE = 352;
 
-__global__ __launch_bounds__(256) void k_setup(JobBufs b) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_setup(JobBufs b) {
     __shared__ PrimGeom stage[SETUP_STAGE];
     const uint32_t NT = b.head->ntri, nd = b.head->ndraws;
     const uint32_t prim_cap = b.prim_cap;
