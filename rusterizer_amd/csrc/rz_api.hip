@@ -541,7 +541,10 @@ int complete_job(rz_ctx* c) {
         Needs n;
         n.prims = std::max<uint64_t>(fc.nprims, c->geom.cap);
         n.clip = std::max<uint64_t>(fc.nclip, c->clip_tri.cap);
-        n.pairs = std::max<uint64_t>(fc.pairs_binned, c->list.cap);
+        // the coarse total is exact only when the coarse count pass ran (no
+        // overflow before binning); otherwise the matrix holds stale counts
+        const bool binned = !(fc.overflow & (OVF_PRIMS | OVF_CLIP | OVF_POLY | OVF_MATRIX));
+        n.pairs = std::max<uint64_t>(binned ? fc.pairs_binned : 0u, c->list.cap);
         n.fine = std::max<uint64_t>(fc.fine_total, c->flist.cap);
         n.mat = std::max<uint64_t>(fc.mat_len, c->bin_counts.cap);
         n.fmat = std::max<uint64_t>(fc.fmat_len, c->fine_counts.cap);
