@@ -1311,6 +1311,7 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
         const uint32_t i = base + lane;
         uint32_t myprim = 0;
         float lane_zlb = 0.0f;
+        uint64_t lane_box = 0;  // quadrant pixels (bit 8 * y + x) inside the prim's bbox
         bool cand = false;
         if (STATS) st_entries += min(64u, lend - base);
         if (i < lend) {
@@ -1329,6 +1330,14 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
             const float yl = fmaxf(qyl, fby0 + 0.125f), yh = fminf(qyh, (float)(bby >> 16) - 0.125f);
             const float zq = g5.w + g6.x * ((g6.x >= 0.0f ? xl : xh) - fbx0) + g6.y * ((g6.y >= 0.0f ? yl : yh) - fby0);
             lane_zlb = fmaxf(zq, g5.z);
+            {
+                const uint32_t cx0 = min(max(bbx & 0xFFFFu, qx0) - qx0, 8u), cx1 = min(max(bbx >> 16, qx0) - qx0, 8u);
+                const uint32_t cy0 = min(max(bby & 0xFFFFu, qy0) - qy0, 8u), cy1 = min(max(bby >> 16, qy0) - qy0, 8u);
+                const uint64_t row = ((1ull << cx1) - (1ull << cx0)) * 0x0101010101010101ull;
+                const uint64_t rows = (cy1 >= 8u ? ~0ull : (1ull << (8u * cy1)) - 1u) &
+                                      (cy0 >= 8u ? 0ull : ~((1ull << (8u * cy0)) - 1u));
+                lane_box = row & rows;
+            }
             cand = edge_may_cover(g1.z, g1.w, g0.x, g0.y, qxl, qxh, qyl, qyh) &&
                    edge_may_cover(g2.x, g2.y, g0.z, g0.w, qxl, qxh, qyl, qyh) &&
                    edge_may_cover(g2.z, g2.w, g1.x, g1.y, qxl, qxh, qyl, qyh) && !(lane_zlb >= qzmax);
@@ -1336,29 +1345,43 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
         uint64_t m = __ballot(cand);
         if (STATS) st_cand += (uint32_t)__popcll(m);
         if (m == 0) continue;
-        uint32_t first = (uint32_t)__ffsll((unsigned long long)m) - 1;
-        uint32_t nprim = (uint32_t)__builtin_amdgcn_readlane((int)myprim, (int)first);
-        float nzlb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lane_zlb), (int)first));
-        PrimGeom nxt = geom[nprim];
+        // the next candidate (list order) that some lane can still take --
+        // pixel inside its bbox and a stored depth above its bound -- is
+        // found from the candidates' registers; the others are skipped
+        // before their record is fetched
+        uint64_t nbox = 0;
+        float nzlb = 0.0f;
+        PrimGeom nxt;
+        uint32_t nxt_prim = 0;
+        auto next_live = [&]() -> bool {
+            while (m != 0) {
+                const uint32_t k = (uint32_t)__ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                nbox = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(lane_box >> 32), (int)k) << 32) |
+                       (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)lane_box, (int)k);
+                nzlb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lane_zlb), (int)k));
+                if ((__ballot(!(nzlb >= lane_zbmax)) & nbox) != 0) {
+                    const uint32_t nprim = (uint32_t)__builtin_amdgcn_readlane((int)myprim, (int)k);
+                    nxt = geom[nprim];  // start loading its record now
+                    nxt_prim = nprim;
+                    return true;
+                }
+            }
+            return false;
+        };
+        if (!next_live()) continue;
         bool dirty = false;
         for (;;) {
-            m &= m - 1;
             const PrimGeom gm = nxt;
-            const uint32_t prim = nprim;
+            const uint32_t prim = nxt_prim;
             const float zlb = nzlb;  // the prim's depth lower bound over the quadrant
-            if (m) {  // start loading the next hit's record now
-                first = (uint32_t)__ffsll((unsigned long long)m) - 1;
-                nprim = (uint32_t)__builtin_amdgcn_readlane((int)myprim, (int)first);
-                nzlb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lane_zlb), (int)first));
-                nxt = geom[nprim];
-            }
+            const uint64_t box = nbox;
+            // chosen with the stored depths before the previous prim's writes
+            const bool more = next_live();
             bool wrote = false;
-            const uint32_t bx0 = gm.bbx & 0xFFFFu, bx1 = gm.bbx >> 16, by0 = gm.bby & 0xFFFFu, by1 = gm.bby >> 16;
-            // qzmax may have dropped since the candidates were chosen
             // live lanes: pixel inside the bbox, and some stored depth of the
             // pixel above the bound (else no covered sample can pass LESS)
-            const bool live = !(zlb >= qzmax) && (px >= bx0) & (px < bx1) & (py >= by0) & (py < by1) &
-                                                     !(zlb >= lane_zbmax);
+            const bool live = ((box >> lane) & 1u) && !(zlb >= lane_zbmax);
             if (__ballot(live) != 0) {
                 // mod.rs:134-146 + 148-170: coverage of the 4 samples
                 // the tie-rule test (e > 0) | (t & !(e < 0)) is a class test of
@@ -1449,7 +1472,7 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
             }
             if (STATS) st_shaded += __ballot(wrote) ? 1u : 0u;
             dirty |= wrote;
-            if (m == 0) break;
+            if (!more) break;
         }
         // stored depths only fall, so a stale qzmax is still an upper bound;
         // it is refreshed once per batch of entries
