@@ -1307,15 +1307,19 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
     const uint32_t lbeg = min(s_idx < fmat_len ? uniform(in.fine_off[s_idx]) : ftot, lend);
 
     const PrimGeom* __restrict__ geom = in.geom;
+    // each batch's entries are loaded one batch ahead
+    uint2 en_next = make_uint2(0u, 0u);
+    if (lbeg + lane < lend) en_next = in.flist[lbeg + lane];
     for (uint32_t base = lbeg; base < lend; base += 64) {
         const uint32_t i = base + lane;
+        const uint2 en = en_next;
+        if (i + 64 < lend) en_next = in.flist[i + 64];
         uint32_t myprim = 0;
         float lane_zlb = 0.0f;
         uint64_t lane_box = 0;  // quadrant pixels (bit 8 * y + x) inside the prim's bbox
         bool cand = false;
         if (STATS) st_entries += min(64u, lend - base);
         if (i < lend) {
-            const uint2 en = in.flist[i];
             myprim = en.x;
             cand = ((en.y >> wave) & 1u) && __uint_as_float(en.y & ~0xFu) < qzmax;
         }
