@@ -135,7 +135,7 @@ struct rz_ctx {
     // pipeline buffers
     DBuf<uint32_t> cls, nsub, suboff, parts;
     DBuf<unsigned long long> parts64;
-    DBuf<uint32_t> clip_tri, clip_redo, clip_n;
+    DBuf<uint32_t> clip_tri, clip_redo, clip_draw;
     DBuf<float> clip_out;  // MAXV * 10 floats per slot
     DBuf<PrimGeom> geom;
     DBuf<float> attrs;  // ATTR_STRIDE per prim
@@ -366,7 +366,7 @@ int ensure_buffers(rz_ctx* c, const JobPlan& pl, const Needs& n_in) {
     if ((rc = grow(c, c->pbin, c->geom.cap))) return rc;
     if ((rc = grow(c, c->clip_tri, n.clip))) return rc;
     if ((rc = grow(c, c->clip_redo, c->clip_tri.cap))) return rc;
-    if ((rc = grow(c, c->clip_n, c->clip_tri.cap))) return rc;
+    if ((rc = grow(c, c->clip_draw, c->clip_tri.cap))) return rc;
     if ((rc = grow(c, c->clip_out, c->clip_tri.cap * MAXV * 10))) return rc;
     if ((rc = grow(c, c->list, n.pairs))) return rc;
     // coarse matrix: sum over scenes of bins * chunks <= max_nb * (slots / CHUNK)
@@ -402,7 +402,7 @@ JobBufs job_bufs(const rz_ctx* c) {
     b.suboff = c->suboff.p;
     b.clip_tri = c->clip_tri.p;
     b.clip_redo = c->clip_redo.p;
-    b.clip_n = c->clip_n.p;
+    b.clip_draw = c->clip_draw.p;
     b.clip_out = c->clip_out.p;
     b.geom = c->geom.p;
     b.attrs = c->attrs.p;
@@ -787,7 +787,7 @@ int rz_destroy(rz_ctx* c) {
     release(c->parts64);
     release(c->clip_tri);
     release(c->clip_redo);
-    release(c->clip_n);
+    release(c->clip_draw);
     release(c->clip_out);
     release(c->geom);
     release(c->attrs);

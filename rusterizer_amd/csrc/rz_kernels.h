@@ -145,9 +145,9 @@ struct JobBufs {
     uint32_t* cls;
     uint32_t* nsub;
     uint32_t* suboff;
-    uint32_t* clip_tri;
+    uint32_t* clip_tri;   // clipped triangle of each clip slot
     uint32_t* clip_redo;  // slots k_clip left to k_clip_redo
-    uint32_t* clip_n;
+    uint32_t* clip_draw;  // its draw
     float* clip_out;
     PrimGeom* geom;
     float* attrs;

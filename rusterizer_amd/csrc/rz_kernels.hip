@@ -266,19 +266,24 @@ __global__ __launch_bounds__(256) void k_classify(JobBufs b) {
     }
     __syncthreads();
     if (wneed == 0) return;
-    // pass 2: each wave hands out its slots in triangle order
+    // pass 2: each wave hands out its slots in triangle order, recording
+    // each clipped triangle and its draw
     uint32_t next = wbase[wave];
     for (uint32_t blk = blockIdx.x; blk * 256u < NT; blk += gridDim.x) {
         const uint32_t t = blk * 256u + threadIdx.x;
         const bool need = t < NT && b.cls[t] == 2u;
         const uint64_t m = __ballot(need);
+        if (m == 0) continue;
+        const uint32_t d = uniform(find_draw(b.draws, nd, blk * 256u));
         if (need) {
             const uint32_t slot = next + mbcnt(m);
             b.cls[t] = 2u + slot;
-            if (slot < b.clip_cap)
+            if (slot < b.clip_cap) {
                 b.clip_tri[slot] = t;
-            else
+                b.clip_draw[slot] = d;
+            } else {
                 atomicOr(&b.cnt->overflow, OVF_CLIP);
+            }
         }
         next += (uint32_t)__popcll(m);
     }
@@ -314,7 +319,7 @@ __device__ __forceinline__ void clip_polygon(const JobBufs& b, uint32_t slot, bo
     for (int j = 0; j < 6; ++j) v.a[j] = 0.0f;
     if (have) {
         t = b.clip_tri[slot];
-        const uint32_t d = find_draw(b.draws, b.head->ndraws, t);
+        const uint32_t d = b.clip_draw[slot];
         const DrawDev& dd = b.draws[d];
         scene = dd.scene;
         if (gl < 3) {
@@ -412,10 +417,7 @@ __device__ __forceinline__ void clip_polygon(const JobBufs& b, uint32_t slot, bo
     } else if (n > 0) {
         if (gl == 0) atomicOr(&b.runs[scene].flags, RZ_FLAG_CLIP_FEWER);
     }
-    if (gl == 0) {
-        b.nsub[t] = ntris;
-        b.clip_n[slot] = n;
-    }
+    if (gl == 0) b.nsub[t] = ntris;
 }
 
 // CLIP_GROUPS clipped triangles per wave (with clip_wide set, every one goes to k_clip_redo)
@@ -733,10 +735,10 @@ __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t
         a4[1] = make_float4(tv[1].a[0], tv[1].a[1], tv[1].a[2], tv[1].a[3]);
         a4[2] = make_float4(tv[2].a[0], tv[2].a[1], tv[2].a[2], tv[2].a[3]);
     } else if (fs == RZ_FS_TEXTURE) {
-        float2* a2 = reinterpret_cast<float2*>(attr);
-        a2[0] = make_float2(tv[0].a[4], tv[0].a[5]);
-        a2[1] = make_float2(tv[1].a[4], tv[1].a[5]);
-        a2[2] = make_float2(tv[2].a[4], tv[2].a[5]);
+        float4* a4 = reinterpret_cast<float4*>(attr);
+        a4[0] = make_float4(tv[0].a[4], tv[0].a[5], tv[1].a[4], tv[1].a[5]);
+        a4[1] = make_float4(tv[2].a[4], tv[2].a[5], 0.0f, 0.0f);
+        a4[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 }
 
