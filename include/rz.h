@@ -66,7 +66,11 @@ int rz_create(uint32_t width, uint32_t height, rz_ctx **out);
 /* Environment read by rz_create / rz_create_shared (debugging aids):
  *   RZ_RASTER_STATS=1  count raster work per frame (rz_frame_stats; slower)
  *   RZ_CLIP_WIDE=1     clip each polygon with a full 64-lane wave instead of
- *                      16 lanes (same results; exercises the wide clipper) */
+ *                      8 lanes (same results; exercises the wide clipper)
+ *   RZ_EXACT_PATHS=1   use none of the per-triangle proofs that select the
+ *                      raster's fast paths (finite barycentric clamps, texel
+ *                      reads without bounds checks): same results, slower;
+ *                      exercises the exact paths */
 /* A further context on the same device sharing ctx's meshes and textures
  * (no reference counterpart: lets several jobs -- frames or batches -- be in
  * flight at once on their own streams over the same resident data). The

@@ -168,6 +168,7 @@ struct rz_ctx {
     std::vector<uint32_t> scene_flags;
     bool raster_stats = false;  // RZ_RASTER_STATS=1 at rz_create
     bool clip_wide = false;     // RZ_CLIP_WIDE=1 at rz_create: one wave per clipped polygon
+    bool exact_paths = false;   // RZ_EXACT_PATHS=1 at rz_create: no per-prim fast-path proofs
 
     bool prof = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -427,6 +428,7 @@ JobBufs job_bufs(const rz_ctx* c) {
     b.fmat_cap = clamp_u32(c->fine_counts.cap);
     b.bins_cap = clamp_u32(c->bin_rng.cap);
     b.clip_wide = c->clip_wide ? 1u : 0u;
+    b.exact_paths = c->exact_paths ? 1u : 0u;
     return b;
 }
 
@@ -721,6 +723,8 @@ int create_impl(rz_pool* pool, uint32_t width, uint32_t height, rz_ctx** out) {
     c->raster_stats = st && st[0] == '1';
     const char* cw = getenv("RZ_CLIP_WIDE");
     c->clip_wide = cw && cw[0] == '1';
+    const char* ex = getenv("RZ_EXACT_PATHS");
+    c->exact_paths = ex && ex[0] == '1';
     if (pool) {
         c->pool = pool;
     } else {

@@ -168,7 +168,8 @@ struct JobBufs {
     unsigned long long* parts64;
     // capacities (elements)
     uint32_t tri_cap, clip_cap, prim_cap, pair_cap, mat_cap, fine_cap, fmat_cap, bins_cap;
-    uint32_t clip_wide;  // clip every polygon with a full wave (k_clip_redo)
+    uint32_t clip_wide;    // clip every polygon with a full wave (k_clip_redo)
+    uint32_t exact_paths;  // setup proves nothing: every prim takes the raster's exact paths
 };
 
 // Grid sizes of one launch sequence. Kernels loop over their work with the

@@ -676,7 +676,7 @@ __device__ __forceinline__ bool finite_bary(const PrimGeom& o) {
 }
 
 __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t H, uint32_t draw, uint32_t fs,
-                                           bool tex_idx32, PrimGeom* g, float* __restrict__ attr,
+                                           bool tex_idx32, bool proofs, PrimGeom* g, float* __restrict__ attr,
                                            uint4* __restrict__ pbin) {
     const float zmin = 0.0f, zmax = 1.0f;
     float sx[3], sy[3], sz[3], w[3];
@@ -716,8 +716,8 @@ __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t
     o.bby = y0 | (y1 << 16);
     o.flags = (tie_rule(o.nx0, o.ny0) ? 1u : 0u) | (tie_rule(o.nx1, o.ny1) ? 2u : 0u) |
               (tie_rule(o.nx2, o.ny2) ? 4u : 0u) | (fs << 8) | (draw << 16);
-    if (fs == RZ_FS_TEXTURE && !(tex_idx32 && uv_in_bounds(tv, w))) o.flags |= PRIM_UV_CHECK;
-    if (finite_bary(o)) o.flags |= PRIM_FINITE_BARY;
+    if (fs == RZ_FS_TEXTURE && !(proofs && tex_idx32 && uv_in_bounds(tv, w))) o.flags |= PRIM_UV_CHECK;
+    if (proofs && finite_bary(o)) o.flags |= PRIM_FINITE_BARY;
     // Depth of a covered sample is ((b0*z0 + b1*z1) + b2*z2) with clamped
     // b_i in [0, 1] and 1 - 2^-23 <= b0+b1+b2 <= 2; three roundings later it
     // is >= zlo below, so a sample can pass LESS only where zbuf > zlo.
@@ -763,6 +763,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         const uint32_t W = sc.W, H = sc.H, fs = dd.fs;
         // texel indices of this draw's texture fit 32-bit (24-bit factor) arithmetic
         const bool tex_idx32 = dd.tw < (1u << 24) && dd.th < (1u << 24) && (uint64_t)dd.tw * dd.th <= 0xFFFFFFFFull;
+        const bool proofs = b.exact_paths == 0;
         const uint32_t lt = t - dd.tri_base;
         // the block's prim range (its 256 triangles belong to draw d: draws are 256-aligned)
         const uint32_t blk_first = uniform(run.prim_base + (b.suboff[t0] - run.prim_off));
@@ -781,7 +782,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                     tv[k].x = p.x; tv[k].y = p.y; tv[k].z = p.z; tv[k].w = p.w;
                 }
                 if (base < prim_cap)
-                    setup_prim(tv, W, H, d, fs, tex_idx32, staged ? stage + (base - blk_first) : b.geom + base,
+                    setup_prim(tv, W, H, d, fs, tex_idx32, proofs, staged ? stage + (base - blk_first) : b.geom + base,
                                b.attrs + (size_t)base * ATTR_STRIDE, b.pbin + base);
                 else
                     ovf = true;
@@ -800,7 +801,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                             for (int j = 0; j < 6; ++j) tv[q].a[j] = p[4 + j];
                         }
                         if (base + k < prim_cap)
-                            setup_prim(tv, W, H, d, fs, tex_idx32,
+                            setup_prim(tv, W, H, d, fs, tex_idx32, proofs,
                                        staged ? stage + (base + k - blk_first) : b.geom + base + k,
                                        b.attrs + (size_t)(base + k) * ATTR_STRIDE, b.pbin + base + k);
                         else

@@ -193,8 +193,8 @@ def test_zfight_redraw(dz):
 
 
 def test_wide_clipper_same_results(monkeypatch):
-    """Clipping with 16 lanes per polygon (default) and with a full wave
-    (RZ_CLIP_WIDE=1, the path for polygons that outgrow 16 lanes) give the
+    """Clipping with 8 lanes per polygon (default) and with a full wave
+    (RZ_CLIP_WIDE=1, the path for polygons that outgrow 8 lanes) give the
     same frame, equal to the oracle."""
     sc = scenes.soup_scene(30_000, 640, 360, seed=1013, clip_frac=0.6)
     narrow = run_gpu(sc)
@@ -204,6 +204,23 @@ def test_wide_clipper_same_results(monkeypatch):
     assert np.array_equal(narrow["img"], wide["img"])
     assert np.array_equal(narrow["dep"].view(np.uint32), wide["dep"].view(np.uint32))
     assert_parity(wide, run_oracle(sc), where="wide clipper")
+
+
+def test_exact_paths_same_results(monkeypatch):
+    """With RZ_EXACT_PATHS=1 setup proves nothing, so every prim takes the
+    raster's exact paths (f32::clamp of each barycentric, bounds-checked
+    texel reads with the out-of-range check per fragment): colour, textured
+    and clipped soups render the same frames and flags, equal to the oracle."""
+    cases = [scenes.soup_scene(20_000, 480, 270, seed=1021, clip_frac=0.3),
+             scenes.soup_scene(20_000, 480, 270, seed=1022, clip_frac=0.3, textured=True)]
+    fast = [run_gpu(sc) for sc in cases]
+    monkeypatch.setenv("RZ_EXACT_PATHS", "1")
+    for sc, f in zip(cases, fast):
+        e = run_gpu(sc)
+        assert np.array_equal(f["img"], e["img"]), sc.name
+        assert np.array_equal(f["dep"].view(np.uint32), e["dep"].view(np.uint32)), sc.name
+        assert f["flags"] == e["flags"], sc.name
+        assert_parity(e, run_oracle(sc), where=f"exact paths {sc.name}")
 
 
 def test_terrain_small():
