@@ -101,9 +101,11 @@ def _cpu_model():
     return "unknown"
 
 
-def oracle_frame_seconds(sc, n_tris=None) -> float:
-    """Single-threaded oracle (C++ restatement of the reference) on sc's first
-    n_tris triangles (all by default): seconds for draw + framebuffer()."""
+def oracle_frame_seconds(sc, n_tris=None, threads: int = 1) -> float:
+    """The oracle (C++ restatement of the reference; single-threaded like the
+    reference unless threads > 1, which renders the same bits tile-parallel)
+    on sc's first n_tris triangles (all by default): seconds for draw +
+    framebuffer()."""
     from oracle.oracle import OracleRenderer
 
     n = sc.ntri if n_tris is None else min(n_tris, sc.ntri)
@@ -112,9 +114,33 @@ def oracle_frame_seconds(sc, n_tris=None) -> float:
         r.bind_texture(0, sc.texture)
     r.set_uniforms(sc.world, sc.view, sc.proj)
     t0 = time.perf_counter()
-    r.render(sc.pos, sc.color, sc.uv, sc.idx[: 3 * n], sc.fs, threads=1)
+    r.render(sc.pos, sc.color, sc.uv, sc.idx[: 3 * n], sc.fs, threads=threads)
     r.framebuffer()
     return time.perf_counter() - t0
+
+
+def raster_work(ctx, run_once, samples: int, width: int = 64, height: int = 64) -> dict:
+    """Raster work counts of one more job on a context sharing ctx's meshes
+    and textures, created with RZ_RASTER_STATS=1 (the counting kernel
+    variant)."""
+    from rusterizer_amd.render import Context
+
+    os.environ["RZ_RASTER_STATS"] = "1"
+    try:
+        sctx = Context(width, height, share=ctx)
+    finally:
+        del os.environ["RZ_RASTER_STATS"]
+    try:
+        run_once(sctx)
+        st = sctx.frame_stats()
+    finally:
+        sctx.close()
+    pixels = samples // 4
+    return {"list_entries": st["raster_entries"], "candidates": st["raster_candidates"],
+            "prims_processed": st["raster_prims"], "covered_samples": st["raster_covered_samples"],
+            "shading_tasks": st["raster_shading_tasks"],
+            "covered_per_output_sample": st["raster_covered_samples"] / max(samples, 1),
+            "shades_per_pixel": st["raster_shading_tasks"] / max(pixels, 1)}
 
 
 def roofline(alg_bytes: int, raster_ms: float, traffic):
@@ -165,6 +191,14 @@ def bench_frame(sc, steps, warmup, cpu: str | None, traffic_name: str, cpu_sampl
     ctx.profile(False)
     frames = prof["frames"] or steps
     raster_ms = prof["raster"] / frames
+
+    def stats_frame(c):  # the texture is already bound in the shared pool
+        c.set_clear_depth(sc.clear_depth)
+        c.set_uniforms(sc.world, sc.view, sc.proj)
+        c.draw(mesh, sc.fs)
+        c.resolve_device(optr)
+
+    work = raster_work(ctx, stats_frame, sc.samples, sc.width, sc.height)
     res = {
         "samples_per_s": sc.samples * steps / dt,
         "triangles_per_s": sc.ntri * steps / dt,
@@ -172,6 +206,7 @@ def bench_frame(sc, steps, warmup, cpu: str | None, traffic_name: str, cpu_sampl
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
         "stage_ms": {k: prof[k] / frames for k in ("geometry", "binning", "raster")},
         "frame_stats": stats,
+        "raster_work": work,
         "clipped_share": stats["clipped"] / max(sc.ntri, 1),
         "warmup_replays": [(w["replays"], w["overflow_bits"]) for w in warm],
         "roofline": roofline(sc.algorithmic_bytes(), raster_ms, load_traffic(traffic_name)),
@@ -280,6 +315,7 @@ def bench_batch(a, steps, warmup, with_cpu, with_plain, tune, scs, seed, t_gen):
     ctx.profile(False)
     ctx.batch_tuning(**tuning)
     eager_ms = sum(prof[k] for k in ("geometry", "binning", "raster")) / prof["frames"]
+    work = raster_work(ctx, lambda c: batch.run(c, items, table), samples)
     res = {
         "workload": f"batched: {len(scs)} textured soup scenes, seed {seed}, {tris} tris, resolutions "
                     f"{{256,512,768,1024}}^2 ({samples // 4} px), 512^2 checkerboard per scene, 4x MSAA",
@@ -292,6 +328,7 @@ def bench_batch(a, steps, warmup, with_cpu, with_plain, tune, scs, seed, t_gen):
         "stage_ms_eager": {k: prof[k] / prof["frames"] for k in ("geometry", "binning", "raster")},
         "eager_ms_per_batch": eager_ms,
         "batch_stats": stats,
+        "raster_work": work,
         "first_batch_replays": (first_stats["replays"], first_stats["overflow_bits"]),
         "setup_s": {"generate": t_gen, "upload": t_up},
         "roofline": roofline(alg_bytes, prof["raster"] / prof["frames"], load_traffic("traffic_raster_batch.json")),
@@ -314,6 +351,7 @@ def bench_batch(a, steps, warmup, with_cpu, with_plain, tune, scs, seed, t_gen):
 
 
 SERVE_WEIGHTS = {"batch": 0.90, "soup": 0.07, "terrain": 0.03}
+FRAME_REPS = 100  # timed frames of the soup and terrain legs (p50 / p99 over them)
 
 
 def bench_serve(kinds: dict, tuning: dict | None, rate: float, n: int, seed: int, slots: int = 3,
@@ -410,7 +448,8 @@ def bench_plain(scs, items, steps: int):
 
 def cpu_batch(scs, budget_s: float):
     """The oracle, single-threaded, on the batch's scenes in order until the
-    time budget is spent (at least one whole scene)."""
+    time budget is spent (at least one whole scene); beside it the same
+    scenes tile-parallel on up to 16 host threads (the same bits)."""
     done, secs, samples, tris = 0, 0.0, 0, 0
     for sc in scs:
         if secs >= budget_s and done > 0:
@@ -419,10 +458,13 @@ def cpu_batch(scs, budget_s: float):
         samples += sc.samples
         tris += sc.ntri
         done += 1
+    nt = max(1, min(16, os.cpu_count() or 1))
+    secs_mt = sum(oracle_frame_seconds(sc, threads=nt) for sc in scs[:done])
     return {"value": samples / secs, "unit": "MSAA samples/s", "cores": 1, "kind": "port",
             "sample": f"first {done} of {len(scs)} scenes ({tris} tris, {samples} samples) rendered whole "
                       f"in {secs:.2f} s on 1 host thread",
-            "triangles_per_s": tris / secs, "host_cpu": _cpu_model(), "host_nproc": os.cpu_count()}
+            "triangles_per_s": tris / secs, "host_cpu": _cpu_model(), "host_nproc": os.cpu_count(),
+            "multithread": {"value": samples / secs_mt, "cores": nt, "seconds": secs_mt}}
 
 
 def main():
@@ -465,12 +507,10 @@ def main():
         a.no_serve = True
     else:
         soup_sc, desc = soup_scene_of(a)
-        soup = dict(workload=desc, **bench_frame(soup_sc, max(5, a.steps // 2), 2, None,
-                                                 "traffic_raster_soup1m.json"))
+        soup = dict(workload=desc, **bench_frame(soup_sc, FRAME_REPS, 2, None, "traffic_raster_soup1m.json"))
         log(f"soup: {soup['ms_per_frame']:.2f} ms per frame")
         terr_sc, desc = terrain_scene_of(a)
-        terrain = dict(workload=desc, **bench_frame(terr_sc, max(5, a.steps // 2), 2, None,
-                                                    "traffic_raster_terrain.json"))
+        terrain = dict(workload=desc, **bench_frame(terr_sc, FRAME_REPS, 2, None, "traffic_raster_terrain.json"))
         log(f"terrain: {terrain['ms_per_frame']:.2f} ms per frame")
     if not a.no_serve:
         from rusterizer_amd import scenes as S

@@ -136,9 +136,11 @@ uint32_t rz_frame_flags(const rz_ctx *ctx);
 /* Counts of the last resolved frame or batch: [input triangles, triangles clipped,
  * rasterizer triangles, coarse bin entries, frame replays, then -- only when
  * the context was created with RZ_RASTER_STATS=1 in the environment, else 0 --
- * raster list entries scanned, candidates after the bbox / depth / edge
- * culls, hits past the depth-plane cull, hits that ran the depth test, hits
- * that wrote samples, then the buffer-overflow reasons that caused replays
+ * raster work: tile-list entries scanned and candidates left by the list
+ * culls (both summed over the 4 quadrant waves of a tile), prims processed
+ * per quadrant, MSAA samples they covered, and shading tasks (distinct
+ * surviving prims of a pixel, each shaded once), then the buffer-overflow
+ * reasons that caused replays
  * (bits: 1 triangles, 2 clip slots, 4 coarse entries, 16 fine entries)]. */
 int rz_frame_stats(const rz_ctx *ctx, uint64_t out[11]);
 

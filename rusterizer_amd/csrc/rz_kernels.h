@@ -50,9 +50,10 @@ struct FrameCounters {
     uint32_t q[8];          // raster work queue heads, one per XCD band
     uint32_t nredo;         // clipped polygons handed from k_clip to k_clip_redo
     uint32_t pad;
-    // raster work counters (stats variant only): list entries scanned,
-    // candidates after the bbox / depth / edge culls, hits processed,
-    // hits with coverage, hits that shaded
+    // raster work counters (stats variant only), per wave: list entries
+    // scanned, candidates after the list culls, prims processed (some pixel
+    // could take them); per sample: samples covered by processed prims; per
+    // pixel: shading tasks (distinct surviving prims of a pixel)
     unsigned long long rstat[5];
 };
 static_assert(offsetof(FrameCounters, fine_chunks) % 8 == 0 &&

@@ -1414,7 +1414,7 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
                 if (!live) cov = 0;
                 if (STATS) {
                     st_hits += 1;
-                    st_cov += __ballot(cov != 0) ? 1u : 0u;
+                    st_cov += (uint32_t)__popc(cov);  // this lane's covered samples
                 }
                 if (__ballot(cov != 0) != 0) {
                     // mod.rs:225-253: linear depth at covered samples; mod.rs:363-378: strict LESS
@@ -1477,7 +1477,6 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
                     }
                 }
             }
-            if (STATS) st_shaded += __ballot(wrote) ? 1u : 0u;
             dirty |= wrote;
             if (!more) break;
         }
@@ -1531,6 +1530,7 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
                 const PrimGeom gp = geom[P];
                 sh.argb[r][i] =
                     shade_fragment(gp, in.attrs + (size_t)P * ATTR_STRIDE, in.draws, ofx, ofy, pass, lut, oob);
+                if (STATS) st_shaded += 1;  // shading tasks run by this lane
             }
         }
     }
@@ -1606,6 +1606,12 @@ __global__ __launch_bounds__(256) void k_raster(JobBufs b, const PrimGeom* __res
         }
         if (tile == NO_TILE) break;
         raster_tile<STATS>(b, in, tile, lut, sh, st_entries, st_cand, st_hits, st_cov, st_shaded);
+    }
+    if (STATS) {  // the last two counts are per lane
+        for (int o = 32; o >= 1; o >>= 1) {
+            st_cov += __shfl_xor(st_cov, o);
+            st_shaded += __shfl_xor(st_shaded, o);
+        }
     }
     if (STATS && (threadIdx.x & 63) == 0) {
         atomicAdd(&b.cnt->rstat[0], (unsigned long long)st_entries);

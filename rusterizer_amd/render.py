@@ -169,7 +169,7 @@ class Context:
         a = (C.c_uint64 * 11)()
         self._check(self._L.rz_frame_stats(self.handle, a))
         keys = ["tris", "clipped", "prims", "bin_entries", "replays", "raster_entries", "raster_candidates",
-                "raster_hits", "raster_covered", "raster_shaded", "overflow_bits"]
+                "raster_prims", "raster_covered_samples", "raster_shading_tasks", "overflow_bits"]
         return dict(zip(keys, (int(x) for x in a)))
 
     # ---- batches
