@@ -1383,7 +1383,9 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
             const uint32_t prim = nxt_prim;
             const float zlb = nzlb;  // the prim's depth lower bound over the quadrant
             const uint64_t box = nbox;
-            // chosen with the stored depths before the previous prim's writes
+            // pick the next live candidate now, so its record loads while this
+            // prim is processed (judged on the stored depths before this
+            // prim's writes: it may turn out to have no live lane)
             const bool more = next_live();
             bool wrote = false;
             // live lanes: pixel inside the bbox, and some stored depth of the
