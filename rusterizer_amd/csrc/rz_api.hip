@@ -143,6 +143,7 @@ struct rz_ctx {
     DBuf<uint32_t> bin_counts, bin_offsets;
     DBuf<uint4> list;
     DBuf<uint2> bin_rng;
+    DBuf<uint4> tile_rec;
     DBuf<unsigned long long> fpack, fpre;
     DBuf<uint32_t> fine_counts, fine_off;
     DBuf<uint2> flist;
@@ -375,6 +376,7 @@ int ensure_buffers(rz_ctx* c, const JobPlan& pl, const Needs& n_in) {
     if ((rc = grow(c, c->bin_counts, mat))) return rc;
     if ((rc = grow(c, c->bin_offsets, c->bin_counts.cap))) return rc;
     if ((rc = grow(c, c->bin_rng, std::max<uint64_t>(pl.nbins, 1)))) return rc;
+    if ((rc = grow(c, c->tile_rec, std::max<uint64_t>(pl.ntiles, 1)))) return rc;
     if ((rc = grow(c, c->fpack, c->bin_rng.cap))) return rc;
     if ((rc = grow(c, c->fpre, c->bin_rng.cap))) return rc;
     if ((rc = grow(c, c->flist, n.fine))) return rc;
@@ -412,6 +414,7 @@ JobBufs job_bufs(const rz_ctx* c) {
     b.bin_offsets = c->bin_offsets.p;
     b.list = c->list.p;
     b.bin_rng = c->bin_rng.p;
+    b.tile_rec = c->tile_rec.p;
     b.fpack = c->fpack.p;
     b.fpre = c->fpre.p;
     b.fine_counts = c->fine_counts.p;
@@ -427,6 +430,7 @@ JobBufs job_bufs(const rz_ctx* c) {
     b.fine_cap = clamp_u32(c->flist.cap);
     b.fmat_cap = clamp_u32(c->fine_counts.cap);
     b.bins_cap = clamp_u32(c->bin_rng.cap);
+    b.tile_cap = clamp_u32(c->tile_rec.cap);
     b.clip_wide = c->clip_wide ? 1u : 0u;
     b.exact_paths = c->exact_paths ? 1u : 0u;
     return b;
@@ -800,6 +804,7 @@ int rz_destroy(rz_ctx* c) {
     release(c->bin_offsets);
     release(c->list);
     release(c->bin_rng);
+    release(c->tile_rec);
     release(c->fpack);
     release(c->fpre);
     release(c->fine_counts);

@@ -1029,6 +1029,35 @@ __global__ __launch_bounds__(256) void k_fine_bin(JobBufs b) {
 }
 
 // ---------------------------------------------------------------------------
+// k_tile_prep: one thread per raster tile: its scene and the range of its
+// fine list (the fine matrix entry of (its coarse bin, its local tile, chunk
+// 0) and of the next tile), so the raster starts a tile with one load
+// instead of a chain of searches and lookups. Ranges are clamped to the list
+// buffer: a bad range can never read past it.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_tile_prep(JobBufs b) {
+    const uint32_t nt = min(b.head->ntiles, b.tile_cap), ns = b.head->nscenes;
+    if (b.cnt->overflow != 0) return;
+    const uint32_t fmat_len = b.cnt->fmat_len, ftot = b.cnt->fine_total;
+    for (uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x; gt < nt; gt += gridDim.x * blockDim.x) {
+        const uint32_t s = find_scene_tile(b.scenes, ns, gt);
+        const SceneCfg& sc = b.scenes[s];
+        const uint32_t ntx = (sc.W + 15) / 16, tile = gt - sc.tile_base;
+        const uint32_t tx = tile % ntx, ty = tile / ntx;
+        const uint32_t cbs = sc.cb, TS = cbs / 16;
+        const uint32_t cbx = (tx * 16) / cbs, cby = (ty * 16) / cbs;
+        const uint32_t g = sc.bin_base + cby * sc.nbx + cbx;
+        const uint32_t lt = ((ty * 16 - cby * cbs) >> 4) * TS + ((tx * 16 - cbx * cbs) >> 4);
+        const uint2 rng = b.bin_rng[g];
+        const uint32_t nchb = (rng.y - rng.x + CHUNK - 1) / CHUNK;
+        const size_t s_idx = (size_t)hi32(b.fpre[g]) + (size_t)lt * nchb, e_idx = s_idx + nchb;
+        const uint32_t lend = min(e_idx < fmat_len ? b.fine_off[e_idx] : ftot, b.fine_cap);
+        const uint32_t lbeg = min(s_idx < fmat_len ? b.fine_off[s_idx] : ftot, lend);
+        b.tile_rec[gt] = make_uint4(s, lbeg, lend, 0u);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // k_raster: one workgroup per 16x16 screen tile at a time, one wave per 8x8
 // quadrant, one lane per pixel. The pixel's 4 MSAA colours and depths live in
 // the lane's registers for the whole frame; the wave walks its tile's fine
@@ -1245,9 +1274,7 @@ struct RasterIn {
     const PrimGeom* __restrict__ geom;
     const float* __restrict__ attrs;
     const uint2* __restrict__ flist;
-    const uint32_t* __restrict__ fine_off;
-    const unsigned long long* __restrict__ fpre;
-    const uint2* __restrict__ bin_rng;
+    const uint4* __restrict__ tile_rec;
     const SceneCfg* __restrict__ scenes;
     const DrawDev* __restrict__ draws;
 };
@@ -1266,8 +1293,9 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
                                             ShadeLds& sh,
                                             uint32_t& st_entries, uint32_t& st_cand, uint32_t& st_hits,
                                             uint32_t& st_cov, uint32_t& st_shaded) {
-    const uint32_t ns = b.head->nscenes;
-    const uint32_t s = uniform(find_scene_tile(in.scenes, ns, gtile));
+    // the tile's scene and list range (k_tile_prep)
+    const uint4 tr = in.tile_rec[gtile];
+    const uint32_t s = uniform(tr.x);
     const SceneCfg& sc = in.scenes[s];
     const uint32_t W = sc.W, H = sc.H;
     const float clear_depth = sc.clear_depth;
@@ -1295,19 +1323,7 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
     // lane_zbmax: largest depth stored in this pixel's 4 samples
     float lane_zbmax = onscreen ? clear_depth : -FLT_MAX_F;
 
-    // this tile's fine list (see k_fine_bin for the layout)
-    const uint32_t cbs = sc.cb, TS = cbs / 16;
-    const uint32_t cbx = (tx * 16) / cbs, cby = (ty * 16) / cbs;
-    const uint32_t g = sc.bin_base + cby * sc.nbx + cbx;
-    const uint32_t lt = ((ty * 16 - cby * cbs) >> 4) * TS + ((tx * 16 - cbx * cbs) >> 4);
-    const unsigned long long fp = in.fpre[g];
-    const uint2 rng = in.bin_rng[g];
-    const uint32_t nchb = uniform((rng.y - rng.x + CHUNK - 1) / CHUNK);
-    const uint32_t fmat_len = uniform(b.cnt->fmat_len), ftot = uniform(b.cnt->fine_total);
-    const size_t s_idx = (size_t)uniform(hi32(fp)) + (size_t)lt * nchb, e_idx = s_idx + nchb;
-    // (clamped to the list buffer: a bad range can never read past it)
-    const uint32_t lend = min(e_idx < fmat_len ? uniform(in.fine_off[e_idx]) : ftot, b.fine_cap);
-    const uint32_t lbeg = min(s_idx < fmat_len ? uniform(in.fine_off[s_idx]) : ftot, lend);
+    const uint32_t lbeg = uniform(tr.y), lend = uniform(tr.z);
 
     const PrimGeom* __restrict__ geom = in.geom;
     // each batch's entries are loaded one batch ahead
@@ -1562,12 +1578,10 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
 template <bool STATS, bool PERSIST>
 __global__ __launch_bounds__(256) void k_raster(JobBufs b, const PrimGeom* __restrict__ geom,
                                                 const float* __restrict__ attrs, const uint2* __restrict__ flist,
-                                                const uint32_t* __restrict__ fine_off,
-                                                const unsigned long long* __restrict__ fpre,
-                                                const uint2* __restrict__ bin_rng,
+                                                const uint4* __restrict__ tile_rec,
                                                 const SceneCfg* __restrict__ scenes,
                                                 const DrawDev* __restrict__ draws) {
-    const RasterIn in{geom, attrs, flist, fine_off, fpre, bin_rng, scenes, draws};
+    const RasterIn in{geom, attrs, flist, tile_rec, scenes, draws};
     __shared__ float lut[256];
     __shared__ uint32_t next_tile;
     __shared__ ShadeLds sh;
@@ -1668,10 +1682,11 @@ void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_s
     k_fine_bin<false><<<at_least1(g.fine), 256, 0, s>>>(b);
     launch_scan<uint32_t>(s, b.fine_counts, b.fine_off, b.fmat_cap, &b.cnt->fmat_len, b.parts, &b.cnt->fine_total);
     k_fine_bin<true><<<at_least1(g.fine), 256, 0, s>>>(b);
+    k_tile_prep<<<at_least1(std::min<uint32_t>((g.ntiles + 255) / 256, 4096u)), 256, 0, s>>>(b);
     if (ev_raster) (void)hipEventRecord(ev_raster, s);
     const uint32_t rgrid = g.raster ? g.raster : at_least1((g.ntiles + 7) / 8 * 8);
 #define RZ_RASTER(ST, PE) \
-    k_raster<ST, PE><<<rgrid, 256, 0, s>>>(b, b.geom, b.attrs, b.flist, b.fine_off, b.fpre, b.bin_rng, b.scenes, b.draws)
+    k_raster<ST, PE><<<rgrid, 256, 0, s>>>(b, b.geom, b.attrs, b.flist, b.tile_rec, b.scenes, b.draws)
     if (raster_stats) {
         if (g.raster) RZ_RASTER(true, true);
         else RZ_RASTER(true, false);
