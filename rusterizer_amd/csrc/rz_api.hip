@@ -144,6 +144,7 @@ struct rz_ctx {
     DBuf<uint4> list;
     DBuf<uint2> bin_rng;
     DBuf<uint4> tile_rec;
+    DBuf<uint2> fchunk_rec;
     DBuf<unsigned long long> fpack, fpre;
     DBuf<uint32_t> fine_counts, fine_off;
     DBuf<uint2> flist;
@@ -380,6 +381,8 @@ int ensure_buffers(rz_ctx* c, const JobPlan& pl, const Needs& n_in) {
     if ((rc = grow(c, c->fpack, c->bin_rng.cap))) return rc;
     if ((rc = grow(c, c->fpre, c->bin_rng.cap))) return rc;
     if ((rc = grow(c, c->flist, n.fine))) return rc;
+    // fine-binning chunks: sum over bins of ceil(len / CHUNK) <= entries / CHUNK + bins
+    if ((rc = grow(c, c->fchunk_rec, c->list.cap / CHUNK + c->bin_rng.cap + 1))) return rc;
     // fine matrix: sum over bins of T * ceil(len / CHUNK) <= max_T * (entries / CHUNK + bins)
     const uint64_t fmat =
         std::max<uint64_t>(n.fmat, (uint64_t)pl.max_T * (c->list.cap / CHUNK + c->bin_rng.cap + 1));
@@ -415,6 +418,7 @@ JobBufs job_bufs(const rz_ctx* c) {
     b.list = c->list.p;
     b.bin_rng = c->bin_rng.p;
     b.tile_rec = c->tile_rec.p;
+    b.fchunk_rec = c->fchunk_rec.p;
     b.fpack = c->fpack.p;
     b.fpre = c->fpre.p;
     b.fine_counts = c->fine_counts.p;
@@ -431,6 +435,7 @@ JobBufs job_bufs(const rz_ctx* c) {
     b.fmat_cap = clamp_u32(c->fine_counts.cap);
     b.bins_cap = clamp_u32(c->bin_rng.cap);
     b.tile_cap = clamp_u32(c->tile_rec.cap);
+    b.fchunk_cap = clamp_u32(c->fchunk_rec.cap);
     b.clip_wide = c->clip_wide ? 1u : 0u;
     b.exact_paths = c->exact_paths ? 1u : 0u;
     return b;
@@ -805,6 +810,7 @@ int rz_destroy(rz_ctx* c) {
     release(c->list);
     release(c->bin_rng);
     release(c->tile_rec);
+    release(c->fchunk_rec);
     release(c->fpack);
     release(c->fpre);
     release(c->fine_counts);

@@ -100,18 +100,6 @@ __device__ __forceinline__ uint32_t find_scene_bin(const SceneCfg* __restrict__ 
     }
     return lo;
 }
-// global coarse bin holding fine-binning chunk gc
-__device__ __forceinline__ uint32_t find_bin_fchunk(const unsigned long long* __restrict__ fpre, uint32_t nb,
-                                                    uint32_t gc) {
-    uint32_t lo = 0, hi = nb;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (lo32(fpre[mid]) <= gc) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 // ---------------------------------------------------------------------------
 // Vertex stage: ((P*V)*W) * (x, y, z, 1) with the reference's dot order
 // ---------------------------------------------------------------------------
@@ -952,6 +940,20 @@ __device__ __forceinline__ uint32_t quad_mask(uint32_t x0, uint32_t x1, uint32_t
     return ((y0 < oy + 8) ? cx : 0u) | ((y1 > oy + 8) ? cx << 2 : 0u);
 }
 
+// k_fchunk_map, one thread per global coarse bin: writes {bin, scene} for
+// each of the bin's fine-binning chunks, so a chunk's workgroup finds its bin
+// with one load instead of two binary searches.
+__global__ __launch_bounds__(256) void k_fchunk_map(JobBufs b) {
+    if (b.cnt->overflow != 0) return;
+    const uint32_t NB = b.head->nbins, ns = b.head->nscenes, cap = b.fchunk_cap;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < NB; g += gridDim.x * blockDim.x) {
+        const uint32_t first = lo32(b.fpre[g]), n = lo32(b.fpack[g]);
+        if (n == 0) continue;
+        const uint32_t s = find_scene_bin(b.scenes, ns, g);
+        for (uint32_t c = 0; c < n && first + c < cap; ++c) b.fchunk_rec[first + c] = make_uint2(g, s);
+    }
+}
+
 template <bool SCATTER>
 __global__ __launch_bounds__(256) void k_fine_bin(JobBufs b) {
     // one workgroup per fine-binning chunk of CHUNK list entries of one coarse
@@ -960,8 +962,7 @@ __global__ __launch_bounds__(256) void k_fine_bin(JobBufs b) {
     __shared__ uint32_t bits[BIN_WORDS * MAX_BINS];
     __shared__ uint32_t wbase[SCATTER ? BIN_WORDS * MAX_BINS : 1];
     if (uniform(b.cnt->overflow) != 0) return;
-    const uint32_t NB = b.head->nbins, ns = b.head->nscenes;
-    const uint32_t NCH = uniform(b.cnt->fine_chunks);
+    const uint32_t NCH = uniform(min(b.cnt->fine_chunks, b.fchunk_cap));
     if (uniform(b.cnt->fmat_len) > b.fmat_cap) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&b.cnt->overflow, OVF_FINE);
         return;
@@ -969,8 +970,8 @@ __global__ __launch_bounds__(256) void k_fine_bin(JobBufs b) {
     const uint32_t t = threadIdx.x, w = t >> 5, below = (1u << (t & 31)) - 1u;
     bool ovf = false;
     for (uint32_t gc = blockIdx.x; gc < NCH; gc += gridDim.x) {
-        const uint32_t g = uniform(find_bin_fchunk(b.fpre, NB, gc));
-        const uint32_t s = uniform(find_scene_bin(b.scenes, ns, g));
+        const uint2 fr = b.fchunk_rec[gc];
+        const uint32_t g = uniform(fr.x), s = uniform(fr.y);
         const SceneCfg& sc = b.scenes[s];
         const uint32_t cb = sc.cb, TS = cb / 16, T = TS * TS;
         if (T > MAX_BINS) continue;  // the host never configures more tiles per coarse bin
@@ -1679,6 +1680,7 @@ void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_s
     // fine_chunks and fmat_len are adjacent: the u64 total fills both
     launch_scan<unsigned long long>(s, b.fpack, b.fpre, b.bins_cap, &b.head->nbins, b.parts64,
                                     reinterpret_cast<unsigned long long*>(&b.cnt->fine_chunks));
+    k_fchunk_map<<<at_least1(std::min<uint32_t>((b.bins_cap + 255) / 256, 1024u)), 256, 0, s>>>(b);
     k_fine_bin<false><<<at_least1(g.fine), 256, 0, s>>>(b);
     launch_scan<uint32_t>(s, b.fine_counts, b.fine_off, b.fmat_cap, &b.cnt->fmat_len, b.parts, &b.cnt->fine_total);
     k_fine_bin<true><<<at_least1(g.fine), 256, 0, s>>>(b);
