@@ -145,6 +145,7 @@ struct rz_ctx {
     DBuf<uint2> bin_rng;
     DBuf<uint4> tile_rec;
     DBuf<uint2> fchunk_rec;
+    DBuf<uint32_t> tchunk_draw, pchunk_scene;
     DBuf<unsigned long long> fpack, fpre;
     DBuf<uint32_t> fine_counts, fine_off;
     DBuf<uint2> flist;
@@ -367,6 +368,8 @@ int ensure_buffers(rz_ctx* c, const JobPlan& pl, const Needs& n_in) {
     if ((rc = grow(c, c->geom, n.prims))) return rc;
     if ((rc = grow(c, c->attrs, c->geom.cap * ATTR_STRIDE))) return rc;
     if ((rc = grow(c, c->pbin, c->geom.cap))) return rc;
+    if ((rc = grow(c, c->tchunk_draw, c->cls.cap / TRI_ALIGN + 1))) return rc;
+    if ((rc = grow(c, c->pchunk_scene, c->geom.cap / CHUNK + 1))) return rc;
     if ((rc = grow(c, c->clip_tri, n.clip))) return rc;
     if ((rc = grow(c, c->clip_redo, c->clip_tri.cap))) return rc;
     if ((rc = grow(c, c->clip_draw, c->clip_tri.cap))) return rc;
@@ -419,6 +422,8 @@ JobBufs job_bufs(const rz_ctx* c) {
     b.bin_rng = c->bin_rng.p;
     b.tile_rec = c->tile_rec.p;
     b.fchunk_rec = c->fchunk_rec.p;
+    b.tchunk_draw = c->tchunk_draw.p;
+    b.pchunk_scene = c->pchunk_scene.p;
     b.fpack = c->fpack.p;
     b.fpre = c->fpre.p;
     b.fine_counts = c->fine_counts.p;
@@ -436,6 +441,8 @@ JobBufs job_bufs(const rz_ctx* c) {
     b.bins_cap = clamp_u32(c->bin_rng.cap);
     b.tile_cap = clamp_u32(c->tile_rec.cap);
     b.fchunk_cap = clamp_u32(c->fchunk_rec.cap);
+    b.tchunk_cap = clamp_u32(c->tchunk_draw.cap);
+    b.pchunk_cap = clamp_u32(c->pchunk_scene.cap);
     b.clip_wide = c->clip_wide ? 1u : 0u;
     b.exact_paths = c->exact_paths ? 1u : 0u;
     return b;
@@ -811,6 +818,8 @@ int rz_destroy(rz_ctx* c) {
     release(c->bin_rng);
     release(c->tile_rec);
     release(c->fchunk_rec);
+    release(c->tchunk_draw);
+    release(c->pchunk_scene);
     release(c->fpack);
     release(c->fpre);
     release(c->fine_counts);

@@ -116,6 +116,22 @@ __device__ __forceinline__ float4 mat_vec_p(const float* __restrict__ M, float x
 
 // main.rs:147-152: projection * view * world, evaluated left to right
 // (matrix.rs:56-79). One thread per draw; written into the draw descriptor.
+// per-chunk owner tables, one thread per chunk (the searches run in parallel
+// here instead of serially at the start of every workgroup that needs them):
+// k_tchunk_map: the draw of each TRI_ALIGN-triangle chunk (draws start at
+// chunk boundaries); k_pchunk_map: the scene of each CHUNK-prim binning chunk
+__global__ __launch_bounds__(256) void k_tchunk_map(JobBufs b) {
+    const uint32_t nc = min((b.head->ntri + TRI_ALIGN - 1) / TRI_ALIGN, b.tchunk_cap), nd = b.head->ndraws;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
+        b.tchunk_draw[c] = find_draw(b.draws, nd, c * TRI_ALIGN);
+}
+__global__ __launch_bounds__(256) void k_pchunk_map(JobBufs b) {
+    if (b.cnt->overflow != 0) return;
+    const uint32_t nc = min(b.cnt->nchunks, b.pchunk_cap), ns = b.head->nscenes;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
+        b.pchunk_scene[c] = find_scene_chunk(b.runs, ns, c);
+}
+
 __global__ __launch_bounds__(256) void k_mvp(JobBufs b) {
     const uint32_t nd = b.head->ndraws;
     for (uint32_t d = blockIdx.x * blockDim.x + threadIdx.x; d < nd; d += gridDim.x * blockDim.x) {
@@ -190,12 +206,12 @@ __global__ __launch_bounds__(256) void k_frame_reset(JobBufs b) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_classify(JobBufs b) {
     __shared__ uint32_t wcount[4], wbase[4];
-    const uint32_t NT = b.head->ntri, nd = b.head->ndraws;
+    const uint32_t NT = b.head->ntri;
     const uint32_t wave = threadIdx.x >> 6;
     uint32_t wneed = 0;  // the wave's clipped triangles over all its chunks
     for (uint32_t blk = blockIdx.x; blk * 256u < NT; blk += gridDim.x) {
         const uint32_t t = blk * 256u + threadIdx.x;
-        const uint32_t d = uniform(find_draw(b.draws, nd, blk * 256u));
+        const uint32_t d = uniform(b.tchunk_draw[blk]);
         const DrawDev& dd = b.draws[d];
         const uint32_t lt = t - dd.tri_base;
         const bool live = lt < dd.ntri;
@@ -262,7 +278,7 @@ __global__ __launch_bounds__(256) void k_classify(JobBufs b) {
         const bool need = t < NT && b.cls[t] == 2u;
         const uint64_t m = __ballot(need);
         if (m == 0) continue;
-        const uint32_t d = uniform(find_draw(b.draws, nd, blk * 256u));
+        const uint32_t d = uniform(b.tchunk_draw[blk]);
         if (need) {
             const uint32_t slot = next + mbcnt(m);
             b.cls[t] = 2u + slot;
@@ -738,12 +754,12 @@ constexpr uint32_t SETUP_STAGE = 352;
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_setup(JobBufs b) {
     __shared__ PrimGeom stage[SETUP_STAGE];
-    const uint32_t NT = b.head->ntri, nd = b.head->ndraws;
+    const uint32_t NT = b.head->ntri;
     const uint32_t prim_cap = b.prim_cap;
     for (uint32_t blk = blockIdx.x; blk * 256u < NT; blk += gridDim.x) {
         const uint32_t t0 = blk * 256u;
         const uint32_t t = t0 + threadIdx.x;
-        const uint32_t d = uniform(find_draw(b.draws, nd, t0));
+        const uint32_t d = uniform(b.tchunk_draw[blk]);
         const DrawDev& dd = b.draws[d];
         const uint32_t scene = dd.scene;
         const SceneCfg& sc = b.scenes[scene];
@@ -832,11 +848,11 @@ __global__ __launch_bounds__(256) void k_bin(JobBufs b) {
     __shared__ uint32_t bits[BIN_WORDS * MAX_BINS];
     __shared__ uint32_t wbase[SCATTER ? BIN_WORDS * MAX_BINS : 1];
     if (uniform(b.cnt->overflow) != 0) return;
-    const uint32_t NC = uniform(b.cnt->nchunks), ns = b.head->nscenes;
+    const uint32_t NC = uniform(b.cnt->nchunks);
     const uint32_t t = threadIdx.x, w = t >> 5, below = (1u << (t & 31)) - 1u;
     bool ovf = false;
     for (uint32_t chunk = blockIdx.x; chunk < NC; chunk += gridDim.x) {
-        const uint32_t s = uniform(find_scene_chunk(b.runs, ns, chunk));
+        const uint32_t s = uniform(b.pchunk_scene[chunk]);
         const SceneCfg& sc = b.scenes[s];
         const SceneRun& r = b.runs[s];
         const uint32_t cb = sc.cb, nbx = sc.nbx, nb = nbx * sc.nby;
@@ -1665,11 +1681,13 @@ void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_s
     const uint32_t scene_blocks = at_least1((g.scenes + 255) / 256);
     k_frame_reset<<<scene_blocks, 256, 0, s>>>(b);
     k_mvp<<<at_least1(g.draws), 256, 0, s>>>(b);
+    k_tchunk_map<<<at_least1(std::min<uint32_t>((g.tris + 255) / 256 + 1, 1024u)), 256, 0, s>>>(b);
     k_classify<<<at_least1(std::min<uint32_t>(g.tris, 2048u)), 256, 0, s>>>(b);
     k_clip<<<at_least1((g.clip + CLIP_GROUPS - 1) / CLIP_GROUPS), 64, 0, s>>>(b);
     k_clip_redo<<<64, 64, 0, s>>>(b);
     launch_scan<uint32_t>(s, b.nsub, b.suboff, b.tri_cap, &b.head->ntri, b.parts, &b.cnt->nprims_raw);
     k_scene_prep<<<1, 1024, 0, s>>>(b);
+    k_pchunk_map<<<at_least1(std::min<uint32_t>(g.bins / 256 + 1, 1024u)), 256, 0, s>>>(b);
     k_setup<<<at_least1(g.tris), 256, 0, s>>>(b);
     if (ev_bin) (void)hipEventRecord(ev_bin, s);
     k_bin<false><<<at_least1(g.bins), 256, 0, s>>>(b);
