@@ -750,10 +750,11 @@ __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t
 // fits SETUP_STAGE records they are assembled in LDS and written with
 // contiguous 16-byte stores (whole cache lines) instead of one scattered
 // 112-byte record per thread.
-constexpr uint32_t SETUP_STAGE = 352;
+constexpr uint32_t SETUP_STAGE = 336;
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_setup(JobBufs b) {
     __shared__ PrimGeom stage[SETUP_STAGE];
+    __shared__ uint32_t poff[256], pcls[256];  // per triangle of the chunk: first prim (chunk-relative), class
     const uint32_t NT = b.head->ntri;
     const uint32_t prim_cap = b.prim_cap;
     for (uint32_t blk = blockIdx.x; blk * 256u < NT; blk += gridDim.x) {
@@ -768,60 +769,60 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         // texel indices of this draw's texture fit 32-bit (24-bit factor) arithmetic
         const bool tex_idx32 = dd.tw < (1u << 24) && dd.th < (1u << 24) && (uint64_t)dd.tw * dd.th <= 0xFFFFFFFFull;
         const bool proofs = b.exact_paths == 0;
-        const uint32_t lt = t - dd.tri_base;
-        // the block's prim range (its 256 triangles belong to draw d: draws are 256-aligned)
-        const uint32_t blk_first = uniform(run.prim_base + (b.suboff[t0] - run.prim_off));
-        const uint32_t blk_end = uniform(run.prim_base + (b.suboff[t0 + 255u] + b.nsub[t0 + 255u] - run.prim_off));
-        const bool staged = blk_end - blk_first <= SETUP_STAGE && blk_end <= prim_cap;
+        // the chunk's prim range (its 256 triangles belong to draw d: draws are 256-aligned)
+        const uint32_t s0 = uniform(b.suboff[t0]);
+        const uint32_t blk_first = uniform(run.prim_base + (s0 - run.prim_off));
+        const uint32_t nblk = uniform(b.suboff[t0 + 255u] + b.nsub[t0 + 255u] - s0);
+        const uint32_t blk_end = blk_first + nblk;
+        const bool staged = nblk <= SETUP_STAGE && blk_end <= prim_cap;
+        poff[threadIdx.x] = b.suboff[t] - s0;
+        pcls[threadIdx.x] = (t - dd.tri_base < dd.ntri) ? b.cls[t] : 0u;
+        __syncthreads();
+        // one thread per prim of the chunk: an input triangle (class 1) or one
+        // fan triangle of a clipped polygon (class 2 + slot), so clipped
+        // triangles' fans spread over the workgroup
         bool ovf = false;
-        if (lt < dd.ntri) {
-            const uint32_t c = b.cls[t];
-            const uint32_t base = run.prim_base + (b.suboff[t] - run.prim_off);
+        for (uint32_t pp = threadIdx.x; pp < nblk; pp += 256u) {
+            // the chunk's triangle owning prim pp: the last one starting at or before it
+            uint32_t j = 0;
+            for (uint32_t step = 128; step >= 1; step >>= 1)
+                if (poff[j + step] <= pp) j += step;
+            const uint32_t c = pcls[j], k = pp - poff[j];
+            const uint32_t prim = blk_first + pp;
+            ClipV tv[3];
             if (c == 1) {
-                ClipV tv[3];
-                for (int k = 0; k < 3; ++k) {
-                    uint32_t vi = dd.idx[(size_t)lt * 3 + k];
+                const uint32_t lt = t0 + j - dd.tri_base;
+                for (int q = 0; q < 3; ++q) {
+                    const uint32_t vi = dd.idx[(size_t)lt * 3 + q];
                     float4 p;
-                    load_vertex(dd, vi, p, tv[k].a);
-                    tv[k].x = p.x; tv[k].y = p.y; tv[k].z = p.z; tv[k].w = p.w;
+                    load_vertex(dd, vi, p, tv[q].a);
+                    tv[q].x = p.x; tv[q].y = p.y; tv[q].z = p.z; tv[q].w = p.w;
                 }
-                if (base < prim_cap)
-                    setup_prim(tv, W, H, d, fs, tex_idx32, proofs, staged ? stage + (base - blk_first) : b.geom + base,
-                               b.attrs + (size_t)base * ATTR_STRIDE, b.pbin + base);
-                else
-                    ovf = true;
-            } else if (c >= 2) {
-                const uint32_t slot = c - 2;
-                const uint32_t n = b.nsub[t];
-                if (slot < b.clip_cap) {
-                    const float* poly = b.clip_out + (size_t)slot * MAXV * 10;
-                    for (uint32_t k = 0; k < n; ++k) {
-                        // clipping.rs:185-190: (v0, v_{k+1}, v_{k+2})
-                        ClipV tv[3];
-                        const uint32_t vid[3] = {0u, k + 1, k + 2};
-                        for (int q = 0; q < 3; ++q) {
-                            const float* p = poly + vid[q] * 10;
-                            tv[q].x = p[0]; tv[q].y = p[1]; tv[q].z = p[2]; tv[q].w = p[3];
-                            for (int j = 0; j < 6; ++j) tv[q].a[j] = p[4 + j];
-                        }
-                        if (base + k < prim_cap)
-                            setup_prim(tv, W, H, d, fs, tex_idx32, proofs,
-                                       staged ? stage + (base + k - blk_first) : b.geom + base + k,
-                                       b.attrs + (size_t)(base + k) * ATTR_STRIDE, b.pbin + base + k);
-                        else
-                            ovf = true;
-                    }
+            } else {
+                // clipping.rs:185-190: fan triangle k is (v0, v_{k+1}, v_{k+2})
+                const uint32_t slot = min(c - 2, b.clip_cap - 1);
+                const float* poly = b.clip_out + (size_t)slot * MAXV * 10;
+                const uint32_t vid[3] = {0u, k + 1, k + 2};
+                for (int q = 0; q < 3; ++q) {
+                    const float* pv = poly + min(vid[q], MAXV - 1) * 10;
+                    tv[q].x = pv[0]; tv[q].y = pv[1]; tv[q].z = pv[2]; tv[q].w = pv[3];
+                    for (int i = 0; i < 6; ++i) tv[q].a[i] = pv[4 + i];
                 }
             }
+            if (prim < prim_cap)
+                setup_prim(tv, W, H, d, fs, tex_idx32, proofs, staged ? stage + pp : b.geom + prim,
+                           b.attrs + (size_t)prim * ATTR_STRIDE, b.pbin + prim);
+            else
+                ovf = true;
         }
         __syncthreads();
         if (staged) {
             const uint4* src = reinterpret_cast<const uint4*>(stage);
             uint4* dst = reinterpret_cast<uint4*>(b.geom + blk_first);
-            const uint32_t n16 = (blk_end - blk_first) * (uint32_t)(sizeof(PrimGeom) / sizeof(uint4));
+            const uint32_t n16 = nblk * (uint32_t)(sizeof(PrimGeom) / sizeof(uint4));
             for (uint32_t i = threadIdx.x; i < n16; i += 256u) dst[i] = src[i];
         }
-        __syncthreads();  // the stage is reused by the next block
+        __syncthreads();  // the stage and the chunk tables are reused by the next chunk
         if (__ballot(ovf) && lane_id() == 0) atomicOr(&b.cnt->overflow, OVF_PRIMS);
     }
 }
