@@ -146,6 +146,7 @@ struct rz_ctx {
     DBuf<uint4> tile_rec;
     DBuf<uint2> fchunk_rec;
     DBuf<uint32_t> tchunk_draw, pchunk_scene;
+    DBuf<unsigned long long> rstat_parts;
     DBuf<unsigned long long> fpack, fpre;
     DBuf<uint32_t> fine_counts, fine_off;
     DBuf<uint2> flist;
@@ -370,6 +371,9 @@ int ensure_buffers(rz_ctx* c, const JobPlan& pl, const Needs& n_in) {
     if ((rc = grow(c, c->pbin, c->geom.cap))) return rc;
     if ((rc = grow(c, c->tchunk_draw, c->cls.cap / TRI_ALIGN + 1))) return rc;
     if ((rc = grow(c, c->pchunk_scene, c->geom.cap / CHUNK + 1))) return rc;
+    if (c->raster_stats &&
+        (rc = grow(c, c->rstat_parts, 5ull * ((uint64_t)pl.ntiles + 8 + 32ull * (uint32_t)c->n_cu))))
+        return rc;
     if ((rc = grow(c, c->clip_tri, n.clip))) return rc;
     if ((rc = grow(c, c->clip_redo, c->clip_tri.cap))) return rc;
     if ((rc = grow(c, c->clip_draw, c->clip_tri.cap))) return rc;
@@ -424,6 +428,7 @@ JobBufs job_bufs(const rz_ctx* c) {
     b.fchunk_rec = c->fchunk_rec.p;
     b.tchunk_draw = c->tchunk_draw.p;
     b.pchunk_scene = c->pchunk_scene.p;
+    b.rstat_parts = c->rstat_parts.p;
     b.fpack = c->fpack.p;
     b.fpre = c->fpre.p;
     b.fine_counts = c->fine_counts.p;
@@ -443,6 +448,7 @@ JobBufs job_bufs(const rz_ctx* c) {
     b.fchunk_cap = clamp_u32(c->fchunk_rec.cap);
     b.tchunk_cap = clamp_u32(c->tchunk_draw.cap);
     b.pchunk_cap = clamp_u32(c->pchunk_scene.cap);
+    b.rstat_cap = clamp_u32(c->rstat_parts.cap / 5);
     b.clip_wide = c->clip_wide ? 1u : 0u;
     b.exact_paths = c->exact_paths ? 1u : 0u;
     return b;
@@ -820,6 +826,7 @@ int rz_destroy(rz_ctx* c) {
     release(c->fchunk_rec);
     release(c->tchunk_draw);
     release(c->pchunk_scene);
+    release(c->rstat_parts);
     release(c->fpack);
     release(c->fpre);
     release(c->fine_counts);

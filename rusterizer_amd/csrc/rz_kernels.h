@@ -168,12 +168,13 @@ struct JobBufs {
     uint2* fchunk_rec;  // per fine-binning chunk: {global coarse bin, scene}
     uint32_t* tchunk_draw;   // per TRI_ALIGN-triangle chunk: its draw
     uint32_t* pchunk_scene;  // per CHUNK-prim binning chunk: its scene
+    unsigned long long* rstat_parts;  // stats variant: per raster workgroup, its 5 work counts
     // scan partials
     uint32_t* parts;
     unsigned long long* parts64;
     // capacities (elements)
     uint32_t tri_cap, clip_cap, prim_cap, pair_cap, mat_cap, fine_cap, fmat_cap, bins_cap, tile_cap, fchunk_cap;
-    uint32_t tchunk_cap, pchunk_cap;
+    uint32_t tchunk_cap, pchunk_cap, rstat_cap;  // rstat_cap in workgroups
     uint32_t clip_wide;    // clip every polygon with a full wave (k_clip_redo)
     uint32_t exact_paths;  // setup proves nothing: every prim takes the raster's exact paths
 };

@@ -1647,13 +1647,44 @@ __global__ __launch_bounds__(256) void k_raster(JobBufs b, const PrimGeom* __res
             st_shaded += __shfl_xor(st_shaded, o);
         }
     }
-    if (STATS && (threadIdx.x & 63) == 0) {
-        atomicAdd(&b.cnt->rstat[0], (unsigned long long)st_entries);
-        atomicAdd(&b.cnt->rstat[1], (unsigned long long)st_cand);
-        atomicAdd(&b.cnt->rstat[2], (unsigned long long)st_hits);
-        atomicAdd(&b.cnt->rstat[3], (unsigned long long)st_cov);
-        atomicAdd(&b.cnt->rstat[4], (unsigned long long)st_shaded);
+    if (STATS) {
+        // the workgroup's sums, one plain store each (k_rstat_reduce adds them
+        // up; an atomic per wave on five shared counters serializes)
+        __shared__ unsigned long long wsum[4][5];
+        const uint32_t w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            wsum[w][0] = st_entries;
+            wsum[w][1] = st_cand;
+            wsum[w][2] = st_hits;
+            wsum[w][3] = st_cov;
+            wsum[w][4] = st_shaded;
+        }
+        __syncthreads();
+        if (threadIdx.x < 5) {
+            const unsigned long long v =
+                wsum[0][threadIdx.x] + wsum[1][threadIdx.x] + wsum[2][threadIdx.x] + wsum[3][threadIdx.x];
+            if (blockIdx.x < b.rstat_cap) b.rstat_parts[(size_t)blockIdx.x * 5 + threadIdx.x] = v;
+            else atomicAdd(&b.cnt->rstat[threadIdx.x], v);
+        }
     }
+}
+
+// stats variant: the raster workgroups' counts (nwg workgroups) into the counters
+__global__ __launch_bounds__(256) void k_rstat_reduce(JobBufs b, uint32_t nwg) {
+    __shared__ unsigned long long part[4][5];
+    if (b.cnt->overflow != 0) return;
+    const uint32_t n = min(nwg, b.rstat_cap);
+    unsigned long long s[5] = {0, 0, 0, 0, 0};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        for (int k = 0; k < 5; ++k) s[k] += b.rstat_parts[(size_t)i * 5 + k];
+    for (int k = 0; k < 5; ++k)
+        for (int o = 32; o >= 1; o >>= 1) s[k] += __shfl_xor(s[k], o);
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 5; ++k) part[threadIdx.x >> 6][k] = s[k];
+    __syncthreads();
+    if (threadIdx.x < 5)
+        atomicAdd(&b.cnt->rstat[threadIdx.x], part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
+                                                   part[3][threadIdx.x]);
 }
 
 // largest index of a device index buffer (device-resident mesh upload check)
@@ -1711,6 +1742,7 @@ void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_s
     if (raster_stats) {
         if (g.raster) RZ_RASTER(true, true);
         else RZ_RASTER(true, false);
+        k_rstat_reduce<<<64, 256, 0, s>>>(b, rgrid);
     } else {
         if (g.raster) RZ_RASTER(false, true);
         else RZ_RASTER(false, false);
