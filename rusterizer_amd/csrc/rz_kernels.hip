@@ -1418,7 +1418,7 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
             const float zlb = nzlb;  // the prim's depth lower bound over the quadrant
             const uint64_t box = nbox;
             // pick the next live candidate now, so its record loads while this
-            // prim is processed (judged on the stored depths before this
+            // prim is processed (chosen with the stored depths before this
             // prim's writes: it may turn out to have no live lane)
             const bool more = next_live();
             bool wrote = false;
