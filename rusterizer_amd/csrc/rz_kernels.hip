@@ -204,15 +204,16 @@ __global__ __launch_bounds__(256) void k_frame_reset(JobBufs b) {
 // Sutherland-Hodgman clipper get a slot in the clip list.
 // cls[t]: 0 = Outside (or padding), 1 = Inside, 2 + slot = clipped.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_classify(JobBufs b) {
+__global__ __launch_bounds__(256) void k_classify(JobBufs b, const uint32_t* __restrict__ tchunk_draw,
+                                                  const DrawDev* __restrict__ draws) {
     __shared__ uint32_t wcount[4], wbase[4];
     const uint32_t NT = b.head->ntri;
     const uint32_t wave = threadIdx.x >> 6;
     uint32_t wneed = 0;  // the wave's clipped triangles over all its chunks
     for (uint32_t blk = blockIdx.x; blk * 256u < NT; blk += gridDim.x) {
         const uint32_t t = blk * 256u + threadIdx.x;
-        const uint32_t d = uniform(b.tchunk_draw[blk]);
-        const DrawDev& dd = b.draws[d];
+        const uint32_t d = uniform(tchunk_draw[blk]);
+        const DrawDev& dd = draws[d];
         const uint32_t lt = t - dd.tri_base;
         const bool live = lt < dd.ntri;
         uint32_t c = 0;
@@ -278,7 +279,7 @@ __global__ __launch_bounds__(256) void k_classify(JobBufs b) {
         const bool need = t < NT && b.cls[t] == 2u;
         const uint64_t m = __ballot(need);
         if (m == 0) continue;
-        const uint32_t d = uniform(b.tchunk_draw[blk]);
+        const uint32_t d = uniform(tchunk_draw[blk]);
         if (need) {
             const uint32_t slot = next + mbcnt(m);
             b.cls[t] = 2u + slot;
@@ -752,7 +753,14 @@ __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t
 // 112-byte record per thread.
 constexpr uint32_t SETUP_STAGE = 336;
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_setup(JobBufs b) {
+// The read-only inputs are __restrict__ kernel arguments, so that the
+// compiler may read the uniform ones (draw, scene and chunk records) with
+// scalar loads and keep them across the prim loop despite the record stores.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_setup(
+    JobBufs b, const DrawDev* __restrict__ draws, const SceneCfg* __restrict__ scenes,
+    const SceneRun* __restrict__ runs, const uint32_t* __restrict__ tchunk_draw,
+    const uint32_t* __restrict__ suboff, const uint32_t* __restrict__ nsub, const uint32_t* __restrict__ cls,
+    const float* __restrict__ clip_out) {
     __shared__ PrimGeom stage[SETUP_STAGE];
     __shared__ uint32_t poff[256], pcls[256];  // per triangle of the chunk: first prim (chunk-relative), class
     const uint32_t NT = b.head->ntri;
@@ -760,23 +768,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     for (uint32_t blk = blockIdx.x; blk * 256u < NT; blk += gridDim.x) {
         const uint32_t t0 = blk * 256u;
         const uint32_t t = t0 + threadIdx.x;
-        const uint32_t d = uniform(b.tchunk_draw[blk]);
-        const DrawDev& dd = b.draws[d];
+        const uint32_t d = uniform(tchunk_draw[blk]);
+        const DrawDev& dd = draws[d];
         const uint32_t scene = dd.scene;
-        const SceneCfg& sc = b.scenes[scene];
-        const SceneRun& run = b.runs[scene];
+        const SceneCfg& sc = scenes[scene];
+        const SceneRun& run = runs[scene];
         const uint32_t W = sc.W, H = sc.H, fs = dd.fs;
         // texel indices of this draw's texture fit 32-bit (24-bit factor) arithmetic
         const bool tex_idx32 = dd.tw < (1u << 24) && dd.th < (1u << 24) && (uint64_t)dd.tw * dd.th <= 0xFFFFFFFFull;
         const bool proofs = b.exact_paths == 0;
         // the chunk's prim range (its 256 triangles belong to draw d: draws are 256-aligned)
-        const uint32_t s0 = uniform(b.suboff[t0]);
+        const uint32_t s0 = uniform(suboff[t0]);
         const uint32_t blk_first = uniform(run.prim_base + (s0 - run.prim_off));
-        const uint32_t nblk = uniform(b.suboff[t0 + 255u] + b.nsub[t0 + 255u] - s0);
+        const uint32_t nblk = uniform(suboff[t0 + 255u] + nsub[t0 + 255u] - s0);
         const uint32_t blk_end = blk_first + nblk;
         const bool staged = nblk <= SETUP_STAGE && blk_end <= prim_cap;
-        poff[threadIdx.x] = b.suboff[t] - s0;
-        pcls[threadIdx.x] = (t - dd.tri_base < dd.ntri) ? b.cls[t] : 0u;
+        poff[threadIdx.x] = suboff[t] - s0;
+        pcls[threadIdx.x] = (t - dd.tri_base < dd.ntri) ? cls[t] : 0u;
         __syncthreads();
         // one thread per prim of the chunk: an input triangle (class 1) or one
         // fan triangle of a clipped polygon (class 2 + slot), so clipped
@@ -801,7 +809,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
             } else {
                 // clipping.rs:185-190: fan triangle k is (v0, v_{k+1}, v_{k+2})
                 const uint32_t slot = min(c - 2, b.clip_cap - 1);
-                const float* poly = b.clip_out + (size_t)slot * MAXV * 10;
+                const float* poly = clip_out + (size_t)slot * MAXV * 10;
                 const uint32_t vid[3] = {0u, k + 1, k + 2};
                 for (int q = 0; q < 3; ++q) {
                     const float* pv = poly + min(vid[q], MAXV - 1) * 10;
@@ -845,7 +853,9 @@ constexpr uint32_t BIN_WORDS = CHUNK / 32;
 static_assert(MAX_BINS == 256 && CHUNK == 256, "one thread per bin and per chunk slot");
 
 template <bool SCATTER>
-__global__ __launch_bounds__(256) void k_bin(JobBufs b) {
+__global__ __launch_bounds__(256) void k_bin(JobBufs b, const uint32_t* __restrict__ pchunk_scene,
+                                             const SceneCfg* __restrict__ scenes, const SceneRun* __restrict__ runs,
+                                             const uint4* __restrict__ pbin, const uint32_t* __restrict__ bin_offsets) {
     __shared__ uint32_t bits[BIN_WORDS * MAX_BINS];
     __shared__ uint32_t wbase[SCATTER ? BIN_WORDS * MAX_BINS : 1];
     if (uniform(b.cnt->overflow) != 0) return;
@@ -853,21 +863,24 @@ __global__ __launch_bounds__(256) void k_bin(JobBufs b) {
     const uint32_t t = threadIdx.x, w = t >> 5, below = (1u << (t & 31)) - 1u;
     bool ovf = false;
     for (uint32_t chunk = blockIdx.x; chunk < NC; chunk += gridDim.x) {
-        const uint32_t s = uniform(b.pchunk_scene[chunk]);
-        const SceneCfg& sc = b.scenes[s];
-        const SceneRun& r = b.runs[s];
+        const uint32_t s = uniform(pchunk_scene[chunk]);
+        const SceneCfg& sc = scenes[s];
+        const SceneRun& r = runs[s];
         const uint32_t cb = sc.cb, nbx = sc.nbx, nb = nbx * sc.nby;
         const uint32_t nch = r.nch;
         const size_t mb = (size_t)r.mat_base + (chunk - r.chunk_base) + (size_t)t * nch;
         const uint32_t p = chunk * CHUNK + t;
-        uint4 pb = make_uint4(0u, 0u, 0u, 0u);
-        if (p < r.prim_base + r.nprims) pb = b.pbin[p];
+        // both loads unconditional (clamped to the scene's first prim and the
+        // chunk's first bin, in bounds whenever the chunk exists) so that they
+        // are in flight together; slots past the scene's prims are masked by pv
+        const bool pv = p < r.prim_base + r.nprims;
+        const uint4 pb = pbin[pv ? p : r.prim_base];
         uint32_t run = 0;
-        if (SCATTER && t < nb) run = b.bin_offsets[mb];
+        if (SCATTER) run = bin_offsets[t < nb ? mb : mb - (size_t)t * nch];
         if (t < nb)
             for (uint32_t k = 0; k < BIN_WORDS; ++k) bits[k * MAX_BINS + t] = 0u;
         const uint32_t x0 = pb.y & 0xFFFFu, x1 = pb.y >> 16, y0 = pb.z & 0xFFFFu, y1 = pb.z >> 16;
-        const bool ne = (x0 < x1) && (y0 < y1);
+        const bool ne = pv && (x0 < x1) && (y0 < y1);
         const uint32_t cx0 = x0 / cb, cx1 = ne ? (x1 - 1) / cb : 0u;
         const uint32_t cy0 = y0 / cb, cy1 = ne ? (y1 - 1) / cb : 0u;
         uint32_t* row = bits + w * MAX_BINS;
@@ -972,7 +985,11 @@ __global__ __launch_bounds__(256) void k_fchunk_map(JobBufs b) {
 }
 
 template <bool SCATTER>
-__global__ __launch_bounds__(256) void k_fine_bin(JobBufs b) {
+__global__ __launch_bounds__(256) void k_fine_bin(JobBufs b, const uint2* __restrict__ fchunk_rec,
+                                                  const SceneCfg* __restrict__ scenes,
+                                                  const unsigned long long* __restrict__ fpre,
+                                                  const uint2* __restrict__ bin_rng, const uint4* __restrict__ list,
+                                                  const uint32_t* __restrict__ fine_off) {
     // one workgroup per fine-binning chunk of CHUNK list entries of one coarse
     // bin, thread t holding entry t; the same LDS bit matrix as k_bin, with
     // the bin's local tiles in place of the scene's bins
@@ -987,13 +1004,13 @@ __global__ __launch_bounds__(256) void k_fine_bin(JobBufs b) {
     const uint32_t t = threadIdx.x, w = t >> 5, below = (1u << (t & 31)) - 1u;
     bool ovf = false;
     for (uint32_t gc = blockIdx.x; gc < NCH; gc += gridDim.x) {
-        const uint2 fr = b.fchunk_rec[gc];
+        const uint2 fr = fchunk_rec[gc];
         const uint32_t g = uniform(fr.x), s = uniform(fr.y);
-        const SceneCfg& sc = b.scenes[s];
+        const SceneCfg& sc = scenes[s];
         const uint32_t cb = sc.cb, TS = cb / 16, T = TS * TS;
         if (T > MAX_BINS) continue;  // the host never configures more tiles per coarse bin
-        const unsigned long long fp = b.fpre[g];
-        const uint2 rng = b.bin_rng[g];
+        const unsigned long long fp = fpre[g];
+        const uint2 rng = bin_rng[g];
         const uint32_t c = gc - lo32(fp);
         const uint32_t nchb = (rng.y - rng.x + CHUNK - 1) / CHUNK;
         const uint32_t bi = g - sc.bin_base;
@@ -1001,10 +1018,12 @@ __global__ __launch_bounds__(256) void k_fine_bin(JobBufs b) {
         const size_t mt = (size_t)hi32(fp) + c + (size_t)t * nchb;
         const uint32_t e = rng.x + c * CHUNK + t;
         const bool valid = e < rng.y;
-        uint4 en = make_uint4(0u, 0u, 0u, 0u);
-        if (valid) en = b.list[e];
+        // both loads unconditional (clamped to the chunk's first entry and
+        // first tile, in bounds whenever the chunk exists) so that they are in
+        // flight together; every use of en below is guarded by valid
+        const uint4 en = list[valid ? e : rng.x];
         uint32_t run = 0;
-        if (SCATTER && t < T) run = b.fine_off[mt];
+        if (SCATTER) run = fine_off[t < T ? mt : (size_t)hi32(fp) + c];
         if (t < T)
             for (uint32_t k = 0; k < BIN_WORDS; ++k) bits[k * MAX_BINS + t] = 0u;
         const uint32_t x0 = en.y & 0xFFFFu, x1 = en.y >> 16, y0 = en.z & 0xFFFFu, y1 = en.z >> 16;
@@ -1714,26 +1733,29 @@ void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_s
     k_frame_reset<<<scene_blocks, 256, 0, s>>>(b);
     k_mvp<<<at_least1(g.draws), 256, 0, s>>>(b);
     k_tchunk_map<<<at_least1(std::min<uint32_t>((g.tris + 255) / 256 + 1, 1024u)), 256, 0, s>>>(b);
-    k_classify<<<at_least1(std::min<uint32_t>(g.tris, 2048u)), 256, 0, s>>>(b);
+    k_classify<<<at_least1(std::min<uint32_t>(g.tris, 2048u)), 256, 0, s>>>(b, b.tchunk_draw, b.draws);
     k_clip<<<at_least1((g.clip + CLIP_GROUPS - 1) / CLIP_GROUPS), 64, 0, s>>>(b);
     k_clip_redo<<<64, 64, 0, s>>>(b);
     launch_scan<uint32_t>(s, b.nsub, b.suboff, b.tri_cap, &b.head->ntri, b.parts, &b.cnt->nprims_raw);
     k_scene_prep<<<1, 1024, 0, s>>>(b);
     k_pchunk_map<<<at_least1(std::min<uint32_t>(g.bins / 256 + 1, 1024u)), 256, 0, s>>>(b);
-    k_setup<<<at_least1(g.tris), 256, 0, s>>>(b);
+    k_setup<<<at_least1(g.tris), 256, 0, s>>>(b, b.draws, b.scenes, b.runs, b.tchunk_draw, b.suboff, b.nsub, b.cls,
+                                              b.clip_out);
     if (ev_bin) (void)hipEventRecord(ev_bin, s);
-    k_bin<false><<<at_least1(g.bins), 256, 0, s>>>(b);
+    k_bin<false><<<at_least1(g.bins), 256, 0, s>>>(b, b.pchunk_scene, b.scenes, b.runs, b.pbin, b.bin_offsets);
     launch_scan<uint32_t>(s, b.bin_counts, b.bin_offsets, b.mat_cap, &b.cnt->mat_len, b.parts,
                           &b.cnt->pairs_binned);
-    k_bin<true><<<at_least1(g.bins), 256, 0, s>>>(b);
+    k_bin<true><<<at_least1(g.bins), 256, 0, s>>>(b, b.pchunk_scene, b.scenes, b.runs, b.pbin, b.bin_offsets);
     k_fine_prep<<<at_least1(std::min<uint32_t>((b.bins_cap + 255) / 256, 1024u)), 256, 0, s>>>(b);
     // fine_chunks and fmat_len are adjacent: the u64 total fills both
     launch_scan<unsigned long long>(s, b.fpack, b.fpre, b.bins_cap, &b.head->nbins, b.parts64,
                                     reinterpret_cast<unsigned long long*>(&b.cnt->fine_chunks));
     k_fchunk_map<<<at_least1(std::min<uint32_t>((b.bins_cap + 255) / 256, 1024u)), 256, 0, s>>>(b);
-    k_fine_bin<false><<<at_least1(g.fine), 256, 0, s>>>(b);
+    k_fine_bin<false><<<at_least1(g.fine), 256, 0, s>>>(b, b.fchunk_rec, b.scenes, b.fpre, b.bin_rng, b.list,
+                                                 b.fine_off);
     launch_scan<uint32_t>(s, b.fine_counts, b.fine_off, b.fmat_cap, &b.cnt->fmat_len, b.parts, &b.cnt->fine_total);
-    k_fine_bin<true><<<at_least1(g.fine), 256, 0, s>>>(b);
+    k_fine_bin<true><<<at_least1(g.fine), 256, 0, s>>>(b, b.fchunk_rec, b.scenes, b.fpre, b.bin_rng, b.list,
+                                                b.fine_off);
     k_tile_prep<<<at_least1(std::min<uint32_t>((g.ntiles + 255) / 256, 4096u)), 256, 0, s>>>(b);
     if (ev_raster) (void)hipEventRecord(ev_raster, s);
     const uint32_t rgrid = g.raster ? g.raster : at_least1((g.ntiles + 7) / 8 * 8);
