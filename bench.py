@@ -517,7 +517,8 @@ def main():
 
         serve = bench_serve({"batch": scs, "soup": [soup_sc], "terrain": [terr_sc]}, b["tuning"], a.serve_rate,
                             a.serve_requests, S.SEEDS["serve"])
-        log(f"serve: p50 {serve['latency_ms']['p50']:.2f} ms, p99 {serve['latency_ms']['p99']:.2f} ms")
+        log(f"serve: p50 {serve['latency_ms']['p50']:.2f} ms, p99 {serve['latency_ms']['p99']:.2f} ms; by kind " +
+            ", ".join(f"{k} p50 {v['p50']:.2f} p99 {v['p99']:.2f}" for k, v in serve["latency_ms_by_kind"].items()))
     line = {
         "metric": METRIC, "value": b["samples_per_s"], "unit": "MSAA samples/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": b["ms_per_batch"], "higher_is_better": True,

@@ -280,6 +280,12 @@ class Server:
             "peak_admitted_bytes": peak_bytes,
             "resident_bytes": self.catalog.resident_bytes,
         }
+        by_kind = collections.defaultdict(list)
+        for r, l in zip(served, lat):
+            by_kind[cat[r.tmpl].kind].append(l)
+        rep["latency_ms_by_kind"] = {
+            k: {"n": len(v), "p50": float(np.percentile(v, 50) * 1e3), "p99": float(np.percentile(v, 99) * 1e3)}
+            for k, v in sorted(by_kind.items())}
         if copy_back:
             d2h = np.array([r.d2h_s for r in served])
             nbytes = sum(4 * cat[r.tmpl].pixels for r in served)

@@ -38,6 +38,21 @@ def test_batch_scenes_match_oracle():
         assert_parity(r, cpu, where=f"{sc.name} {sc.width}x{sc.height}")
 
 
+def test_batch_full_size_workload():
+    """The full batched workload (256 textured soups, 9,203,286 triangles,
+    127,795,200 px, its own seed) rendered as one job at its real buffer
+    sizes; every 8th scene and the scene with the most triangles are checked
+    against the oracle."""
+    scs = scenes.batch_scenes()
+    res, stats = run_batch(scs)
+    assert len(scs) == 256 and stats["tris"] == sum(s.ntri for s in scs) == 9_203_286
+    assert sum(s.width * s.height for s in scs) == 127_795_200
+    pick = sorted(set(range(0, len(scs), 8)) | {max(range(len(scs)), key=lambda i: scs[i].ntri)})
+    for i in pick:
+        sc = scs[i]
+        assert_parity(res[i], run_oracle(sc), where=f"batch scene {i} {sc.width}x{sc.height} {sc.ntri} tris")
+
+
 def test_batch_multi_draw_scene_and_empty_scene():
     a = scenes.soup_scene(3000, 200, 150, seed=81)
     b = scenes.soup_scene(2500, 200, 150, seed=82, textured=True)

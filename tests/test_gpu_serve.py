@@ -62,6 +62,9 @@ def test_serving_mix_equals_alone():
             srv.close()
         assert rep["requests"] == 400 and rep["rejected"] == 0
         assert set(rep["kinds"]) == {"batch", "soup", "terrain"}
+        by_kind = rep["latency_ms_by_kind"]
+        assert {k: v["n"] for k, v in by_kind.items()} == rep["kinds"]
+        assert all(0.0 < v["p50"] <= v["p99"] for v in by_kind.values())
         assert rep["batches"] >= 400 // 12
         assert rep["max_batches_in_flight"] >= 2  # batches overlapped on the device
         mixed = {}
