@@ -765,10 +765,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     __shared__ uint32_t poff[256], pcls[256];  // per triangle of the chunk: first prim (chunk-relative), class
     const uint32_t NT = b.head->ntri;
     const uint32_t prim_cap = b.prim_cap;
+    // the chunk table words of the workgroup's next chunk are loaded one
+    // chunk ahead (NT is a multiple of 256, so t0 + 255 is in range)
+    uint32_t n_d = 0, n_s0 = 0, n_end = 0, n_sub = 0, n_cls = 0;
+    auto fetch = [&](uint32_t c) {
+        if (c * 256u < NT) {
+            const uint32_t c0 = c * 256u;
+            n_d = tchunk_draw[c];
+            n_s0 = suboff[c0];
+            n_end = suboff[c0 + 255u] + nsub[c0 + 255u];
+            n_sub = suboff[c0 + threadIdx.x];
+            n_cls = cls[c0 + threadIdx.x];
+        }
+    };
+    fetch(blockIdx.x);
     for (uint32_t blk = blockIdx.x; blk * 256u < NT; blk += gridDim.x) {
         const uint32_t t0 = blk * 256u;
         const uint32_t t = t0 + threadIdx.x;
-        const uint32_t d = uniform(tchunk_draw[blk]);
+        const uint32_t d = uniform(n_d), s0 = uniform(n_s0), s_end = uniform(n_end), my_sub = n_sub, my_cls = n_cls;
+        fetch(blk + gridDim.x);
         const DrawDev& dd = draws[d];
         const uint32_t scene = dd.scene;
         const SceneCfg& sc = scenes[scene];
@@ -778,13 +793,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         const bool tex_idx32 = dd.tw < (1u << 24) && dd.th < (1u << 24) && (uint64_t)dd.tw * dd.th <= 0xFFFFFFFFull;
         const bool proofs = b.exact_paths == 0;
         // the chunk's prim range (its 256 triangles belong to draw d: draws are 256-aligned)
-        const uint32_t s0 = uniform(suboff[t0]);
         const uint32_t blk_first = uniform(run.prim_base + (s0 - run.prim_off));
-        const uint32_t nblk = uniform(suboff[t0 + 255u] + nsub[t0 + 255u] - s0);
+        const uint32_t nblk = s_end - s0;
         const uint32_t blk_end = blk_first + nblk;
         const bool staged = nblk <= SETUP_STAGE && blk_end <= prim_cap;
-        poff[threadIdx.x] = suboff[t] - s0;
-        pcls[threadIdx.x] = (t - dd.tri_base < dd.ntri) ? cls[t] : 0u;
+        poff[threadIdx.x] = my_sub - s0;
+        pcls[threadIdx.x] = (t - dd.tri_base < dd.ntri) ? my_cls : 0u;
         __syncthreads();
         // one thread per prim of the chunk: an input triangle (class 1) or one
         // fan triangle of a clipped polygon (class 2 + slot), so clipped
