@@ -82,6 +82,15 @@ int rz_destroy(rz_ctx *ctx);
 const char *rz_last_error(const rz_ctx *ctx);
 /* The HIP stream (hipStream_t) all work of ctx runs on. */
 void *rz_stream(const rz_ctx *ctx);
+/* Restricts the CUs ctx's work may run on (no reference counterpart: lets a
+ * server keep some CUs free of long jobs, so that short jobs on another
+ * context do not wait behind their workgroups). mask holds `words` 32-bit
+ * words, bit i of word w selecting CU 32*w + i, as for
+ * hipExtStreamCreateWithCUMask; words = 0 lifts the restriction. ctx gets a
+ * new stream (rz_stream changes), so it must be idle: no frame or batch
+ * recorded or in flight (RZ_ERR_STATE). RZ_ERR_ARG if the mask selects no CU
+ * of the device. Results do not depend on the mask. */
+int rz_set_cu_mask(rz_ctx *ctx, const uint32_t *mask, uint32_t words);
 
 /* Depth the MSAA buffers are cleared to (CLEAR_DEPTH, src/rasterizer/buffers.rs:6;
  * default FLT_MAX). Takes effect from the next frame. */

@@ -45,6 +45,7 @@ SIGNATURES = {
     "rz_destroy": (C.c_int, [_vp]),
     "rz_last_error": (C.c_char_p, [_vp]),
     "rz_stream": (_vp, [_vp]),
+    "rz_set_cu_mask": (C.c_int, [_vp, _u32, C.c_uint32]),
     "rz_set_clear_depth": (C.c_int, [_vp, C.c_float]),
     "rz_set_uniforms": (C.c_int, [_vp, _f, _f, _f]),
     "rz_bind_texture": (C.c_int, [_vp, C.c_uint32, _u8, C.c_uint32, C.c_uint32]),

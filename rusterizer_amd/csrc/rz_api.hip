@@ -1142,6 +1142,27 @@ int rz_batch_tuning(rz_ctx* c, uint32_t bin_target, uint32_t raster_per_cu, int 
     return RZ_OK;
 }
 
+int rz_set_cu_mask(rz_ctx* c, const uint32_t* mask, uint32_t words) {
+    if (!c) return RZ_ERR_ARG;
+    if (words > 0 && !mask) return fail(c, RZ_ERR_ARG, "null mask");
+    if (words > 1024) return fail(c, RZ_ERR_ARG, "mask longer than 1024 words");
+    if (c->recording != JobKind::None || c->in_flight || !c->draws.empty())
+        return fail(c, RZ_ERR_STATE, "not between jobs");
+    if (words > 0) {
+        bool any = false;
+        for (uint32_t i = 0; i < (uint32_t)c->n_cu && i / 32 < words; ++i) any |= ((mask[i / 32] >> (i % 32)) & 1u) != 0;
+        if (!any) return fail(c, RZ_ERR_ARG, "the mask selects no CU of the device");
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    hipStream_t ns = nullptr;
+    const hipError_t e = words > 0 ? hipExtStreamCreateWithCUMask(&ns, words, mask)
+                                   : hipStreamCreateWithFlags(&ns, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(c, RZ_ERR_HIP, std::string("stream creation: ") + hipGetErrorString(e));
+    (void)hipStreamDestroy(c->stream);
+    c->stream = ns;
+    return RZ_OK;
+}
+
 int rz_batch_get_tuning(const rz_ctx* c, uint32_t out[3]) {
     if (!c || !out) return RZ_ERR_ARG;
     out[0] = c->bin_target;

@@ -98,6 +98,19 @@ class Context:
     def stream(self) -> int:
         return int(self._L.rz_stream(self.handle) or 0)
 
+    def set_cu_mask(self, cus=None):
+        """Restrict this context's work to the given CU indices (None: all
+        CUs); the context must be idle. Replaces its stream."""
+        if cus is None:
+            self._check(self._L.rz_set_cu_mask(self.handle, None, 0))
+            return
+        cus = [int(i) for i in cus]
+        words = (max(cus) // 32 + 1) if cus else 1
+        m = (C.c_uint32 * words)()
+        for i in cus:
+            m[i // 32] |= 1 << (i % 32)
+        self._check(self._L.rz_set_cu_mask(self.handle, m, words))
+
     def set_clear_depth(self, depth: float):
         self._check(self._L.rz_set_clear_depth(self.handle, float(depth)))
 

@@ -138,12 +138,28 @@ class Server:
     """Continuous batching over `slots` contexts of the catalog's group."""
 
     def __init__(self, catalog: Catalog, slots: int = 3, budget_bytes: float = 200e9, max_batch: int = 64,
-                 max_batch_tris: int = 12_000_000, tuning: dict | None = None):
+                 max_batch_tris: int = 12_000_000, tuning: dict | None = None, small_cus: int = 0,
+                 small_tris: int = 250_000, small_px: int = 1 << 20):
+        """small_cus > 0 (with slots >= 2): slot 0 serves only small requests
+        (at most small_tris triangles and small_px pixels) on CUs
+        [0, small_cus); the other slots serve any request on the remaining
+        CUs, so a small request never waits behind a long job's workgroups."""
         self.catalog = catalog
         self.ctxs = [Context(64, 64, share=catalog.root) for _ in range(slots)]
         for c in self.ctxs:
             if tuning:
                 c.batch_tuning(**tuning)
+        self.small_slot = small_cus > 0 and slots >= 2
+        self.small_tris, self.small_px = small_tris, small_px
+        if self.small_slot:
+            import torch
+
+            n_cu = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
+            if not 0 < small_cus < n_cu:
+                raise ValueError(f"small_cus must be in (0, {n_cu})")
+            self.ctxs[0].set_cu_mask(range(small_cus))
+            for c in self.ctxs[1:]:
+                c.set_cu_mask(range(small_cus, n_cu))
         self.budget = budget_bytes
         self.max_batch = max_batch
         self.max_batch_tris = max_batch_tris
@@ -153,6 +169,9 @@ class Server:
         for c in self.ctxs:
             c.close()
         self.ctxs = []
+
+    def is_small(self, tmpl: Template) -> bool:
+        return tmpl.scene.ntri <= self.small_tris and tmpl.pixels <= self.small_px
 
     def est_bytes(self, tmpl: Template) -> int:
         return tmpl.pixels * (4 + BYTES_PER_PX) + tmpl.scene.ntri * BYTES_PER_TRI
@@ -200,11 +219,14 @@ class Server:
                     busy[s] = None
                 else:
                     nbusy += 1
-            # admission: fill every free slot from the queue, oldest first
+            # admission: fill every free slot from the queue, oldest first (the
+            # small-request slot passes over large requests, leaving them queued)
             for s, ctx in enumerate(self.ctxs):
                 if busy[s] is not None or not queue:
                     continue
+                small_only = self.small_slot and s == 0
                 take: list[Request] = []
+                skipped: list[Request] = []
                 tris = 0
                 while queue and len(take) < self.max_batch:
                     r = queue[0]
@@ -215,6 +237,9 @@ class Server:
                         r.rejected = True
                         done += 1
                         continue
+                    if small_only and not self.is_small(t):
+                        skipped.append(queue.popleft())
+                        continue
                     if take and tris + t.scene.ntri > self.max_batch_tris:
                         break
                     if in_flight_bytes + need > self.budget:
@@ -224,6 +249,7 @@ class Server:
                     in_flight_bytes += need
                     tris += t.scene.ntri
                     take.append(r)
+                queue.extendleft(reversed(skipped))
                 if not take:
                     continue
                 peak_bytes = max(peak_bytes, in_flight_bytes)
@@ -274,6 +300,7 @@ class Server:
             "batches": len(batches),
             "batch_size": {"mean": float(np.mean(batches)), "max": int(max(batches))},
             "slots": len(self.ctxs),
+            "small_slot": self.small_slot,
             "max_batches_in_flight": max_concurrent,
             "overlap_share": overlap_polls / max(busy_polls, 1),
             "budget_bytes": self.budget,

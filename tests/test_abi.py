@@ -33,6 +33,7 @@ def test_null_context_errors():
     assert L.rz_resolve(None, None) < 0
     assert L.rz_set_uniforms(None, None, None, None) < 0
     assert L.rz_frame_flags(None) == 0
+    assert L.rz_set_cu_mask(None, None, 0) < 0
     h = ctypes.c_void_p()
     assert L.rz_create(0, 10, ctypes.byref(h)) < 0
     assert L.rz_create(20000, 10, ctypes.byref(h)) < 0

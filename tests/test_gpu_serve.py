@@ -110,3 +110,64 @@ def test_serving_budget_admission():
     finally:
         cat.destroy()
         root.close()
+
+
+def test_cu_mask_same_results_and_errors():
+    """A context restricted to a few CUs renders the same frame; the mask is
+    refused while a frame is recorded and when it selects no CU."""
+    import torch
+
+    from harness import run_gpu
+    from rusterizer_amd import scenes
+    from rusterizer_amd.render import Context, RzError
+
+    sc = scenes.soup_scene(20000, 300, 200, seed=91)  # colour FS: the context binds no texture
+    ref = run_gpu(sc)
+    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+    ctx = Context(sc.width, sc.height)
+    try:
+        s0 = ctx.stream
+        ctx.set_cu_mask(range(n_cu - 8, n_cu))
+        assert ctx.stream != s0
+        got = run_gpu(sc, ctx=ctx)
+        assert np.array_equal(got["img"], ref["img"]) and np.array_equal(got["dep"], ref["dep"])
+        with pytest.raises(RzError):
+            ctx.set_cu_mask([n_cu + 40])  # no CU of the device
+        m = ctx.mesh(sc.pos, sc.color, sc.uv, sc.idx)
+        ctx.draw(m, sc.fs)
+        with pytest.raises(RzError):
+            ctx.set_cu_mask(None)  # a frame is being recorded
+        ctx.resolve()
+        ctx.set_cu_mask(None)
+        assert np.array_equal(run_gpu(sc, ctx=ctx)["img"], ref["img"])
+        m.destroy()
+    finally:
+        ctx.close()
+
+
+def test_serving_small_slot_on_reserved_cus():
+    """With CUs reserved for small requests, every request still equals its
+    image rendered alone, and no large request runs on the small slot."""
+    import torch
+
+    from rusterizer_amd.serve import Server, make_requests
+
+    root, cat = _catalog()
+    try:
+        alone = _alone(cat)
+        reqs = make_requests(cat, 300, rate=40_000.0, weights={"batch": 0.8, "soup": 0.1, "terrain": 0.1}, seed=11)
+        small_tris = max(t.scene.ntri for t in cat.templates if t.kind == "batch")
+        srv = Server(cat, slots=3, max_batch=12, small_cus=32, small_tris=small_tris)
+        try:
+            rep = srv.run(reqs, keep_outputs=True)
+            for r in reqs:
+                assert torch.equal(r.out, alone[r.tmpl]), (r.rid, cat.templates[r.tmpl].kind)
+                if not srv.is_small(cat.templates[r.tmpl]):
+                    assert r.slot != 0, r.rid
+        finally:
+            srv.close()
+        assert rep["requests"] == 300 and rep["small_slot"]
+        assert any(r.slot == 0 for r in reqs)
+    finally:
+        cat.destroy()
+        root.close()
