@@ -1361,12 +1361,16 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
     const float qxl = (float)qx0 + 0.125f, qxh = (float)qx0 + 7.875f;
     const float qyl = (float)qy0 + 0.125f, qyh = (float)qy0 + 7.875f;
 
-    // per sample: stored depth and the prim that wrote it last (NO_PRIM:
-    // cleared); per sample nibble of pm: that prim's passing samples when it
-    // wrote, which fixes where it is shaded
+    // per sample: stored depth (registers), and in LDS the prim that wrote
+    // it last (NO_PRIM: cleared) with that prim's passing samples when it
+    // wrote, which fix where it is shaded. The winners are written during the
+    // list walk and read only after it, so they live in this wave's
+    // shading-task row (sample k of the lane at k * 64 + lane) rather than
+    // in registers, which keeps the walk's register count down.
     float zb0 = clear_depth, zb1 = clear_depth, zb2 = clear_depth, zb3 = clear_depth;
-    uint32_t pr0 = NO_PRIM, pr1 = NO_PRIM, pr2 = NO_PRIM, pr3 = NO_PRIM;
-    uint32_t pm = 0;
+    uint2* const win = sh.task[wave];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) win[k * 64 + lane] = make_uint2(NO_PRIM, 0u);
     bool oob = false;
     // qzmax: largest depth stored in the quadrant's on-screen samples
     float qzmax = __int_as_float(
@@ -1534,13 +1538,11 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
                         // mod.rs:380-397 writes the fragment's colour and depth to its
                         // passing samples; the colour is computed after the list, once
                         // per surviving (prim, pixel), from the same inputs
-                        if (pass & 1u) { pr0 = prim; zb0 = sd[0]; }
-                        if (pass & 2u) { pr1 = prim; zb1 = sd[1]; }
-                        if (pass & 4u) { pr2 = prim; zb2 = sd[2]; }
-                        if (pass & 8u) { pr3 = prim; zb3 = sd[3]; }
-                        const uint32_t sel = ((pass & 1u) ? 0xFu : 0u) | ((pass & 2u) ? 0xF0u : 0u) |
-                                             ((pass & 4u) ? 0xF00u : 0u) | ((pass & 8u) ? 0xF000u : 0u);
-                        pm = (pm & ~sel) | ((pass * 0x1111u) & sel);
+                        const uint2 wv = make_uint2(prim, pass);
+                        if (pass & 1u) { win[lane] = wv; zb0 = sd[0]; }
+                        if (pass & 2u) { win[64 + lane] = wv; zb1 = sd[1]; }
+                        if (pass & 4u) { win[128 + lane] = wv; zb2 = sd[2]; }
+                        if (pass & 8u) { win[192 + lane] = wv; zb3 = sd[3]; }
                         lane_zbmax = fmaxf(fmaxf(zb0, zb1), fmaxf(zb2, zb3));
                         wrote = true;
                     }
@@ -1561,7 +1563,11 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
     }
     // deferred shading: one task per distinct surviving prim of a pixel (its
     // first sample heads it); the workgroup's tasks are shaded 64 at a time
-    // by all four waves, then each pixel gathers its samples' colours
+    // by all four waves, then each pixel gathers its samples' colours. The
+    // winners are read back before the task row they share is rewritten.
+    const uint2 w0 = win[lane], w1 = win[64 + lane], w2 = win[128 + lane], w3 = win[192 + lane];
+    const uint32_t pr0 = w0.x, pr1 = w1.x, pr2 = w2.x, pr3 = w3.x;
+    const uint32_t pm = w0.y | w1.y << 4 | w2.y << 8 | w3.y << 12;
     const bool h1 = pr1 != pr0, h2 = (pr2 != pr0) & (pr2 != pr1), h3 = (pr3 != pr0) & (pr3 != pr1) & (pr3 != pr2);
     const bool v0 = pr0 != NO_PRIM, v1 = pr1 != NO_PRIM, v2 = pr2 != NO_PRIM, v3 = pr3 != NO_PRIM;
     const bool hd0 = v0, hd1 = v1 & h1, hd2 = v2 & h2, hd3 = v3 & h3;
@@ -1626,12 +1632,13 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
     }
 }
 
+// The production variant is held to 64 VGPRs (8 waves per SIMD): its few
+// spills are values kept across the list walk, stored once per tile, and the
+// extra waves hide more of the walk's load latency than they cost.
 template <bool STATS, bool PERSIST>
-__global__ __launch_bounds__(256) void k_raster(JobBufs b, const PrimGeom* __restrict__ geom,
-                                                const float* __restrict__ attrs, const uint2* __restrict__ flist,
-                                                const uint4* __restrict__ tile_rec,
-                                                const SceneCfg* __restrict__ scenes,
-                                                const DrawDev* __restrict__ draws) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STATS || PERSIST ? 1 : 8, 8))) void k_raster(
+    JobBufs b, const PrimGeom* __restrict__ geom, const float* __restrict__ attrs, const uint2* __restrict__ flist,
+    const uint4* __restrict__ tile_rec, const SceneCfg* __restrict__ scenes, const DrawDev* __restrict__ draws) {
     const RasterIn in{geom, attrs, flist, tile_rec, scenes, draws};
     __shared__ float lut[256];
     __shared__ uint32_t next_tile;
