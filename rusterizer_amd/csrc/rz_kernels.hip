@@ -1557,9 +1557,17 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
             qzmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(wave_max(lane_zbmax))));
         }
     }
-    if (onscreen && sc.msaa_depth) {
-        const float4 d4 = make_float4(zb0, zb1, zb2, zb3);
-        *reinterpret_cast<float4*>(sc.msaa_depth + ((size_t)py * W + px) * 4) = d4;
+    {
+        // the pixel's coordinates are recomputed here and below rather than
+        // held across the walk and the shading (registers are the raster's
+        // occupancy limit)
+        uint32_t l2 = lane;
+        asm volatile("" : "+v"(l2));
+        const uint32_t px2 = qx0 + (l2 & 7), py2 = qy0 + (l2 >> 3);
+        if (px2 < W && py2 < H && sc.msaa_depth) {
+            const float4 d4 = make_float4(zb0, zb1, zb2, zb3);
+            *reinterpret_cast<float4*>(sc.msaa_depth + ((size_t)py2 * W + px2) * 4) = d4;
+        }
     }
     // deferred shading: one task per distinct surviving prim of a pixel (its
     // first sample heads it); the workgroup's tasks are shaded 64 at a time
@@ -1616,14 +1624,17 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
     const uint32_t cb2 = (vmask & 4u) ? wargb[(slots >> 16) & 0xFFu] : CLEAR_COLOR;
     const uint32_t cb3 = (vmask & 8u) ? wargb[slots >> 24] : CLEAR_COLOR;
     if (__ballot(oob) && lane == 0) atomicOr(&b.runs[s].flags, RZ_FLAG_TEXEL_OOB);
-    if (onscreen) {
+    uint32_t l3 = lane;
+    asm volatile("" : "+v"(l3));
+    const uint32_t px3 = qx0 + (l3 & 7), py3 = qy0 + (l3 >> 3);
+    if (px3 < W && py3 < H) {
         // buffers.rs:111-125
         uint32_t rs = ((cb0 & 0x00FF0000u) >> 16) + ((cb1 & 0x00FF0000u) >> 16) + ((cb2 & 0x00FF0000u) >> 16) +
                       ((cb3 & 0x00FF0000u) >> 16);
         uint32_t gs = ((cb0 & 0x0000FF00u) >> 8) + ((cb1 & 0x0000FF00u) >> 8) + ((cb2 & 0x0000FF00u) >> 8) +
                       ((cb3 & 0x0000FF00u) >> 8);
         uint32_t bs = (cb0 & 0xFFu) + (cb1 & 0xFFu) + (cb2 & 0xFFu) + (cb3 & 0xFFu);
-        size_t o = (size_t)py * W + px;
+        size_t o = (size_t)py3 * W + px3;
         sc.out[o] = (0xFFu << 24) | (rs / 4u) << 16 | (gs / 4u) << 8 | (bs / 4u);
         if (sc.msaa_color) {
             uint4 c4 = make_uint4(cb0, cb1, cb2, cb3);
