@@ -349,6 +349,9 @@ __device__ __forceinline__ void clip_polygon(const JobBufs& b, uint32_t slot, bo
             case 4: dcur = v.w + v.z; break;
             default: dcur = v.w - v.z; break;
         }
+        // a plane with every vertex of every polygon of the wave inside it
+        // leaves the polygons unchanged (each edge emits its current vertex)
+        if (__ballot(act && !(dcur >= 0.0f)) == 0) continue;
         const int prev = live ? (int)gb + (gl == 0 ? (int)n - 1 : (int)gl - 1) : (int)lane;
         ClipV pv;
         pv.x = __shfl(v.x, prev);
