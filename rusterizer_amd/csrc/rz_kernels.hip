@@ -1274,7 +1274,7 @@ __device__ __forceinline__ bool texel_oob(uint32_t tw, uint32_t th, float tu, fl
     const uint32_t yf0 = sat_u32(floorf(y)), yc = sat_u32(ceilf(y));
     const uint64_t n = (uint64_t)tw * th;
     auto past = [&](uint32_t X, uint32_t Y) { return Y >= th || (uint64_t)X >= n - (uint64_t)Y * tw; };
-    return past(xf0, yf0) | past(xc, yf0) | past(xf0, yc) | past(xc, yc);
+    return past(xf0, yf0) || past(xc, yf0) || past(xf0, yc) || past(xc, yc);
 }
 
 // FragCoords.depths[0] (mod.rs:225-262): the prim's depth at sample 0 when
