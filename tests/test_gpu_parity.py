@@ -306,3 +306,39 @@ def test_tie_rule_on_shared_edges():
         assert covered.mean() > 0.5
         return
     raise AssertionError("no winding of the lattice was front-facing")
+
+
+def _nonfinite_scene(fs):
+    """Triangles with NaN, +-inf and 1e30 positions, a vertex on the camera
+    plane (w = 0), NaN colour and NaN / inf / out-of-range uv beside ordinary
+    ones: whatever the reference's arithmetic makes of them (rejects, clips,
+    empty boxes, NaN depths that never pass LESS), the GPU must make the same."""
+    nan, inf = F(np.nan), F(np.inf)
+    tris = [
+        [[-0.5, -0.5, 2], [0.5, -0.5, 2], [0.0, 0.5, 2]],
+        [[nan, -0.5, 2], [0.5, -0.4, 2], [0.1, 0.5, 2]],
+        [[-0.6, -0.6, 2], [inf, -0.5, 2], [0.0, 0.6, 2]],
+        [[-0.6, 0.6, 2], [-inf, -0.5, 2], [0.2, 0.4, 2]],
+        [[-1e30, -0.4, 2], [1e30, -0.4, 2], [0.0, 1e30, 2]],
+        [[-0.3, -0.3, 0], [0.3, -0.3, 2], [0.0, 0.3, 2]],
+        [[-0.4, -0.2, 1.5], [0.4, -0.2, 1.5], [0.0, 0.4, 1.5]],
+    ]
+    tris = [[t[0], t[2], t[1]] for t in tris]  # front-facing for this camera
+    pos = np.array(tris, F).reshape(-1, 3)
+    pos[:, 2] = -pos[:, 2]
+    n = pos.shape[0]
+    col = np.tile(np.array([[0.9, 0.3, 0.1, 1.0]], F), (n, 1))
+    col[4] = [nan, 0.5, 0.5, 1.0]
+    uv = np.tile(np.array([[0.25, 0.75]], F), (n, 1))
+    uv[7] = [nan, 0.5]
+    uv[10] = [inf, 0.2]
+    uv[19] = [1.5, -0.25]
+    proj = hm.project(0.5, 50.0, F(1.0), hm.FRAC_PI_2)
+    return scenes.Scene(f"nonfinite_fs{fs}", 96, 80, pos, col, uv, np.arange(n, dtype=np.uint32), hm.identity(),
+                        hm.identity(), proj, fs, meshes.checkerboard(64, 8))
+
+
+@pytest.mark.parametrize("fs", [scenes.FS_COLOR, scenes.FS_TEXTURE, scenes.FS_DEBUG])
+def test_nonfinite_inputs(fs):
+    g, c = check(_nonfinite_scene(fs))
+    assert c["stats"]["clipped"] >= 3 and (c["img"] != 0xFF191919).sum() > 100
