@@ -85,6 +85,13 @@ def test_too_many_bins_rejected():
         ctx.close()
 
 
+@pytest.mark.parametrize("w,h", [(16384, 24), (24, 16384)])
+def test_widest_screens(w, h):
+    """Screens at the 16384-pixel side limit (bbox corners use all 15 bits,
+    64 coarse bins along one axis)."""
+    check(scenes.soup_scene(20000, w, h, seed=15, textured=True))
+
+
 def test_one_pixel_screen():
     sc = scenes.soup_scene(500, 1, 1, seed=13)
     check(sc)
