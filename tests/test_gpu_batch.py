@@ -53,6 +53,21 @@ def test_batch_full_size_workload():
         assert_parity(res[i], run_oracle(sc), where=f"batch scene {i} {sc.width}x{sc.height} {sc.ntri} tris")
 
 
+def test_batch_many_tiny_scenes():
+    """3000 scenes of 1x1 to 48x48 pixels and 1 to 60 triangles in one job
+    (many scenes per coarse-binning chunk and per raster band); a sample of
+    them against the oracle, and every image against its plain render."""
+    scs = scenes.batch_scenes(n_scenes=3000, seed=93, tri_lo=1, tri_hi=60, resolutions=(1, 7, 16, 33, 48),
+                              tex_size=16)
+    res, stats = run_batch(scs, msaa=True)
+    assert stats["tris"] == sum(s.ntri for s in scs)
+    for i in range(0, len(scs), 97):
+        assert_parity(res[i], run_oracle(scs[i], threads=1), where=f"tiny scene {i}")
+    for i in range(1, len(scs), 211):
+        plain = run_gpu(scs[i], msaa=False)
+        assert np.array_equal(res[i]["img"], plain["img"]), i
+
+
 def test_batch_multi_draw_scene_and_empty_scene():
     a = scenes.soup_scene(3000, 200, 150, seed=81)
     b = scenes.soup_scene(2500, 200, 150, seed=82, textured=True)
