@@ -92,6 +92,20 @@ def test_widest_screens(w, h):
     check(scenes.soup_scene(20000, w, h, seed=15, textured=True))
 
 
+@pytest.mark.parametrize("tw,th", [(1, 1), (3, 257), (257, 3), (1000, 7)])
+def test_odd_texture_sizes(tw, th):
+    """Textures of one texel and of extreme aspect ratios, with uv in [0, 1]
+    and, for a second soup, in [-0.5, 1.5) (clamped low, row-wrapping or
+    out-of-range reads high, as the reference's linear texel index does)."""
+    rng = np.random.default_rng(tw * 1000 + th)
+    tex = rng.integers(0, 256, (th, tw, 4), dtype=np.uint8)
+    sc = scenes.soup_scene(3000, 160, 120, seed=16, textured=True, texture=tex)
+    check(sc)
+    wide = scenes.soup_scene(3000, 160, 120, seed=17, textured=True, texture=tex)
+    wide.uv = (wide.uv * 2.0 - 0.5).astype(np.float32)
+    check(wide)
+
+
 def test_one_pixel_screen():
     sc = scenes.soup_scene(500, 1, 1, seed=13)
     check(sc)
