@@ -106,6 +106,16 @@ def test_odd_texture_sizes(tw, th):
     check(wide)
 
 
+@pytest.mark.parametrize("tw,th", [((1 << 24) + 1, 1), (1, (1 << 24) + 1)])
+def test_texture_side_past_24_bits(tw, th):
+    """A texture side of 2^24 + 1 texels takes the raster's exact texel
+    path (64-bit index arithmetic) instead of the 32-bit fast path."""
+    rng = np.random.default_rng(18)
+    tex = rng.integers(0, 256, (th, tw, 4), dtype=np.uint8)
+    sc = scenes.soup_scene(2000, 128, 96, seed=19, textured=True, texture=tex)
+    check(sc)
+
+
 def test_one_pixel_screen():
     sc = scenes.soup_scene(500, 1, 1, seed=13)
     check(sc)
