@@ -1646,9 +1646,10 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
     }
 }
 
-// The production variant is held to 64 VGPRs (8 waves per SIMD): its few
-// spills are values kept across the list walk, stored once per tile, and the
-// extra waves hide more of the walk's load latency than they cost.
+// The production variant is held to 64 VGPRs (8 waves per SIMD): the extra
+// waves hide more of the walk's load latency than anything the limit costs.
+// Its few spills sit in the shader's exact texel path (64-bit texel index
+// arithmetic), which only textures with a side past 2^24 texels take.
 template <bool STATS, bool PERSIST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STATS || PERSIST ? 1 : 8, 8))) void k_raster(
     JobBufs b, const PrimGeom* __restrict__ geom, const float* __restrict__ attrs, const uint2* __restrict__ flist,
