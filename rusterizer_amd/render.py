@@ -267,6 +267,16 @@ class Texture:
     def from_rgba8(rgba8) -> "Texture":
         return Texture(rgba8)
 
+    @staticmethod
+    def from_raw_file(path: str, width: int, height: int) -> "Texture":
+        """A texture from a file of raw RGBA8 texels, row-major from the
+        upper-left texel: the stand-in for the PNG decode of
+        src/texture.rs:26-45 (the decoded pixel layout it produces)."""
+        data = np.fromfile(path, dtype=np.uint8)
+        if data.size != width * height * 4:
+            raise ValueError(f"{path}: {data.size} bytes, expected {width} x {height} x 4")
+        return Texture(data.reshape(height, width, 4))
+
 
 class Mesh:
     """src/mesh.rs:5-12. vertices (n,3) f32, indices (3k,) int, attributes

@@ -91,3 +91,22 @@ def test_poisson_arrivals_rate_and_determinism():
     # exponential inter-arrival times: mean 1/rate, coefficient of variation 1
     assert abs(gaps.mean() * 2000.0 - 1.0) < 0.03
     assert abs(gaps.std() / gaps.mean() - 1.0) < 0.05
+
+
+def test_texture_from_raw_file(tmp_path):
+    from rusterizer_amd import meshes
+    from rusterizer_amd.render import Texture
+
+    tex = meshes.demo_checkerboard()
+    h, w = tex.shape[:2]
+    path = tmp_path / "demo.rgba"
+    tex.tofile(path)
+    t = Texture.from_raw_file(str(path), w, h)
+    assert (t.width, t.height) == (w, h) and np.array_equal(t.buf, tex)
+    # upper-left texel first, rows in order: texel (x, y) is byte 4 * (y * w + x)
+    raw = np.fromfile(path, dtype=np.uint8)
+    assert np.array_equal(t.buf[7, 123], raw[4 * (7 * w + 123):4 * (7 * w + 123) + 4])
+    import pytest
+
+    with pytest.raises(ValueError):
+        Texture.from_raw_file(str(path), w + 1, h)
