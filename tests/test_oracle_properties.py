@@ -153,11 +153,15 @@ def test_perspective_correct_world_affine(oracle):
 RG = np.array([[5 / 8, 1 / 8], [7 / 8, 5 / 8], [3 / 8, 7 / 8], [1 / 8, 3 / 8]], np.float32)
 
 
-def numpy_frame(tris_ndc, cols, W, H):
+def numpy_frame(tris_ndc, cols, W, H, fs="color"):
     """Independent model of the raster for triangles already inside the NDC
     cube: per triangle (in order) per pixel, 4-sample coverage with the tie
     rule, strict-LESS depth, shading point = pixel centre if all 4 samples
-    pass else the first passing one, colour FS, box-filter resolve."""
+    pass else the first passing one, box-filter resolve. fs="color" shades
+    the interpolated vertex colour (main.rs:72); fs="debug" shades
+    grayscale(depths[0]) (main.rs:73-75, color.rs:65-72), where depths are the
+    depths of the raster-covered samples, 0.0 where a sample is not covered
+    (mod.rs:225-253), taken before the depth test (mod.rs:458-463)."""
     cbuf = np.full((H, W, 4), 0xFF191919, np.uint32)
     zbuf = np.full((H, W, 4), np.finfo(np.float32).max, np.float32)
     ys, xs = np.mgrid[0:H, 0:W]
@@ -215,8 +219,11 @@ def numpy_frame(tris_ndc, cols, W, H):
         ww = np.where(ww < 0, F(0), np.where(ww > 1, F(1), ww)).astype(np.float32)
         argb = np.zeros((H, W), np.uint32)
         for ch, sh in ((3, 24), (0, 16), (1, 8), (2, 0)):
-            val = ((col[0][ch] * u).astype(np.float32) + (col[1][ch] * v).astype(np.float32)).astype(np.float32)
-            val = (val + (col[2][ch] * ww).astype(np.float32)).astype(np.float32)
+            if fs == "debug":
+                val = np.full((H, W), F(1), np.float32) if ch == 3 else dep[..., 0]
+            else:
+                val = ((col[0][ch] * u).astype(np.float32) + (col[1][ch] * v).astype(np.float32)).astype(np.float32)
+                val = (val + (col[2][ch] * ww).astype(np.float32)).astype(np.float32)
             q = (val * F(255)).astype(np.float32)
             qi = np.where(np.isnan(q) | (q <= 0), 0, np.minimum(q, 4294967295.0)).astype(np.uint64).astype(np.uint32)
             argb |= (qi << np.uint32(sh)).astype(np.uint32)
@@ -230,8 +237,9 @@ def numpy_frame(tris_ndc, cols, W, H):
     return img, cbuf, zbuf
 
 
+@pytest.mark.parametrize("fs", ["color", "debug"])
 @pytest.mark.parametrize("seed", range(6))
-def test_tiny_frames_match_numpy_model(oracle, seed):
+def test_tiny_frames_match_numpy_model(oracle, seed, fs):
     rng = np.random.default_rng(100 + seed)
     W, H = 24, 16
     n = 6
@@ -246,15 +254,98 @@ def test_tiny_frames_match_numpy_model(oracle, seed):
     r = oracle.OracleRenderer(W, H)
     pos = ndc.reshape(-1, 3)
     r.set_uniforms(hm.identity(), hm.identity(), hm.identity())
-    r.render(pos, cols.reshape(-1, 4), None, np.arange(3 * n), oracle.FS_COLOR)
+    r.render(pos, cols.reshape(-1, 4), None, np.arange(3 * n),
+             oracle.FS_DEBUG if fs == "debug" else oracle.FS_COLOR)
     col_o, dep_o = r.read_msaa()
     img_o = r.framebuffer()
     tris = np.concatenate([ndc, np.ones((n, 3, 1), np.float32)], -1)
-    img_m, col_m, dep_m = numpy_frame(tris, cols, W, H)
+    img_m, col_m, dep_m = numpy_frame(tris, cols, W, H, fs)
     assert np.array_equal(dep_o.view(np.uint32), dep_m.view(np.uint32))
     assert np.array_equal(col_o, col_m)
     assert np.array_equal(img_o, img_m)
     assert (img_o != 0xFF191919).mean() > 0.1  # the scene draws something
+    if fs == "debug":
+        # partial pixels, where sample 0 may be uncovered (depths[0] = 0.0),
+        # are part of the frame: some pixel has a mix of written and clear samples
+        written = col_o != 0xFF191919
+        assert (written.any(-1) & ~written.all(-1)).any()
+
+
+def test_debug_shader_uses_sample0_depth_not_shading_point(oracle):
+    """The Debug FS reads depths[0], the covered sample-0 depth, even when
+    the fragment is shaded at another sample: a pixel whose sample 0 is
+    outside the triangle gets grayscale(0.0) = 0xFF000000 on the samples that
+    pass; one with sample 0 inside gets trunc(z * 255)."""
+    W, H = 8, 8
+    # the rectangle x in [0, 6.5], y in [0, 8] (screen px) inside the NDC cube
+    # (no clipping), split along its diagonal; the vertical edge x = 6.5 cuts
+    # pixel 6: samples 0 and 1 (x-offsets 5/8, 7/8) lie right of it, samples
+    # 2 and 3 (3/8, 1/8) left
+    z = F(0.2)  # NDC z -> screen depth (0.2 + 1) * 0.5 = 0.6
+    xe = F(6.5) / F(4) - F(1)
+    rect = np.array([[-1, -1, z], [-1, 1, z], [xe, 1, z], [-1, -1, z], [xe, 1, z], [xe, -1, z]], np.float32)
+    r = oracle.OracleRenderer(W, H)
+    r.set_uniforms(hm.identity(), hm.identity(), hm.identity())
+    r.render(rect, None, None, np.arange(6), oracle.FS_DEBUG)
+    col, dep = r.read_msaa()
+    assert r.stats()["clipped"] == 0  # nothing clipped: one fragment per triangle and pixel
+    zs = F(F(F(z + F(1)) * F(0.5)) * F(1)) + F(0)
+    fmax = np.finfo(np.float32).max
+    # row 6, pixel 6 (lower-right triangle only): samples 2, 3 pass, depths[0] = 0.0
+    assert dep[6, 6, 0] == fmax and dep[6, 6, 1] == fmax
+    assert col[6, 6, 2] == 0xFF000000 and col[6, 6, 3] == 0xFF000000
+    assert col[6, 6, 0] == 0xFF191919 and col[6, 6, 1] == 0xFF191919
+    # row 6, pixel 5: fully covered by the same triangle -> grayscale(depth of sample 0)
+    q = int(F(dep[6, 5, 0] * F(255)))
+    assert abs(float(dep[6, 5, 0]) - float(zs)) < 1e-6
+    assert (col[6, 5] == (0xFF000000 | q << 16 | q << 8 | q)).all()
+
+
+# ------------------------------------------------------ vertex shader order
+def _dot(a, b):
+    """Vector::dot (vector.rs:17-23): sum from 0.0, left to right, f32."""
+    s = F(0)
+    for x, y in zip(a, b):
+        s = F(s + F(F(x) * F(y)))
+    return s
+
+
+def _mm(a, b):
+    """Matrix * Matrix (matrix.rs:56-79): element (i, j) = row i . column j."""
+    return np.array([[_dot(a[i, :], b[:, j]) for j in range(4)] for i in range(4)], np.float32)
+
+
+def _mv(a, v):
+    """Matrix * Vector (vector.rs:219-240): element i = row i . v."""
+    return np.array([_dot(a[i, :], v) for i in range(4)], np.float32)
+
+
+def test_vertex_shader_is_left_associated(oracle):
+    """The demo VS (main.rs:147-152) is `projection * view * world *
+    vertex.extend(1.0)`, which Rust evaluates left to right:
+    ((P * V) * W) * v. f32 products are not associative, so on these inputs
+    P * (V * W) and P * (V * (W * v)) round differently; the oracle's
+    rzo_mvp + rzo_mat_vec must give exactly the left-associated bits."""
+    rng = np.random.default_rng(7)
+    differ_right = differ_seq = 0
+    for _ in range(8):
+        world = rng.uniform(-3, 3, (4, 4)).astype(np.float32)
+        view = rng.uniform(-3, 3, (4, 4)).astype(np.float32)
+        proj = rng.uniform(-3, 3, (4, 4)).astype(np.float32)
+        m_o = oracle.mvp(world, view, proj).reshape(4, 4)
+        m_l = _mm(_mm(proj, view), world)
+        assert np.array_equal(m_o.view(np.uint32), m_l.view(np.uint32))
+        m_r = _mm(proj, _mm(view, world))
+        for _ in range(16):
+            p = rng.uniform(-5, 5, 3).astype(np.float32)
+            v = np.array([p[0], p[1], p[2], 1.0], np.float32)
+            got = oracle.mat_vec(m_o, v)
+            want = _mv(m_l, v)
+            assert np.array_equal(np.asarray(got, np.float32).view(np.uint32), want.view(np.uint32))
+            differ_right += not np.array_equal(_mv(m_r, v), want)
+            differ_seq += not np.array_equal(_mv(proj, _mv(view, _mv(world, v))), want)
+    # the alternative associations are distinguishable on these inputs
+    assert differ_right > 10 and differ_seq > 10, (differ_right, differ_seq)
 
 
 def test_hand_checked_depth_order(oracle):
