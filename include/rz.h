@@ -210,6 +210,38 @@ int rz_batch_tuning(rz_ctx *ctx, uint32_t bin_target, uint32_t raster_per_cu, in
 /* The current tuning: [bin_target, raster_per_cu, use_graph]. */
 int rz_batch_get_tuning(const rz_ctx *ctx, uint32_t out[3]);
 
+/* ------------------------------------------------------- allocation-free frame
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): one frame of one draw, rendered exactly as
+ * rz_set_uniforms + rz_draw + rz_resolve_device would render it in a fresh
+ * context of that size, with every working buffer taken from a fixed
+ * 128 MiB device array that the library holds per GPU until the process
+ * ends. The call makes no device allocation and no host-to-device copy.
+ *
+ * Inputs are device pointers: pos nv*3 f32, color nv*4 f32 (or NULL:
+ * zeros), uv nv*2 f32 (or NULL: zeros), idx ni u32 (ni a multiple of 3, at
+ * most 3 * 262144, every index < nv, checked on the device), the texture as
+ * th rows of tw RGBA8 texels (for RZ_FS_TEXTURE; else may be NULL); world,
+ * view, proj are host row-major f32[16]; argb_dev receives width*height
+ * 0xAARRGGBB words. Work runs on `stream` (a hipStream_t; NULL = the null
+ * stream); the call returns after the frame finished there. *flags_out (may
+ * be NULL) receives the frame's RZ_FLAG_* bits; *message (may be NULL) a
+ * static description of a failure ("" on success).
+ *
+ * Errors: RZ_ERR_ARG / RZ_ERR_STATE as rz_draw and rz_resolve_device;
+ * RZ_ERR_LIMIT when the screen or the triangle count is beyond the limits
+ * above, or when the frame's primitives or bin lists outgrow the array
+ * (rz_resolve_device grows its buffers instead); the output is then not
+ * defined. Calls on one GPU share its array: calls that overlap in time on
+ * the same GPU (two host threads, or two streams) race and must not be
+ * made. */
+int rz_render_noalloc(void *stream, uint32_t width, uint32_t height, float clear_depth, const float world[16],
+                      const float view[16], const float proj[16], const float *pos_dev, const float *color_dev,
+                      const float *uv_dev, uint32_t nv, const uint32_t *idx_dev, uint64_t ni,
+                      const uint32_t *texture_dev, uint32_t tw, uint32_t th, int vertex_shader,
+                      int fragment_shader, uint32_t *argb_dev, uint32_t *flags_out, const char **message);
+
 /* ------------------------------------------------------------ tuning/timing */
 
 /* Coarse bin side in pixels: a multiple of 16 up to 256 with at most 256 bins

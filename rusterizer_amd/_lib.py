@@ -73,6 +73,9 @@ SIGNATURES = {
     "rz_set_coarse_bin": (C.c_int, [_vp, C.c_uint32]),
     "rz_profile": (C.c_int, [_vp, C.c_int]),
     "rz_profile_read": (C.c_int, [_vp, _f, _u32]),
+    "rz_render_noalloc": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_float, _f, _f, _f, _vp, _vp, _vp,
+                                    C.c_uint32, _vp, C.c_uint64, _vp, C.c_uint32, C.c_uint32, C.c_int, C.c_int,
+                                    _vp, _u32, C.POINTER(C.c_char_p)]),
 }
 
 _lib = None

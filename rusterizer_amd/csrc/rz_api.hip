@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -36,6 +37,7 @@ constexpr uint32_t MAX_COARSE_PX = 256;
 constexpr uint32_t MAX_DRAWS = 65536;   // draw index lives in 16 bits of a prim's flags
 constexpr uint32_t MAX_SCENES = 65536;
 constexpr uint32_t MAX_REPLAYS = 4;
+constexpr uint64_t NOALLOC_MAX_TRIS = 262144;  // rz_render_noalloc: triangles per call
 
 const char* kNullCtxMsg = "null context";
 
@@ -787,6 +789,281 @@ int create_impl(rz_pool* pool, uint32_t width, uint32_t height, rz_ctx** out) {
 }
 
 }  // namespace
+
+// ===========================================================================
+// Allocation-free frame (rz_render_noalloc)
+//
+// Every working buffer of the pipeline is carved out of `scratch`, one
+// file-scope device array per GPU, held until the process ends. The job
+// table is written by a kernel from its arguments, so the call makes no
+// device allocation and no host-to-device copy; it reads back the frame
+// counters (and, before that, the largest index) into host stack memory.
+// Calls that overlap on one GPU share the array and race.
+// ===========================================================================
+namespace {
+
+constexpr size_t NOALLOC_SCRATCH_BYTES = 128ull << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+// per device: the array's device address (hipGetSymbolAddress, taken once
+// with that device current); the array's name is only a host-side stand-in
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+hipError_t scratch_for_device(int dev, unsigned char** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+// Bump allocator over [0, cap): offsets only (base may be null while sizing).
+struct Carve {
+    unsigned char* base;
+    size_t off = 0, cap;
+    template <class T>
+    T* take(size_t n) {
+        off = (off + 255) & ~(size_t)255;
+        T* p = reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(base) + off);
+        off += n * sizeof(T);
+        return p;
+    }
+};
+
+// the one-scene, one-draw job table as written by k_write_words: JobHead at
+// word 0, DrawDev at 64, Mat3 at 128, SceneCfg at 192 (256-byte offsets)
+constexpr uint32_t NA_OFF_DRAW = 64, NA_OFF_MAT = 128, NA_OFF_SCENE = 192, NA_TABLE_WORDS = 212;
+static_assert(sizeof(JobHead) <= 4 * NA_OFF_DRAW && sizeof(DrawDev) <= 4 * (NA_OFF_MAT - NA_OFF_DRAW) &&
+                  sizeof(Mat3) <= 4 * (NA_OFF_SCENE - NA_OFF_MAT) &&
+                  4 * NA_OFF_SCENE + sizeof(SceneCfg) <= 4 * NA_TABLE_WORDS && NA_TABLE_WORDS <= WORD_BLOCK,
+              "allocation-free job table layout");
+
+// Lays the working buffers of a W x H frame of ntri_pad (padded) triangles
+// out in the scratch array: fixed-size buffers from the frame's shape, then
+// the coarse and fine lists split what is left. Returns false if the fixed
+// part does not fit.
+bool noalloc_layout(unsigned char* base, uint32_t ntri_pad, const SceneCfg& sc, uint32_t ntiles, JobBufs& b,
+                    uint32_t** maxidx, size_t* fixed_bytes) {
+    Carve cv{base, 0, NOALLOC_SCRATCH_BYTES};
+    uint32_t* table = cv.take<uint32_t>(NA_TABLE_WORDS);
+    b = JobBufs{};
+    b.head = reinterpret_cast<const JobHead*>(table);
+    b.draws = reinterpret_cast<DrawDev*>(table + NA_OFF_DRAW);
+    b.mats = reinterpret_cast<const Mat3*>(table + NA_OFF_MAT);
+    b.scenes = reinterpret_cast<const SceneCfg*>(table + NA_OFF_SCENE);
+    b.runs = cv.take<SceneRun>(1);
+    b.cnt = cv.take<FrameCounters>(1);
+    *maxidx = cv.take<uint32_t>(64);
+    const uint64_t nbins = (uint64_t)sc.nbx * sc.nby, T = (uint64_t)(sc.cb / 16) * (sc.cb / 16);
+    const uint64_t tri_cap = std::max<uint64_t>(ntri_pad, 256);
+    const uint64_t prims = ntri_pad + ntri_pad / 4 + 4096 + CHUNK;
+    const uint64_t clip = ntri_pad / 8 + 1024;
+    const uint64_t mat = nbins * (prims / CHUNK + 1);
+    b.cls = cv.take<uint32_t>(tri_cap);
+    b.nsub = cv.take<uint32_t>(tri_cap);
+    b.suboff = cv.take<uint32_t>(tri_cap);
+    b.tchunk_draw = cv.take<uint32_t>(tri_cap / TRI_ALIGN + 1);
+    b.geom = cv.take<PrimGeom>(prims);
+    b.attrs = cv.take<float>(prims * ATTR_STRIDE);
+    b.pbin = cv.take<uint4>(prims);
+    b.pchunk_scene = cv.take<uint32_t>(prims / CHUNK + 1);
+    b.clip_tri = cv.take<uint32_t>(clip);
+    b.clip_redo = cv.take<uint32_t>(clip);
+    b.clip_draw = cv.take<uint32_t>(clip);
+    b.clip_out = cv.take<float>(clip * MAXV * 10);
+    b.bin_counts = cv.take<uint32_t>(mat);
+    b.bin_offsets = cv.take<uint32_t>(mat);
+    b.bin_rng = cv.take<uint2>(nbins);
+    b.tile_rec = cv.take<uint4>(ntiles);
+    b.fpack = cv.take<unsigned long long>(nbins);
+    b.fpre = cv.take<unsigned long long>(nbins);
+    b.parts64 = cv.take<unsigned long long>(nbins / SCAN_TILE + 2);
+    // what is left: a third for the coarse list (16 B per entry, plus its
+    // share of the fine-binning chunk table and fine count matrices), the
+    // rest for the 8-byte fine entries and the scan partials
+    const size_t slack = 16 * 256;
+    *fixed_bytes = cv.off + slack;
+    if (cv.off + slack >= NOALLOC_SCRATCH_BYTES) return false;
+    const size_t left = NOALLOC_SCRATCH_BYTES - cv.off - slack;
+    // bytes per coarse entry: 16 in the list, plus the fine-binning chunk
+    // record (8 B) and two fine count matrices (8 B per tile) per CHUNK entries
+    const size_t per_pair = 16 + (8 + 8 * T) / CHUNK + 1;
+    uint64_t pairs = left / 3 / per_pair;
+    const uint64_t fch = pairs / CHUNK + nbins + 1;
+    const uint64_t fmat = T * fch;
+    if (pairs < 1024) return false;
+    b.list = cv.take<uint4>(pairs);
+    b.fchunk_rec = cv.take<uint2>(fch);
+    b.fine_counts = cv.take<uint32_t>(fmat);
+    b.fine_off = cv.take<uint32_t>(fmat);
+    const uint64_t scan_n = std::max<uint64_t>({tri_cap, mat, fmat});
+    b.parts = cv.take<uint32_t>(scan_n / SCAN_TILE + 2);
+    cv.off = (cv.off + 255) & ~(size_t)255;
+    if (cv.off + 4096 >= NOALLOC_SCRATCH_BYTES) return false;
+    const uint64_t fine = (NOALLOC_SCRATCH_BYTES - cv.off - 256) / sizeof(uint2);
+    b.flist = cv.take<uint2>(fine);
+    if (cv.off > NOALLOC_SCRATCH_BYTES) return false;
+    const uint64_t lim = 0xF0000000ull;
+    b.tri_cap = (uint32_t)std::min(tri_cap, lim);
+    b.clip_cap = (uint32_t)clip;
+    b.prim_cap = (uint32_t)prims;
+    b.pair_cap = (uint32_t)pairs;
+    b.mat_cap = (uint32_t)mat;
+    b.fine_cap = (uint32_t)fine;
+    b.fmat_cap = (uint32_t)fmat;
+    b.bins_cap = (uint32_t)nbins;
+    b.tile_cap = ntiles;
+    b.fchunk_cap = (uint32_t)fch;
+    b.tchunk_cap = (uint32_t)(tri_cap / TRI_ALIGN + 1);
+    b.pchunk_cap = (uint32_t)(prims / CHUNK + 1);
+    b.rstat_parts = nullptr;
+    b.rstat_cap = 0;
+    b.clip_wide = 0;
+    b.exact_paths = 0;
+    return true;
+}
+
+int noalloc_fail(const char** message, int code, const char* msg) {
+    if (message) *message = msg;
+    return code;
+}
+
+}  // namespace
+
+extern "C" int rz_render_noalloc(void* stream, uint32_t width, uint32_t height, float clear_depth,
+                                 const float world[16], const float view[16], const float proj[16],
+                                 const float* pos_dev, const float* color_dev, const float* uv_dev, uint32_t nv,
+                                 const uint32_t* idx_dev, uint64_t ni, const uint32_t* texture_dev, uint32_t tw,
+                                 uint32_t th, int vertex_shader, int fragment_shader, uint32_t* argb_dev,
+                                 uint32_t* flags_out, const char** message) {
+    if (message) *message = "";
+    if (flags_out) *flags_out = 0;
+    // argument checks first: a rejected call touches no device
+    if (width == 0 || height == 0 || width > MAX_DIM || height > MAX_DIM)
+        return noalloc_fail(message, RZ_ERR_ARG, "width and height must be in [1, 16384]");
+    if (((width + MAX_COARSE_PX - 1) / MAX_COARSE_PX) * ((height + MAX_COARSE_PX - 1) / MAX_COARSE_PX) > MAX_BINS)
+        return noalloc_fail(message, RZ_ERR_LIMIT, "the screen needs more than 256 coarse bins");
+    if (!world || !view || !proj) return noalloc_fail(message, RZ_ERR_ARG, "null matrix");
+    if (!argb_dev) return noalloc_fail(message, RZ_ERR_ARG, "null output");
+    if (!pos_dev || (!idx_dev && ni)) return noalloc_fail(message, RZ_ERR_ARG, "null pointer");
+    if (ni % 3 != 0) return noalloc_fail(message, RZ_ERR_ARG, "index count is not a multiple of 3");
+    if (vertex_shader != RZ_VS_MVP) return noalloc_fail(message, RZ_ERR_ARG, "unknown vertex shader");
+    if (fragment_shader != RZ_FS_TEXTURE && fragment_shader != RZ_FS_COLOR && fragment_shader != RZ_FS_DEBUG)
+        return noalloc_fail(message, RZ_ERR_ARG, "unknown fragment shader");
+    if (fragment_shader == RZ_FS_TEXTURE && (!texture_dev || tw == 0 || th == 0))
+        return noalloc_fail(message, RZ_ERR_STATE, "texture fragment shader with no texture");
+    if (fragment_shader == RZ_FS_TEXTURE && (uint64_t)tw * th >= (1ull << 31))
+        return noalloc_fail(message, RZ_ERR_LIMIT, "texture too large");
+    const uint64_t ntri = ni / 3;
+    if (ntri > NOALLOC_MAX_TRIS)
+        return noalloc_fail(message, RZ_ERR_LIMIT, "more triangles than the allocation-free path takes (262144)");
+    const uint32_t ntri_pad = (uint32_t)((ntri + TRI_ALIGN - 1) / TRI_ALIGN * TRI_ALIGN);
+
+    // the job table (one scene, one draw), as rz_draw + rz_resolve_device build it
+    WordBlock blk;
+    memset(&blk, 0, sizeof(blk));
+    JobHead* head = reinterpret_cast<JobHead*>(blk.w);
+    DrawDev* dd = reinterpret_cast<DrawDev*>(blk.w + NA_OFF_DRAW);
+    Mat3* mat = reinterpret_cast<Mat3*>(blk.w + NA_OFF_MAT);
+    SceneCfg* sc = reinterpret_cast<SceneCfg*>(blk.w + NA_OFF_SCENE);
+    sc->W = width;
+    sc->H = height;
+    sc->clear_depth = clear_depth;
+    sc->cb = coarse_side(width, height, 0, MAX_BINS);
+    sc->nbx = (width + sc->cb - 1) / sc->cb;
+    sc->nby = (height + sc->cb - 1) / sc->cb;
+    sc->tri_begin = 0;
+    sc->tri_end = ntri_pad;
+    sc->out = argb_dev;
+    const uint32_t ntiles = ((width + 15) / 16) * ((height + 15) / 16);
+    dd->pos = pos_dev;
+    dd->col = fragment_shader == RZ_FS_COLOR ? color_dev : nullptr;
+    dd->uv = fragment_shader == RZ_FS_TEXTURE ? uv_dev : nullptr;
+    dd->idx = idx_dev;
+    dd->ntri = (uint32_t)ntri;
+    dd->fs = (uint32_t)fragment_shader;
+    if (fragment_shader == RZ_FS_TEXTURE) {
+        dd->tex = texture_dev;
+        dd->tw = tw;
+        dd->th = th;
+    }
+    memcpy(mat->world, world, sizeof(mat->world));
+    memcpy(mat->view, view, sizeof(mat->view));
+    memcpy(mat->proj, proj, sizeof(mat->proj));
+    head->ndraws = 1;
+    head->nscenes = 1;
+    head->ntri = ntri_pad;
+    head->ntiles = ntiles;
+    head->nbins = sc->nbx * sc->nby;
+
+    JobBufs b;
+    uint32_t* d_maxidx = nullptr;
+    size_t fixed = 0;
+    if (!noalloc_layout(nullptr, ntri_pad, *sc, ntiles, b, &d_maxidx, &fixed))
+        return noalloc_fail(message, RZ_ERR_LIMIT,
+                            "the frame's fixed buffers do not fit the 128 MiB scratch of the allocation-free path");
+
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return noalloc_fail(message, RZ_ERR_HIP, "hipGetDevice failed");
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return noalloc_fail(message, RZ_ERR_LIMIT, "device ordinal above 63");
+    unsigned char* base = nullptr;
+    if (scratch_for_device(dev, &base) != hipSuccess)
+        return noalloc_fail(message, RZ_ERR_HIP, "hipGetSymbolAddress of the scratch array failed");
+    noalloc_layout(base, ntri_pad, *sc, ntiles, b, &d_maxidx, &fixed);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+
+    // every index < nv (the reference indexes vertex_buf[idx], render.rs:78-83)
+    if (ni) {
+        uint32_t hmax = 0;
+        launch_fill_u32(s, d_maxidx, 1, 0u);
+        launch_max_index(s, idx_dev, ni, d_maxidx);
+        if (hipMemcpyAsync(&hmax, d_maxidx, sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return noalloc_fail(message, RZ_ERR_HIP, "index check failed");
+        if (hmax >= nv) return noalloc_fail(message, RZ_ERR_ARG, "index out of range");
+    }
+
+    launch_write_words(s, blk, NA_TABLE_WORDS, reinterpret_cast<uint32_t*>(const_cast<JobHead*>(b.head)));
+    JobGrid g{};
+    g.draws = 1;
+    g.tris = std::min<uint32_t>((ntri_pad + 255) / 256, 8192);
+    g.clip = std::min<uint32_t>(b.clip_cap, 131072);
+    g.bins = std::min<uint32_t>(b.prim_cap / CHUNK + 1, 8192);
+    g.fine = std::min<uint32_t>(b.pair_cap / CHUNK + b.bins_cap, 8192);
+    g.raster = 0;
+    g.ntiles = ntiles;
+    g.scenes = 1;
+    launch_job(s, b, g, false, nullptr, nullptr);
+    if (hipGetLastError() != hipSuccess) return noalloc_fail(message, RZ_ERR_HIP, "kernel launch failed");
+    FrameCounters fc;
+    SceneRun run;
+    if (hipMemcpyAsync(&fc, b.cnt, sizeof(fc), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(&run, b.runs, sizeof(run), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return noalloc_fail(message, RZ_ERR_HIP, "frame failed on the device");
+    if (fc.overflow & OVF_POLY) return noalloc_fail(message, RZ_ERR_LIMIT, "a clipped polygon exceeded 16 vertices");
+    // what outgrew the array (rz_resolve_device grows its buffers instead)
+    if (fc.overflow & OVF_PRIMS)
+        return noalloc_fail(message, RZ_ERR_LIMIT, "the frame's primitives do not fit the 128 MiB scratch");
+    if (fc.overflow & OVF_CLIP)
+        return noalloc_fail(message, RZ_ERR_LIMIT, "the frame's clipped triangles do not fit the 128 MiB scratch");
+    if (fc.overflow & OVF_MATRIX)
+        return noalloc_fail(message, RZ_ERR_LIMIT, "the coarse count matrix does not fit the 128 MiB scratch");
+    if (fc.overflow & OVF_PAIRS)
+        return noalloc_fail(message, RZ_ERR_LIMIT, "the coarse bin lists do not fit the 128 MiB scratch");
+    if (fc.overflow)
+        return noalloc_fail(message, RZ_ERR_LIMIT, "the tile bin lists do not fit the 128 MiB scratch");
+    if (flags_out) *flags_out = run.flags;
+    return RZ_OK;
+}
 
 // ===========================================================================
 // C ABI

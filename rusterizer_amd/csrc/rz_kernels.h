@@ -193,8 +193,19 @@ struct JobGrid {
     uint32_t scenes;     // scene count (k_frame_reset / k_scene_prep loops)
 };
 
+// A block of words passed to a kernel by value (kernel arguments hold up to
+// 4 KiB): a small job table set up without any host-to-device copy.
+constexpr uint32_t WORD_BLOCK = 256;
+struct WordBlock {
+    uint32_t w[WORD_BLOCK];
+};
+
 // ---- launchers (all asynchronous on `s`) ---------------------------------
 void launch_max_index(hipStream_t s, const uint32_t* idx, uint64_t n, uint32_t* out);  // *out must start at 0
+// dst[0, n) = blk.w[0, n), n <= WORD_BLOCK (one small kernel)
+void launch_write_words(hipStream_t s, const WordBlock& blk, uint32_t n, uint32_t* dst);
+// p[0, n) = v (one small kernel; used where a memset node would be captured)
+void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v);
 // the whole pipeline of one job, from counter reset to resolved outputs
 void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_stats, hipEvent_t ev_bin,
                 hipEvent_t ev_raster);

@@ -1752,6 +1752,16 @@ __global__ __launch_bounds__(256) void k_max_index(const uint32_t* __restrict__ 
     if (lane_id() == 0) atomicMax(out, m);
 }
 
+// copies a block of words passed by value (a job table built on the host)
+// into device memory, so a job can be set up without a host-to-device copy
+__global__ __launch_bounds__(256) void k_write_words(WordBlock blk, uint32_t n, uint32_t* __restrict__ dst) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = blk.w[i];
+}
+
+__global__ __launch_bounds__(64) void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -1759,6 +1769,16 @@ void launch_max_index(hipStream_t s, const uint32_t* idx, uint64_t n, uint32_t* 
     if (n == 0) return;
     uint64_t blocks = (n + 255) / 256;
     k_max_index<<<(uint32_t)(blocks < 1024 ? blocks : 1024), 256, 0, s>>>(idx, n, out);
+}
+
+void launch_write_words(hipStream_t s, const WordBlock& blk, uint32_t n, uint32_t* dst) {
+    if (n == 0) return;
+    k_write_words<<<1, 256, 0, s>>>(blk, n < WORD_BLOCK ? n : WORD_BLOCK, dst);
+}
+
+void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v) {
+    if (n == 0) return;
+    k_fill_u32<<<1, 64, 0, s>>>(p, n, v);
 }
 
 static uint32_t at_least1(uint32_t v) { return v ? v : 1u; }

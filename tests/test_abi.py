@@ -48,3 +48,27 @@ def test_create_rejects_unsupported_screens():
     assert L.rz_create(8192, 8192, ctypes.byref(h)) == -5
     assert L.rz_create(4097, 4096, ctypes.byref(h)) == -5
     assert not h.value
+
+
+def test_noalloc_rejects_bad_arguments_without_a_device():
+    """rz_render_noalloc checks its arguments before any device call."""
+    L = _lib.load()
+    m = (ctypes.c_float * 16)(*([1.0, 0, 0, 0, 0] * 3 + [1.0]))
+    msg = ctypes.c_char_p()
+    flags = ctypes.c_uint32(7)
+    dummy = 0x1000  # never dereferenced: each call fails its checks first
+
+    def call(w=64, h=64, fs=1, vs=0, ni=3, nv=3, pos=dummy, idx=dummy, out=dummy, tex=None, tw=0, th=0):
+        return L.rz_render_noalloc(None, w, h, 1.0, m, m, m, pos, None, None, nv, idx, ni, tex, tw, th, vs, fs, out,
+                                   ctypes.byref(flags), ctypes.byref(msg))
+
+    assert call(w=0) == -1 and b"width" in msg.value and flags.value == 0
+    assert call(w=16384, h=16384) == -5
+    assert call(out=None) == -1
+    assert call(pos=None) == -1
+    assert call(ni=4) == -1 and b"multiple of 3" in msg.value
+    assert call(vs=1) == -1
+    assert call(fs=3) == -1
+    assert call(fs=0) == -3  # texture shader without a texture
+    assert call(fs=0, tex=dummy, tw=65536, th=65536) == -5
+    assert call(ni=3 * 262145) == -5 and b"262144" in msg.value
