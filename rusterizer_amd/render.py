@@ -272,6 +272,8 @@ class Texture:
         """A texture from a file of raw RGBA8 texels, row-major from the
         upper-left texel: the stand-in for the PNG decode of
         src/texture.rs:26-45 (the decoded pixel layout it produces)."""
+        if width <= 0 or height <= 0:
+            raise ValueError(f"texture size must be positive, got {width} x {height}")
         data = np.fromfile(path, dtype=np.uint8)
         if data.size != width * height * 4:
             raise ValueError(f"{path}: {data.size} bytes, expected {width} x {height} x 4")
@@ -338,6 +340,10 @@ class Renderer:
         self._ctx = Context(width, height)
         self._uniforms = Uniforms(self._ctx)
         self._meshes: dict[int, tuple] = {}
+        # device meshes replaced (Mesh.invalidate) or whose Mesh is gone; a
+        # draw of the current frame may still use them, so they are
+        # destroyed after the next framebuffer()
+        self._retired: list[DeviceMesh] = []
 
     @property
     def context(self) -> Context:
@@ -348,14 +354,32 @@ class Renderer:
 
     def _device_mesh(self, mesh: Mesh) -> DeviceMesh:
         key = id(mesh)
-        hit = self._meshes.get(key)
+        hit = self._meshes.pop(key, None)
         if hit is not None:
             ref, ver, dm = hit
             if ref() is mesh and ver == mesh.version:
+                self._meshes[key] = hit
                 return dm
+            self._retired.append(dm)
         dm = self._ctx.mesh(mesh.vertices, mesh.colors, mesh.uvs, mesh.indices)
-        self._meshes[key] = (weakref.ref(mesh), mesh.version, dm)
+        meshes, retired = self._meshes, self._retired
+
+        def gone(r, key=key):  # the Mesh was collected: retire its device copy
+            e = meshes.get(key)
+            if e is not None and e[0] is r:
+                del meshes[key]
+                retired.append(e[2])
+
+        self._meshes[key] = (weakref.ref(mesh, gone), mesh.version, dm)
         return dm
+
+    def _free_retired(self):
+        while self._retired:
+            self._retired.pop().destroy()
+
+    def device_meshes(self) -> int:
+        """Device meshes held (current and retired): for leak checks."""
+        return len(self._meshes) + len(self._retired)
 
     def render(self, mesh: Mesh, vertex_shader: VertexShader = VertexShader.MVP,
                fragment_shader: FragmentShader = FragmentShader.TEXTURE):
@@ -367,13 +391,16 @@ class Renderer:
 
     def framebuffer(self) -> np.ndarray:
         """Rasterizer::framebuffer (mod.rs:520-522): resolve + clear; (H, W) u32 ARGB."""
-        return self._ctx.resolve()
+        img = self._ctx.resolve()
+        self._free_retired()
+        return img
 
     def close(self):
-        for _, _, dm in self._meshes.values():
+        for dm in [e[2] for e in self._meshes.values()] + self._retired:
             try:
                 dm.destroy()
             except RzError:
                 pass
         self._meshes.clear()
+        self._retired.clear()
         self._ctx.close()

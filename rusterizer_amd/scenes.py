@@ -66,16 +66,18 @@ def _camera_at_origin():
 
 
 # ------------------------------------------------------------------ cube
-def cube_scene(width: int = 800, height: int = 600) -> Scene:
+def cube_scene(width: int = 800, height: int = 600, clear_depth: float = CLEAR_DEPTH_MAX) -> Scene:
     """mesh::cube(1.0), textured, 60 degree projection; the camera sits close
-    enough that the near plane cuts the cube."""
+    enough that the near plane cuts the cube. clear_depth: f32::MAX as the
+    reference clears (buffers.rs:6), or 1.0 as the plain cube workload
+    specifies."""
     m = meshes.cube(1.0)
     view = hm.Camera(pos=(0.15, 0.1, -0.8), up=(0.0, 1.0, 0.0), dir=(0.1, 0.05, 1.0)).get_view_matrix()
     proj = hm.project(0.45, 50.0, F(height) / F(width), F(math.pi / 3))
     world = hm.rotate(F(0.5), F(0.7), F(0.2))
     tex = meshes.checkerboard(256, 8)
     return Scene("cube", width, height, m.vertices, m.colors, m.uvs, m.indices, world, view, proj,
-                 FS_TEXTURE, tex)
+                 FS_TEXTURE, tex, clear_depth)
 
 
 # ------------------------------------------------------------------ soup

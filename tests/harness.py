@@ -1,6 +1,6 @@
 """Shared helpers: run a Scene through the HIP library and through the oracle,
-and compare the results at the reference's bar (coverage and depth bit-exact,
-resolved colour within 1 LSB per channel)."""
+and compare them: coverage and depth bit-exact, colour (MSAA samples and the
+resolved image) bit-exact too, which is tighter than the 1-LSB bar."""
 from __future__ import annotations
 
 import os
@@ -74,7 +74,11 @@ def channel_diff(a: np.ndarray, b: np.ndarray) -> int:
     return int(np.abs(a - b).max()) if a.size else 0
 
 
-def assert_parity(gpu: dict, cpu: dict, where: str = ""):
+def assert_parity(gpu: dict, cpu: dict, where: str = "", colour_lsb: int = 0):
+    """Depth and coverage bit-exact; colour exact by default. The kernels
+    keep the reference's operation order, no FMA contraction and the exact
+    byte->float table, so colour matches the oracle bit for bit; colour_lsb=1
+    relaxes it to the 1-LSB bar for a caller that documents why."""
     # depth: bit-exact per sample (this also pins coverage: every covered,
     # depth-passing sample carries its interpolated depth)
     dg = gpu["dep"].view(np.uint32)
@@ -85,10 +89,10 @@ def assert_parity(gpu: dict, cpu: dict, where: str = ""):
     wg = gpu["col"] != CLEAR_COLOR
     wc = cpu["col"] != CLEAR_COLOR
     assert np.array_equal(wg, wc), f"{where}: coverage differs at {np.argwhere(wg != wc)[:5].tolist()}"
-    # colour: within 1 LSB per channel (MSAA samples and the resolved image)
+    # colour: MSAA samples and the resolved image
     d_msaa = channel_diff(gpu["col"], cpu["col"])
     d_img = channel_diff(gpu["img"], cpu["img"])
-    assert d_msaa <= 1 and d_img <= 1, f"{where}: colour differs by {d_msaa} / {d_img} LSB"
+    assert d_msaa <= colour_lsb and d_img <= colour_lsb, f"{where}: colour differs by {d_msaa} / {d_img} LSB"
     assert gpu["flags"] == cpu["flags"] & 5, f"{where}: flags gpu={gpu['flags']} cpu={cpu['flags']}"
     return dict(exact_colour=bool(np.array_equal(gpu["col"], cpu["col"]) and np.array_equal(gpu["img"], cpu["img"])))
 

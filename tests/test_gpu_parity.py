@@ -38,6 +38,20 @@ def test_cube_textured():
     assert (g["img"] != 0xFF191919).mean() > 0.05
 
 
+def test_cube_textured_clear_depth_one():
+    """The plain cube workload clears depth to 1.0 (the reference clears to
+    f32::MAX): every visible sample lies inside the far plane, so the image
+    equals the f32::MAX one; depths are checked against the oracle."""
+    g1, _ = check(scenes.cube_scene(clear_depth=1.0))
+    gm = run_gpu(scenes.cube_scene())
+    assert np.array_equal(g1["img"], gm["img"])
+    # the same samples are written; the others keep their clear depth
+    fmax = np.finfo(np.float32).max
+    untouched = gm["dep"] == fmax
+    assert (g1["dep"][untouched] == 1.0).all()
+    assert np.array_equal(g1["dep"][~untouched], gm["dep"][~untouched])
+
+
 @pytest.mark.parametrize("fs", [scenes.FS_COLOR, scenes.FS_TEXTURE, scenes.FS_DEBUG])
 def test_soup_small_all_shaders(fs):
     sc = scenes.soup_scene(3000, 640, 480, seed=7, textured=(fs == scenes.FS_TEXTURE))

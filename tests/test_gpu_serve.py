@@ -5,6 +5,7 @@ HBM budget."""
 import numpy as np
 import pytest
 
+from rusterizer_amd import _lib as L
 from rusterizer_amd import scenes
 
 pytestmark = pytest.mark.gpu
@@ -71,6 +72,62 @@ def test_serving_mix_equals_alone():
         for r in reqs:
             mixed.setdefault(r.batch, set()).add(cat.templates[r.tmpl].kind)
         assert any(len(k) >= 2 for k in mixed.values())  # kinds shared batches
+    finally:
+        cat.destroy()
+        root.close()
+
+
+def test_serving_full_size_catalog():
+    """Serving at its real size: the 256 batched scenes (seed 1002), the
+    1M-triangle 1080p soup and the 4M-triangle 4K terrain resident, Poisson
+    arrivals at 2000/s (seed 1004, 90% / 7% / 3%), 300 requests with their
+    outputs kept. Every output equals its template rendered alone, and the
+    rendered-alone images of the terrain and of sampled batch scenes equal
+    the oracle's (the 1M soup frame is checked against the oracle by
+    test_soup_full_size)."""
+    import torch
+
+    from harness import run_oracle
+    from rusterizer_amd.render import Context
+    from rusterizer_amd.serve import Catalog, Server, make_requests
+
+    kinds = {"batch": scenes.batch_scenes(), "soup": [scenes.soup_scene()], "terrain": [scenes.terrain_scene()]}
+    root = Context(64, 64)
+    cat = Catalog(root, kinds)
+    try:
+        reqs = make_requests(cat, 300, rate=2000.0, weights={"batch": 0.90, "soup": 0.07, "terrain": 0.03},
+                             seed=scenes.SEEDS["serve"])
+        srv = Server(cat, slots=3, max_batch=64)
+        try:
+            rep = srv.run(reqs, keep_outputs=True)
+        finally:
+            srv.close()
+        assert rep["requests"] == 300 and rep["rejected"] == 0
+        assert set(rep["kinds"]) == {"batch", "soup", "terrain"}, rep["kinds"]
+        used = sorted({r.tmpl for r in reqs})
+        alone = {}
+        for i in used:  # each template used, rendered alone (a batch of one scene)
+            t = cat.templates[i]
+            img = torch.empty(t.pixels, dtype=torch.int32, device="cuda")
+            row = np.array([t.row], dtype=L.batch_item_dtype())
+            row[0]["argb_dev"] = img.data_ptr()
+            cat.root.batch_begin()
+            cat.root.batch_add(row)
+            cat.root.batch_submit()
+            cat.root.batch_wait()
+            alone[i] = img
+        for r in reqs:  # the first mismatch fails the test
+            assert torch.equal(r.out, alone[r.tmpl]), (r.rid, cat.templates[r.tmpl].kind, r.batch)
+        rng = np.random.default_rng(5)
+        batch_used = [i for i in used if cat.templates[i].kind == "batch"]
+        sample = [i for i in used if cat.templates[i].kind == "terrain"]
+        sample += [batch_used[int(k)] for k in rng.choice(len(batch_used), 3, replace=False)]
+        for i in sample:
+            t = cat.templates[i]
+            exp = run_oracle(t.scene)["img"]
+            got = alone[i].cpu().numpy().view(np.uint32).reshape(t.scene.height, t.scene.width)
+            print(f"serve full size: template {i} ({t.kind}, {t.scene.width}x{t.scene.height}) vs oracle", flush=True)
+            assert np.array_equal(got, exp), (i, t.kind)
     finally:
         cat.destroy()
         root.close()

@@ -110,3 +110,9 @@ def test_texture_from_raw_file(tmp_path):
 
     with pytest.raises(ValueError):
         Texture.from_raw_file(str(path), w + 1, h)
+    # a 0 x N size with an empty file passes the byte count but is no texture
+    empty = tmp_path / "empty.rgba"
+    empty.write_bytes(b"")
+    for size in ((0, 5), (5, 0), (-1, -1)):
+        with pytest.raises(ValueError, match="positive"):
+            Texture.from_raw_file(str(empty), *size)
