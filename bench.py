@@ -9,12 +9,13 @@ replay), inputs resident in HBM. Beside it, on the same scenes:
   - the plain path: one renderer per scene, rz_draw + rz_resolve_device per
     scene, one scene per launch sequence;
   - the CPU oracle (single thread) on a bounded sample of the scenes.
-The plain frames of the 1M-triangle soup (1920x1080, colour FS, seed 1001)
-and of the 4M-triangle terrain (3840x2160, 4096^2 texture, seed 1003) are
-timed too and reported under "soup" and "terrain"; --workload soup|terrain
-makes one of them the headline (with its CPU baseline).
+The plain frames of the near-clipped textured cube (800x600, depth cleared
+to 1.0), of the 1M-triangle soup (1920x1080, colour FS, seed 1001) and of
+the 4M-triangle terrain (3840x2160, 4096^2 texture, seed 1003) are timed too,
+each with its CPU baseline, and reported under "cube", "soup" and "terrain";
+--workload cube|soup|terrain makes one of them the headline.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch|soup|terrain] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch|soup|terrain|cube] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -30,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+PEAK_CLOCK_HZ = 2.4e9  # MI355X peak engine clock
 METRIC = "MSAA samples/s + triangles/s (batched vs plain GPU vs CPU); p99 render latency"
 
 
@@ -38,7 +40,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("batch", "soup", "terrain"), default="batch")
+    ap.add_argument("--workload", choices=("batch", "soup", "terrain", "cube"), default="batch")
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--tris", type=int, default=1_000_000, help="soup triangles")
     ap.add_argument("--width", type=int, default=1920)
@@ -54,7 +56,8 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson request rate (1/s)")
     ap.add_argument("--serve-requests", type=int, default=4000)
-    ap.add_argument("--cpu-sample-tris", type=int, default=300_000)
+    ap.add_argument("--cpu-sample-tris", type=int, default=150_000,
+                    help="triangles in the soup and terrain CPU samples")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy (HBM counter calibration)")
     return ap.parse_args()
@@ -101,22 +104,30 @@ def _cpu_model():
     return "unknown"
 
 
-def oracle_frame_seconds(sc, n_tris=None, threads: int = 1) -> float:
+def oracle_frame_seconds(sc, n_tris=None, threads: int = 1, stride: int = 1) -> float:
     """The oracle (C++ restatement of the reference; single-threaded like the
     reference unless threads > 1, which renders the same bits tile-parallel)
-    on sc's first n_tris triangles (all by default): seconds for draw +
-    framebuffer()."""
+    on sc's first n_tris triangles (all by default), or on every stride-th
+    triangle: seconds for draw + framebuffer()."""
     from oracle.oracle import OracleRenderer
 
     n = sc.ntri if n_tris is None else min(n_tris, sc.ntri)
+    idx = sc.idx[: 3 * n]
+    if stride > 1:
+        idx = np.ascontiguousarray(idx.reshape(-1, 3)[::stride].reshape(-1))
     r = OracleRenderer(sc.width, sc.height, clear_depth=sc.clear_depth)
     if sc.texture is not None:
         r.bind_texture(0, sc.texture)
     r.set_uniforms(sc.world, sc.view, sc.proj)
     t0 = time.perf_counter()
-    r.render(sc.pos, sc.color, sc.uv, sc.idx[: 3 * n], sc.fs, threads=threads)
+    r.render(sc.pos, sc.color, sc.uv, idx, sc.fs, threads=threads)
     r.framebuffer()
     return time.perf_counter() - t0
+
+
+def host_threads() -> int:
+    """Host threads for the tile-parallel oracle: the box's CPU share (16)."""
+    return max(1, min(16, os.cpu_count() or 1))
 
 
 def raster_work(ctx, run_once, samples: int, width: int = 64, height: int = 64) -> dict:
@@ -143,18 +154,48 @@ def raster_work(ctx, run_once, samples: int, width: int = 64, height: int = 64) 
             "shades_per_pixel": st["raster_shading_tasks"] / max(pixels, 1)}
 
 
-def roofline(alg_bytes: int, raster_ms: float, traffic):
+def load_counters(name: str):
+    """Per-launch instruction counts of the raster kernel measured with
+    rocprofv3 --pmc (SQ_INSTS_VALU, ...); None if not measured."""
+    p = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return None
+
+
+def roofline(alg_bytes: int, raster_ms: float, traffic, counters=None):
+    """HBM roofline of the raster kernel, and beside it how much of the
+    chip's VALU issue rate its vector instructions use: each CU issues at
+    most one wave64 VALU instruction per clock (4 SIMDs, 4 cycles each), so
+    valu_issue_frac = VALU wave-instructions / (CUs x clock x duration)."""
     achieved = alg_bytes / (raster_ms * 1e-3) / 1e9
-    return {"bound": "hbm", "kernel": "k_raster", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes_per_launch": alg_bytes,
-            "raster_ms": raster_ms}
+    r = {"bound": "hbm", "kernel": "k_raster", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes_per_launch": alg_bytes,
+         "raster_ms": raster_ms}
+    if counters:
+        import torch
+
+        n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+        cycles = n_cu * PEAK_CLOCK_HZ * raster_ms * 1e-3
+        r["valu_issue_frac"] = counters["SQ_INSTS_VALU"] / cycles
+        r["salu_issue_frac"] = counters["SQ_INSTS_SALU"] / cycles
+        r["issue_basis"] = (f"{counters['SQ_INSTS_VALU']:.3e} VALU and {counters['SQ_INSTS_SALU']:.3e} SALU "
+                            f"wave-instructions per launch ({counters['source']}) over {n_cu} CUs x "
+                            f"{PEAK_CLOCK_HZ / 1e9:.1f} GHz x {raster_ms:.3f} ms; the nominal peak is one wave64 "
+                            f"instruction per CU per cycle, which a value near or above 1 reaches")
+    return r
 
 
 # ------------------------------------------------------------ plain frames
-def bench_frame(sc, steps, warmup, cpu: str | None, traffic_name: str, cpu_sample_tris: int = 300_000):
+def bench_frame(sc, steps, warmup, cpu: str | None, traffic_name: str | None, cpu_sample_tris: int = 300_000,
+                counters_name: str | None = None):
     """One scene drawn and resolved per step through the plain API.
     cpu: None, "sample" (first cpu_sample_tris triangles, time scaled --
-    for i.i.d. soups) or "full" (the whole frame once)."""
+    for i.i.d. soups), "stride" (every k-th triangle, about cpu_sample_tris
+    of them, time scaled -- for ordered meshes) or "full" (the whole frame
+    once); beside the 1-thread figure the whole frame is rendered
+    tile-parallel on the host's threads."""
     import torch
 
     from rusterizer_amd.render import Context
@@ -209,12 +250,26 @@ def bench_frame(sc, steps, warmup, cpu: str | None, traffic_name: str, cpu_sampl
         "raster_work": work,
         "clipped_share": stats["clipped"] / max(sc.ntri, 1),
         "warmup_replays": [(w["replays"], w["overflow_bits"]) for w in warm],
-        "roofline": roofline(sc.algorithmic_bytes(), raster_ms, load_traffic(traffic_name)),
+        "roofline": roofline(sc.algorithmic_bytes(), raster_ms, load_traffic(traffic_name) if traffic_name else None,
+                             load_counters(counters_name) if counters_name else None),
     }
-    if cpu == "sample":
+    sample_kw, scale = {}, 1.0  # the CPU sample and the factor that scales its time to a frame
+    if cpu == "stride":
+        k = max(1, -(-sc.ntri // cpu_sample_tris))
+        n = len(range(0, sc.ntri, k))
+        sample_kw, scale = {"stride": k}, sc.ntri / max(n, 1)
+        t = oracle_frame_seconds(sc, **sample_kw)
+        frame_s = t * scale
+        res["cpu_baseline"] = {
+            "value": sc.samples / frame_s, "unit": "MSAA samples/s", "cores": 1, "kind": "port",
+            "sample": f"every {k}th triangle ({n} of {sc.ntri}) rendered in {t:.2f} s on 1 host thread; "
+                      f"frame time scaled by {sc.ntri / max(n, 1):.2f}",
+            "triangles_per_s": sc.ntri / frame_s, "host_cpu": _cpu_model(), "host_nproc": os.cpu_count()}
+    elif cpu == "sample":
         n = min(cpu_sample_tris, sc.ntri)
-        t = oracle_frame_seconds(sc, n)
-        frame_s = t * sc.ntri / max(n, 1)
+        sample_kw, scale = {"n_tris": n}, sc.ntri / max(n, 1)
+        t = oracle_frame_seconds(sc, **sample_kw)
+        frame_s = t * scale
         res["cpu_baseline"] = {
             "value": sc.samples / frame_s, "unit": "MSAA samples/s", "cores": 1, "kind": "port",
             "sample": f"first {n} of {sc.ntri} triangles rendered in {t:.2f} s on 1 host thread; "
@@ -226,6 +281,12 @@ def bench_frame(sc, steps, warmup, cpu: str | None, traffic_name: str, cpu_sampl
             "value": sc.samples / t, "unit": "MSAA samples/s", "cores": 1, "kind": "port",
             "sample": f"the whole frame ({sc.ntri} triangles) rendered in {t:.2f} s on 1 host thread",
             "triangles_per_s": sc.ntri / t, "host_cpu": _cpu_model(), "host_nproc": os.cpu_count()}
+    if cpu is not None:
+        nt = host_threads()
+        t = oracle_frame_seconds(sc, threads=nt, **sample_kw)
+        res["cpu_baseline"]["multithread"] = {"value": sc.samples / (t * scale), "cores": nt, "seconds": t,
+                                              "sample": "the same sample, tile-parallel (same bits)"}
+        res["vs_cpu"] = res["samples_per_s"] / res["cpu_baseline"]["value"]
     mesh.destroy()
     ctx.close()
     return res
@@ -238,6 +299,14 @@ def soup_scene_of(a):
     sc = scenes.soup_scene(a.tris, a.width, a.height, seed=seed)
     return sc, (f"triangle soup {sc.ntri} tris, {sc.width}x{sc.height}, 4x MSAA, colour FS, "
                 f"f32 depth LESS, seed {seed}")
+
+
+def cube_scene_of():
+    from rusterizer_amd import scenes
+
+    sc = scenes.cube_scene(800, 600, clear_depth=1.0)
+    return sc, ("cube 12 tris (mesh::cube(1.0)), 800x600, 256^2 checkerboard, 60 degree camera with the near "
+                "plane cutting the cube, 4x MSAA, bilinear texture FS, f32 depth cleared to 1.0, LESS")
 
 
 def terrain_scene_of(a):
@@ -331,7 +400,8 @@ def bench_batch(a, steps, warmup, with_cpu, with_plain, tune, scs, seed, t_gen):
         "raster_work": work,
         "first_batch_replays": (first_stats["replays"], first_stats["overflow_bits"]),
         "setup_s": {"generate": t_gen, "upload": t_up},
-        "roofline": roofline(alg_bytes, prof["raster"] / prof["frames"], load_traffic("traffic_raster_batch.json")),
+        "roofline": roofline(alg_bytes, prof["raster"] / prof["frames"], load_traffic("traffic_raster_batch.json"),
+                             load_counters("counters_raster_batch.json")),
         "samples_per_batch": samples,
         "tris_per_batch": tris,
     }
@@ -458,7 +528,7 @@ def cpu_batch(scs, budget_s: float):
         samples += sc.samples
         tris += sc.ntri
         done += 1
-    nt = max(1, min(16, os.cpu_count() or 1))
+    nt = host_threads()
     secs_mt = sum(oracle_frame_seconds(sc, threads=nt) for sc in scs[:done])
     return {"value": samples / secs, "unit": "MSAA samples/s", "cores": 1, "kind": "port",
             "sample": f"first {done} of {len(scs)} scenes ({tris} tris, {samples} samples) rendered whole "
@@ -481,17 +551,23 @@ def main():
         copy_probe()
         return
     with_cpu = not a.no_cpu
-    if a.workload in ("soup", "terrain"):
+    if a.workload in ("soup", "terrain", "cube"):
         if a.workload == "soup":
             sc, desc = soup_scene_of(a)
             s = bench_frame(sc, a.steps, a.warmup, "sample" if with_cpu else None, "traffic_raster_soup1m.json",
-                            a.cpu_sample_tris)
+                            a.cpu_sample_tris, "counters_raster_soup1m.json")
             cfg = {"workload": desc, "triangles": sc.ntri, "width": sc.width, "height": sc.height, "msaa": 4}
-        else:
+        elif a.workload == "terrain":
             sc, desc = terrain_scene_of(a)
-            s = bench_frame(sc, a.steps, a.warmup, "full" if with_cpu else None, "traffic_raster_terrain.json")
+            s = bench_frame(sc, a.steps, a.warmup, "full" if with_cpu else None, "traffic_raster_terrain.json",
+                            a.cpu_sample_tris, "counters_raster_terrain4m.json")
             cfg = {"workload": desc, "triangles": sc.ntri, "width": sc.width, "height": sc.height, "msaa": 4,
                    "texture": 4096}
+        else:
+            sc, desc = cube_scene_of()
+            s = bench_frame(sc, a.steps, a.warmup, "full" if with_cpu else None, None)
+            cfg = {"workload": desc, "triangles": sc.ntri, "width": sc.width, "height": sc.height, "msaa": 4,
+                   "texture": 256}
         line = {"metric": METRIC, "value": s["samples_per_s"], "unit": "MSAA samples/s", "n_gpus": 1,
                 "steps": a.steps, "warmup": a.warmup, "ms_per_step": s["ms_per_frame"], "higher_is_better": True,
                 "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": cfg,
@@ -502,15 +578,23 @@ def main():
     scs, bseed, t_gen = batch_scenes_of(a)
     log(f"generated {len(scs)} scenes in {t_gen:.1f} s")
     b = bench_batch(a, a.steps, a.warmup, with_cpu, not a.no_plain, not a.no_tune, scs, bseed, t_gen)
-    soup = terrain = serve = None
+    cube = soup = terrain = serve = None
     if a.no_extra:
         a.no_serve = True
     else:
+        cpu_mode = (lambda m: m if with_cpu else None)
+        cube_sc, desc = cube_scene_of()
+        cube = dict(workload=desc, **bench_frame(cube_sc, FRAME_REPS, 2, cpu_mode("full"), None))
+        log(f"cube: {cube['ms_per_frame']:.3f} ms per frame")
         soup_sc, desc = soup_scene_of(a)
-        soup = dict(workload=desc, **bench_frame(soup_sc, FRAME_REPS, 2, None, "traffic_raster_soup1m.json"))
+        soup = dict(workload=desc, **bench_frame(soup_sc, FRAME_REPS, 2, cpu_mode("sample"),
+                                                 "traffic_raster_soup1m.json", a.cpu_sample_tris,
+                                                 "counters_raster_soup1m.json"))
         log(f"soup: {soup['ms_per_frame']:.2f} ms per frame")
         terr_sc, desc = terrain_scene_of(a)
-        terrain = dict(workload=desc, **bench_frame(terr_sc, FRAME_REPS, 2, None, "traffic_raster_terrain.json"))
+        terrain = dict(workload=desc, **bench_frame(terr_sc, FRAME_REPS, 2, cpu_mode("full"),
+                                                    "traffic_raster_terrain.json", a.cpu_sample_tris,
+                                                    "counters_raster_terrain4m.json"))
         log(f"terrain: {terrain['ms_per_frame']:.2f} ms per frame")
     if not a.no_serve:
         from rusterizer_amd import scenes as S
@@ -533,6 +617,7 @@ def main():
         "roofline": b["roofline"],
         "cpu_baseline": b.get("cpu_baseline"),
         "batch": b,
+        "cube": cube,
         "soup": soup,
         "terrain": terrain,
         "serve": serve,
