@@ -32,6 +32,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 PEAK_CLOCK_HZ = 2.4e9  # MI355X peak engine clock
+VALU_PER_CU_CYCLE = 2  # wave64 FP32 VALU instructions: 128 lanes per CU per clock
 METRIC = "MSAA samples/s + triangles/s (batched vs plain GPU vs CPU); p99 render latency"
 
 
@@ -50,8 +51,11 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-scene leg (batch workload)")
     ap.add_argument("--no-tune", action="store_true", help="use the default batch tuning")
     ap.add_argument("--tuning", default=None,
-                    help="fixed batch tuning bin_target,raster_per_cu,use_graph (e.g. 64,0,1); skips the autotuner")
+                    help="fixed batch tuning bin_target,raster_per_cu,use_graph[,fine_tile] (e.g. 64,0,1,16); "
+                         "skips the autotuner")
     ap.add_argument("--no-extra", action="store_true", help="batch workload only: skip the soup/terrain/serve legs")
+    ap.add_argument("--fine-tile", type=int, default=None,
+                    help="cube/soup/terrain workloads: side of the binned screen tiles (8, 16 or 32)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson request rate (1/s)")
@@ -166,9 +170,11 @@ def load_counters(name: str):
 
 def roofline(alg_bytes: int, raster_ms: float, traffic, counters=None):
     """HBM roofline of the raster kernel, and beside it how much of the
-    chip's VALU issue rate its vector instructions use: each CU issues at
-    most one wave64 VALU instruction per clock (4 SIMDs, 4 cycles each), so
-    valu_issue_frac = VALU wave-instructions / (CUs x clock x duration)."""
+    chip's instruction issue rates it uses. VALU: 157.3 TFLOP/s of FP32 FMA
+    over 256 CUs at 2.4 GHz is 128 lanes per CU per clock, two wave64 VALU
+    instructions per CU-cycle; SALU: one per CU-cycle (one scalar unit per
+    CU). valu_issue_frac = VALU wave-instructions / (2 x CUs x clock x
+    duration), salu_issue_frac likewise with 1."""
     achieved = alg_bytes / (raster_ms * 1e-3) / 1e9
     r = {"bound": "hbm", "kernel": "k_raster", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
          "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes_per_launch": alg_bytes,
@@ -178,18 +184,18 @@ def roofline(alg_bytes: int, raster_ms: float, traffic, counters=None):
 
         n_cu = torch.cuda.get_device_properties(0).multi_processor_count
         cycles = n_cu * PEAK_CLOCK_HZ * raster_ms * 1e-3
-        r["valu_issue_frac"] = counters["SQ_INSTS_VALU"] / cycles
+        r["valu_issue_frac"] = counters["SQ_INSTS_VALU"] / (VALU_PER_CU_CYCLE * cycles)
         r["salu_issue_frac"] = counters["SQ_INSTS_SALU"] / cycles
         r["issue_basis"] = (f"{counters['SQ_INSTS_VALU']:.3e} VALU and {counters['SQ_INSTS_SALU']:.3e} SALU "
                             f"wave-instructions per launch ({counters['source']}) over {n_cu} CUs x "
-                            f"{PEAK_CLOCK_HZ / 1e9:.1f} GHz x {raster_ms:.3f} ms; the nominal peak is one wave64 "
-                            f"instruction per CU per cycle, which a value near or above 1 reaches")
+                            f"{PEAK_CLOCK_HZ / 1e9:.1f} GHz x {raster_ms:.3f} ms; peaks {VALU_PER_CU_CYCLE} VALU "
+                            f"and 1 SALU wave-instruction per CU-cycle")
     return r
 
 
 # ------------------------------------------------------------ plain frames
 def bench_frame(sc, steps, warmup, cpu: str | None, traffic_name: str | None, cpu_sample_tris: int = 300_000,
-                counters_name: str | None = None):
+                counters_name: str | None = None, fine_tile: int | None = None):
     """One scene drawn and resolved per step through the plain API.
     cpu: None, "sample" (first cpu_sample_tris triangles, time scaled --
     for i.i.d. soups), "stride" (every k-th triangle, about cpu_sample_tris
@@ -202,6 +208,8 @@ def bench_frame(sc, steps, warmup, cpu: str | None, traffic_name: str | None, cp
 
     ctx = Context(sc.width, sc.height)
     ctx.set_clear_depth(sc.clear_depth)
+    if fine_tile:
+        ctx.set_fine_tile(fine_tile)
     if sc.texture is not None:
         ctx.bind_texture(0, sc.texture)
     ctx.set_uniforms(sc.world, sc.view, sc.proj)
@@ -349,8 +357,8 @@ def bench_batch(a, steps, warmup, with_cpu, with_plain, tune, scs, seed, t_gen):
         batch.run(ctx, items, table)
 
     if a.tuning:
-        bt, rp, g = (int(v) for v in a.tuning.split(","))
-        ctx.batch_tuning(bt, rp, bool(g))
+        v = [int(x) for x in a.tuning.split(",")]
+        ctx.batch_tuning(v[0], v[1], bool(v[2]), fine_tile=v[3] if len(v) > 3 else None)
         tune = False
     once()  # first batch: buffers grow to fit
     first_stats = ctx.frame_stats()
@@ -435,7 +443,7 @@ def bench_serve(kinds: dict, tuning: dict | None, rate: float, n: int, seed: int
     from rusterizer_amd.serve import Catalog, Server, make_requests
 
     root = Context(64, 64)
-    cat = Catalog(root, kinds)
+    cat = Catalog(root, kinds, tuning=tuning)
     srv = Server(cat, slots=slots, budget_bytes=budget, max_batch=64, tuning=tuning)
     try:
         warm = make_requests(cat, min(400, n), rate, SERVE_WEIGHTS, seed + 100)
@@ -555,17 +563,17 @@ def main():
         if a.workload == "soup":
             sc, desc = soup_scene_of(a)
             s = bench_frame(sc, a.steps, a.warmup, "sample" if with_cpu else None, "traffic_raster_soup1m.json",
-                            a.cpu_sample_tris, "counters_raster_soup1m.json")
+                            a.cpu_sample_tris, "counters_raster_soup1m.json", a.fine_tile)
             cfg = {"workload": desc, "triangles": sc.ntri, "width": sc.width, "height": sc.height, "msaa": 4}
         elif a.workload == "terrain":
             sc, desc = terrain_scene_of(a)
             s = bench_frame(sc, a.steps, a.warmup, "full" if with_cpu else None, "traffic_raster_terrain.json",
-                            a.cpu_sample_tris, "counters_raster_terrain4m.json")
+                            a.cpu_sample_tris, "counters_raster_terrain4m.json", a.fine_tile)
             cfg = {"workload": desc, "triangles": sc.ntri, "width": sc.width, "height": sc.height, "msaa": 4,
                    "texture": 4096}
         else:
             sc, desc = cube_scene_of()
-            s = bench_frame(sc, a.steps, a.warmup, "full" if with_cpu else None, None)
+            s = bench_frame(sc, a.steps, a.warmup, "full" if with_cpu else None, None, fine_tile=a.fine_tile)
             cfg = {"workload": desc, "triangles": sc.ntri, "width": sc.width, "height": sc.height, "msaa": 4,
                    "texture": 256}
         line = {"metric": METRIC, "value": s["samples_per_s"], "unit": "MSAA samples/s", "n_gpus": 1,

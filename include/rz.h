@@ -247,6 +247,36 @@ int rz_render_noalloc(void *stream, uint32_t width, uint32_t height, float clear
 /* Coarse bin side in pixels: a multiple of 16 up to 256 with at most 256 bins
  * on the screen, or 0 = automatic. Not inside a frame. */
 int rz_set_coarse_bin(rz_ctx *ctx, uint32_t pixels);
+/* Device memory in bytes (no reference counterpart: lets a server admit
+ * work against what the device really holds):
+ *   out[0]  held by ctx now: its pipeline buffers, job tables and outputs
+ *           (buffers grow to fit the jobs run so far and are kept);
+ *   out[1]  the meshes and textures resident in ctx's group;
+ *   out[2]  what ctx will hold once the frame or batch being recorded has
+ *           run, by the same estimate the library sizes its buffers with
+ *           (a job that outgrows the estimate grows them further when it
+ *           runs, which out[0] then shows); out[0] when nothing is recorded. */
+int rz_memory(const rz_ctx *ctx, uint64_t out[3]);
+/* What ctx would hold (as out[2] above) after running a batch of exactly
+ * these n items with its current tuning, without recording anything: lets
+ * a server size a batch to its memory budget before it records it. */
+int rz_batch_need(const rz_ctx *ctx, const rz_batch_item *items, uint32_t n, uint64_t *bytes);
+/* Frees ctx's pipeline buffers (they grow again with the next job, which
+ * then re-captures its graph): lets a server give memory back under
+ * pressure. Between jobs; meshes, textures and tables stay. */
+int rz_trim(rz_ctx *ctx);
+/* Side in pixels (8, 16 or 32; 0 = 16) of the screen tiles whose primitive
+ * lists binning builds and the raster walks (no reference counterpart: the
+ * reference walks each triangle's bounding box; it tunes work per list
+ * against list count). The raster still works on 16x16 tiles, one wave per
+ * 8x8 quadrant: with 8 each wave walks its own quadrant's list, with 32 four
+ * raster tiles share one list. A scene whose coarse bins cannot hold tiles of
+ * that side (more than 256 tiles per bin, or a side that does not divide the
+ * bin) uses 16. Results do not depend on it. Between jobs; frames and
+ * batches. */
+int rz_set_fine_tile(rz_ctx *ctx, uint32_t side);
+/* The side last set (16 by default). */
+uint32_t rz_get_fine_tile(const rz_ctx *ctx);
 /* Enable (1) / disable (0) per-stage HIP event timing; enabling resets sums. */
 int rz_profile(rz_ctx *ctx, int enable);
 /* Summed milliseconds per stage (RZ_NSTAGES floats) and the number of frames

@@ -161,6 +161,11 @@ extern "C" {
 
     // tuning and timing
     pub fn rz_set_coarse_bin(ctx: *mut rz_ctx, pixels: u32) -> c_int;
+    pub fn rz_memory(ctx: *const rz_ctx, out: *mut u64) -> c_int;
+    pub fn rz_batch_need(ctx: *const rz_ctx, items: *const rz_batch_item, n: u32, bytes: *mut u64) -> c_int;
+    pub fn rz_trim(ctx: *mut rz_ctx) -> c_int;
+    pub fn rz_set_fine_tile(ctx: *mut rz_ctx, side: u32) -> c_int;
+    pub fn rz_get_fine_tile(ctx: *const rz_ctx) -> u32;
     pub fn rz_profile(ctx: *mut rz_ctx, enable: c_int) -> c_int;
     pub fn rz_profile_read(ctx: *const rz_ctx, ms: *mut f32, frames: *mut u32) -> c_int;
 }

@@ -71,6 +71,11 @@ SIGNATURES = {
     "rz_batch_tuning": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_int]),
     "rz_batch_get_tuning": (C.c_int, [_vp, _u32]),
     "rz_set_coarse_bin": (C.c_int, [_vp, C.c_uint32]),
+    "rz_memory": (C.c_int, [_vp, C.POINTER(C.c_uint64)]),
+    "rz_batch_need": (C.c_int, [_vp, _vp, C.c_uint32, C.POINTER(C.c_uint64)]),
+    "rz_trim": (C.c_int, [_vp]),
+    "rz_set_fine_tile": (C.c_int, [_vp, C.c_uint32]),
+    "rz_get_fine_tile": (C.c_uint32, [_vp]),
     "rz_profile": (C.c_int, [_vp, C.c_int]),
     "rz_profile_read": (C.c_int, [_vp, _f, _u32]),
     "rz_render_noalloc": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_float, _f, _f, _f, _vp, _vp, _vp,

@@ -1,13 +1,17 @@
 """Autotuner of the batched path.
 
-The knobs (rz_batch_tuning): the largest number of coarse bins per scene
-when the bin side is chosen automatically (the side is the smallest power of
-two >= 16 px giving at most that many bins; the raster tile is fixed at
-16x16 px, so this sets how many raster tiles one coarse bin holds), the
-number of persistent raster workgroups per CU (0 = one workgroup per tile),
-and whether the batch replays its captured HIP graph. Every setting renders
-bit-identical images; only the time changes. The tuner times each candidate
-on the caller's real batch and keeps the fastest.
+The knobs: the side of the screen tiles whose primitive lists binning
+builds and the raster walks (rz_set_fine_tile: 8, 16 or 32 px); the bin
+capacity, i.e. the largest number of coarse bins per scene when the bin side
+is chosen automatically (the side is the smallest power of two >= 16 px
+giving at most that many bins, so it sets how much screen -- and how many
+primitives -- one coarse bin holds; 8x8 tiles cap the side at 128 px); the
+number of persistent raster workgroups per CU (0 = one workgroup per tile);
+and whether the batch replays its captured HIP graph (rz_batch_tuning).
+Bin lists have no fixed capacity: binning counts, scans and then writes
+each list at its exact length. Every setting renders bit-identical images;
+only the time changes. The tuner times each candidate on the caller's real
+batch and keeps the fastest.
 """
 from __future__ import annotations
 
@@ -16,13 +20,14 @@ import json
 import os
 import time
 
+FINE_TILES = (8, 16, 32)
 BIN_TARGETS = (16, 64, 256)
-RASTER_PER_CU = (0, 4, 8)
+RASTER_PER_CU = (0, 4)
 
 
-def candidates(bin_targets=BIN_TARGETS, raster_per_cu=RASTER_PER_CU, graph=(True,)):
-    for bt, rp, g in itertools.product(bin_targets, raster_per_cu, graph):
-        yield dict(bin_target=bt, raster_per_cu=rp, use_graph=g)
+def candidates(fine_tiles=FINE_TILES, bin_targets=BIN_TARGETS, raster_per_cu=RASTER_PER_CU, graph=(True,)):
+    for ft, bt, rp, g in itertools.product(fine_tiles, bin_targets, raster_per_cu, graph):
+        yield dict(fine_tile=ft, bin_target=bt, raster_per_cu=rp, use_graph=g)
 
 
 def signature(scene_sizes, tris: int) -> str:

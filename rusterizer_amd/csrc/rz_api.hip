@@ -59,6 +59,7 @@ struct rz_pool {
     int device = 0;
     std::vector<Texture> textures;
     std::vector<rz_ctx*> ctxs;
+    uint64_t resident_bytes = 0;  // device bytes of the group's meshes and textures
 };
 
 struct rz_mesh {
@@ -70,6 +71,7 @@ struct rz_mesh {
     uint32_t nv = 0;
     uint32_t ntri = 0;
     uint32_t pending = 0;  // recorded draws not yet finished, over all contexts
+    uint64_t bytes = 0;    // device bytes of its streams
 };
 
 namespace {
@@ -110,6 +112,7 @@ struct rz_ctx {
     float next_clear_depth = FLT_MAX;
     Mat3 uni{};
     uint32_t coarse_req = 0;  // plain frame: coarse bin side, 0 = automatic
+    uint32_t fine_req = 16;   // fine-binning tile side asked for (8, 16 or 32; per scene where it fits)
     // batch tuning
     uint32_t bin_target = MAX_BINS;  // automatic bins: smallest power-of-2 side with at most this many bins
     uint32_t raster_per_cu = 0;      // persistent raster workgroups per CU (0: one workgroup per tile)
@@ -141,6 +144,7 @@ struct rz_ctx {
     DBuf<float> clip_out;  // MAXV * 10 floats per slot
     DBuf<PrimGeom> geom;
     DBuf<float> attrs;  // ATTR_STRIDE per prim
+    DBuf<float> uvat;   // UV_STRIDE per prim
     DBuf<uint4> pbin;
     DBuf<uint32_t> bin_counts, bin_offsets;
     DBuf<uint4> list;
@@ -234,6 +238,25 @@ uint32_t coarse_side(uint32_t W, uint32_t H, uint32_t req, uint32_t target) {
     return cb;
 }
 
+uint32_t nbins_of(uint32_t W, uint32_t H, uint32_t cb) { return ((W + cb - 1) / cb) * ((H + cb - 1) / cb); }
+
+// Coarse bin side and fine tile shift of a scene for the requested fine tile
+// side: the fine tile must divide the coarse bin and a bin may hold at most
+// MAX_BINS fine tiles. 8x8 tiles shrink an automatic bin to 128 px when the
+// scene still fits MAX_BINS bins; where the request cannot be met the scene
+// uses 16x16 tiles, which fit every bin side.
+void scene_bins(uint32_t W, uint32_t H, uint32_t req_cb, uint32_t target, uint32_t fine_req, uint32_t& cb,
+                uint32_t& fts) {
+    cb = coarse_side(W, H, req_cb, target);
+    fts = 4;
+    if (fine_req == 8) {
+        if (!req_cb && cb > 128 && nbins_of(W, H, 128) <= MAX_BINS) cb = 128;
+        if (cb <= 128) fts = 3;
+    } else if (fine_req == 32) {
+        if (cb % 32 == 0) fts = 5;
+    }
+}
+
 uint32_t clamp_u32(uint64_t v) { return v > 0xFFFFFFF0ull ? 0xFFFFFFF0u : (uint32_t)v; }
 
 // table block layout at capacities (dcap draws, scap scenes)
@@ -294,8 +317,10 @@ int build_job(rz_ctx* c, JobPlan& pl) {
         g.W = ps.W;
         g.H = ps.H;
         g.clear_depth = ps.clear_depth;
-        g.cb = c->recording == JobKind::Frame ? coarse_side(ps.W, ps.H, c->coarse_req, MAX_BINS)
-                                               : coarse_side(ps.W, ps.H, 0, c->bin_target);
+        if (c->recording == JobKind::Frame)
+            scene_bins(ps.W, ps.H, c->coarse_req, MAX_BINS, c->fine_req, g.cb, g.fts);
+        else
+            scene_bins(ps.W, ps.H, 0, c->bin_target, c->fine_req, g.cb, g.fts);
         g.nbx = (ps.W + g.cb - 1) / g.cb;
         g.nby = (ps.H + g.cb - 1) / g.cb;
         g.tile_base = (uint32_t)tiles;
@@ -333,7 +358,7 @@ int build_job(rz_ctx* c, JobPlan& pl) {
         tiles += (uint64_t)((ps.W + 15) / 16) * ((ps.H + 15) / 16);
         bins += (uint64_t)g.nbx * g.nby;
         pl.max_nb = std::max(pl.max_nb, g.nbx * g.nby);
-        pl.max_T = std::max(pl.max_T, (g.cb / 16) * (g.cb / 16));
+        pl.max_T = std::max(pl.max_T, (g.cb >> g.fts) * (g.cb >> g.fts));
     }
     if (d != nd) return fail(c, RZ_ERR_STATE, "draws are not grouped by scene");
     if (tiles > 0xFFFF0000ull || bins > 0xFFFF0000ull) return fail(c, RZ_ERR_LIMIT, "job too large");
@@ -354,7 +379,37 @@ struct Needs {
     uint64_t prims = 0, clip = 0, pairs = 0, fine = 0, mat = 0, fmat = 0;
 };
 
-int ensure_buffers(rz_ctx* c, const JobPlan& pl, const Needs& n_in) {
+// capacity grow() gives a buffer of cap elements asked for need
+size_t grown_cap(size_t cap, size_t need, bool allocated) {
+    if (cap >= need && allocated) return cap;
+    size_t n = std::max<size_t>(need, 1024);
+    return n + n / 4;
+}
+
+// Grows every pipeline buffer to fit a job (Real), or only adds up the bytes
+// the buffers would hold afterwards (Dry: nothing is allocated).
+struct RealGrow {
+    rz_ctx* c;
+    int rc = RZ_OK;
+    template <class T>
+    size_t operator()(DBuf<T>& b, size_t need) {
+        if (rc == RZ_OK) rc = grow(c, b, need);
+        return b.cap;
+    }
+};
+struct DryGrow {
+    uint64_t bytes = 0;
+    int rc = RZ_OK;
+    template <class T>
+    size_t operator()(DBuf<T>& b, size_t need) {
+        const size_t cap = grown_cap(b.cap, need, b.p != nullptr);
+        bytes += cap * sizeof(T);
+        return cap;
+    }
+};
+
+template <class G>
+int size_buffers(rz_ctx* c, const JobPlan& pl, const Needs& n_in, G& g) {
     Needs n = n_in;
     const uint64_t NT = pl.ntri;
     n.prims = std::max<uint64_t>(n.prims, NT + NT / 4 + 4096 + (uint64_t)pl.nscenes * CHUNK);
@@ -364,44 +419,109 @@ int ensure_buffers(rz_ctx* c, const JobPlan& pl, const Needs& n_in) {
     const uint64_t lim = 0xF0000000ull;
     if (n.prims > lim || n.pairs > lim || n.fine > lim || n.clip > lim)
         return fail(c, RZ_ERR_LIMIT, "job needs more than 2^32 pipeline entries");
-    int rc;
-    if ((rc = grow(c, c->cls, std::max<uint64_t>(NT, 256)))) return rc;
-    if ((rc = grow(c, c->nsub, c->cls.cap))) return rc;
-    if ((rc = grow(c, c->suboff, c->cls.cap))) return rc;
-    if ((rc = grow(c, c->geom, n.prims))) return rc;
-    if ((rc = grow(c, c->attrs, c->geom.cap * ATTR_STRIDE))) return rc;
-    if ((rc = grow(c, c->pbin, c->geom.cap))) return rc;
-    if ((rc = grow(c, c->tchunk_draw, c->cls.cap / TRI_ALIGN + 1))) return rc;
-    if ((rc = grow(c, c->pchunk_scene, c->geom.cap / CHUNK + 1))) return rc;
-    if (c->raster_stats &&
-        (rc = grow(c, c->rstat_parts, 5ull * ((uint64_t)pl.ntiles + 8 + 32ull * (uint32_t)c->n_cu))))
-        return rc;
-    if ((rc = grow(c, c->clip_tri, n.clip))) return rc;
-    if ((rc = grow(c, c->clip_redo, c->clip_tri.cap))) return rc;
-    if ((rc = grow(c, c->clip_draw, c->clip_tri.cap))) return rc;
-    if ((rc = grow(c, c->clip_out, c->clip_tri.cap * MAXV * 10))) return rc;
-    if ((rc = grow(c, c->list, n.pairs))) return rc;
+    const size_t cls = g(c->cls, std::max<uint64_t>(NT, 256));
+    g(c->nsub, cls);
+    g(c->suboff, cls);
+    const size_t geom = g(c->geom, n.prims);
+    g(c->attrs, geom * ATTR_STRIDE);
+    g(c->uvat, geom * UV_STRIDE);
+    g(c->pbin, geom);
+    g(c->tchunk_draw, cls / TRI_ALIGN + 1);
+    g(c->pchunk_scene, geom / CHUNK + 1);
+    if (c->raster_stats) g(c->rstat_parts, 5ull * ((uint64_t)pl.ntiles + 8 + 32ull * (uint32_t)c->n_cu));
+    const size_t clip = g(c->clip_tri, n.clip);
+    g(c->clip_redo, clip);
+    g(c->clip_draw, clip);
+    g(c->clip_out, clip * MAXV * 10);
+    const size_t list = g(c->list, n.pairs);
     // coarse matrix: sum over scenes of bins * chunks <= max_nb * (slots / CHUNK)
-    const uint64_t mat = std::max<uint64_t>(n.mat, (uint64_t)pl.max_nb * (c->geom.cap / CHUNK + 1));
-    if ((rc = grow(c, c->bin_counts, mat))) return rc;
-    if ((rc = grow(c, c->bin_offsets, c->bin_counts.cap))) return rc;
-    if ((rc = grow(c, c->bin_rng, std::max<uint64_t>(pl.nbins, 1)))) return rc;
-    if ((rc = grow(c, c->tile_rec, std::max<uint64_t>(pl.ntiles, 1)))) return rc;
-    if ((rc = grow(c, c->fpack, c->bin_rng.cap))) return rc;
-    if ((rc = grow(c, c->fpre, c->bin_rng.cap))) return rc;
-    if ((rc = grow(c, c->flist, n.fine))) return rc;
+    const uint64_t mat = std::max<uint64_t>(n.mat, (uint64_t)pl.max_nb * (geom / CHUNK + 1));
+    const size_t mat_cap = g(c->bin_counts, mat);
+    g(c->bin_offsets, mat_cap);
+    const size_t bins = g(c->bin_rng, std::max<uint64_t>(pl.nbins, 1));
+    g(c->tile_rec, 4 * std::max<uint64_t>(pl.ntiles, 1));
+    g(c->fpack, bins);
+    g(c->fpre, bins);
+    g(c->flist, n.fine);
     // fine-binning chunks: sum over bins of ceil(len / CHUNK) <= entries / CHUNK + bins
-    if ((rc = grow(c, c->fchunk_rec, c->list.cap / CHUNK + c->bin_rng.cap + 1))) return rc;
+    g(c->fchunk_rec, list / CHUNK + bins + 1);
     // fine matrix: sum over bins of T * ceil(len / CHUNK) <= max_T * (entries / CHUNK + bins)
-    const uint64_t fmat =
-        std::max<uint64_t>(n.fmat, (uint64_t)pl.max_T * (c->list.cap / CHUNK + c->bin_rng.cap + 1));
+    const uint64_t fmat = std::max<uint64_t>(n.fmat, (uint64_t)pl.max_T * (list / CHUNK + bins + 1));
     if (mat > lim || fmat > lim) return fail(c, RZ_ERR_LIMIT, "bin matrices exceed 2^32 entries");
-    if ((rc = grow(c, c->fine_counts, fmat))) return rc;
-    if ((rc = grow(c, c->fine_off, c->fine_counts.cap))) return rc;
-    const size_t scan_n = std::max({c->cls.cap, c->bin_counts.cap, c->fine_counts.cap});
-    if ((rc = grow(c, c->parts, scan_n / SCAN_TILE + 2))) return rc;
-    if ((rc = grow(c, c->parts64, c->bin_rng.cap / SCAN_TILE + 2))) return rc;
-    return RZ_OK;
+    const size_t fmat_cap = g(c->fine_counts, fmat);
+    g(c->fine_off, fmat_cap);
+    const size_t scan_n = std::max({cls, mat_cap, fmat_cap});
+    g(c->parts, scan_n / SCAN_TILE + 2);
+    g(c->parts64, bins / SCAN_TILE + 2);
+    return g.rc;
+}
+
+int ensure_buffers(rz_ctx* c, const JobPlan& pl, const Needs& n_in) {
+    RealGrow g{c};
+    return size_buffers(c, pl, n_in, g);
+}
+
+// Sizes of a job (no tables written): draw(i) gives the triangle count of
+// draw i, scene(i, W, H) the size of scene i.
+template <class DrawTris, class SceneSize>
+void plan_sizes(const rz_ctx* c, bool frame, uint32_t nd, uint32_t ns, DrawTris draw, SceneSize scene, JobPlan& pl) {
+    pl = JobPlan{};
+    pl.ndraws = nd;
+    pl.nscenes = ns;
+    uint64_t tri = 0, tiles = 0, bins = 0;
+    for (uint32_t i = 0; i < nd; ++i) {
+        const uint32_t n = draw(i);
+        tri += ((uint64_t)n + TRI_ALIGN - 1) / TRI_ALIGN * TRI_ALIGN;
+        pl.tris += n;
+    }
+    for (uint32_t i = 0; i < ns; ++i) {
+        uint32_t W, H, cb, fts;
+        scene(i, W, H);
+        if (frame)
+            scene_bins(W, H, c->coarse_req, MAX_BINS, c->fine_req, cb, fts);
+        else
+            scene_bins(W, H, 0, c->bin_target, c->fine_req, cb, fts);
+        const uint32_t nb = nbins_of(W, H, cb);
+        tiles += (uint64_t)((W + 15) / 16) * ((H + 15) / 16);
+        bins += nb;
+        pl.max_nb = std::max(pl.max_nb, nb);
+        pl.max_T = std::max(pl.max_T, (cb >> fts) * (cb >> fts));
+    }
+    pl.ntri = clamp_u32(tri);
+    pl.ntiles = clamp_u32(tiles);
+    pl.nbins = clamp_u32(bins);
+}
+
+// The plan of the recorded scenes and draws.
+void plan_job(const rz_ctx* c, JobPlan& pl) {
+    plan_sizes(
+        c, c->recording == JobKind::Frame, (uint32_t)c->draws.size(), (uint32_t)c->scenes.size(),
+        [&](uint32_t i) { return c->draws[i].mesh->ntri; },
+        [&](uint32_t i, uint32_t& W, uint32_t& H) {
+            W = c->scenes[i].W;
+            H = c->scenes[i].H;
+        },
+        pl);
+}
+
+template <class T>
+uint64_t held(const DBuf<T>& b) {
+    return b.p ? (uint64_t)b.cap * sizeof(T) : 0;
+}
+
+// device bytes of the context's pipeline buffers, outputs and tables
+uint64_t ctx_bytes(const rz_ctx* c) {
+    uint64_t n = held(c->cls) + held(c->nsub) + held(c->suboff) + held(c->parts) + held(c->parts64) +
+                 held(c->clip_tri) + held(c->clip_redo) + held(c->clip_draw) + held(c->clip_out) + held(c->geom) +
+                 held(c->attrs) + held(c->uvat) + held(c->pbin) + held(c->bin_counts) + held(c->bin_offsets) + held(c->list) +
+                 held(c->bin_rng) + held(c->tile_rec) + held(c->fchunk_rec) + held(c->tchunk_draw) +
+                 held(c->pchunk_scene) + held(c->rstat_parts) + held(c->fpack) + held(c->fpre) +
+                 held(c->fine_counts) + held(c->fine_off) + held(c->flist) + held(c->out) + held(c->msaa_color) +
+                 held(c->msaa_depth);
+    n += c->d_job ? c->job_bytes : 0;
+    n += c->d_runs ? sizeof(SceneRun) * c->scap : 0;
+    n += c->d_cnt ? sizeof(FrameCounters) : 0;
+    return n;
 }
 
 JobBufs job_bufs(const rz_ctx* c) {
@@ -421,6 +541,7 @@ JobBufs job_bufs(const rz_ctx* c) {
     b.clip_out = c->clip_out.p;
     b.geom = c->geom.p;
     b.attrs = c->attrs.p;
+    b.uvat = c->uvat.p;
     b.pbin = c->pbin.p;
     b.bin_counts = c->bin_counts.p;
     b.bin_offsets = c->bin_offsets.p;
@@ -446,7 +567,7 @@ JobBufs job_bufs(const rz_ctx* c) {
     b.fine_cap = clamp_u32(c->flist.cap);
     b.fmat_cap = clamp_u32(c->fine_counts.cap);
     b.bins_cap = clamp_u32(c->bin_rng.cap);
-    b.tile_cap = clamp_u32(c->tile_rec.cap);
+    b.tile_cap = clamp_u32(c->tile_rec.cap / 4);
     b.fchunk_cap = clamp_u32(c->fchunk_rec.cap);
     b.tchunk_cap = clamp_u32(c->tchunk_draw.cap);
     b.pchunk_cap = clamp_u32(c->pchunk_scene.cap);
@@ -685,6 +806,7 @@ int mesh_create(rz_ctx* c, const float* pos, const float* color, const float* uv
         if (!src || bytes == 0) return RZ_OK;
         hipError_t e = hipMalloc(dst, bytes);
         if (e != hipSuccess) return fail(c, RZ_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        m->bytes += bytes;
         e = hipMemcpy(*dst, src, bytes, kind);
         if (e != hipSuccess) return fail(c, RZ_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
         return RZ_OK;
@@ -718,6 +840,7 @@ int mesh_create(rz_ctx* c, const float* pos, const float* color, const float* uv
         delete m;
         return rc;
     }
+    c->pool->resident_bytes += m->bytes;
     *out = m;
     return RZ_OK;
 }
@@ -862,7 +985,7 @@ bool noalloc_layout(unsigned char* base, uint32_t ntri_pad, const SceneCfg& sc, 
     b.runs = cv.take<SceneRun>(1);
     b.cnt = cv.take<FrameCounters>(1);
     *maxidx = cv.take<uint32_t>(64);
-    const uint64_t nbins = (uint64_t)sc.nbx * sc.nby, T = (uint64_t)(sc.cb / 16) * (sc.cb / 16);
+    const uint64_t nbins = (uint64_t)sc.nbx * sc.nby, T = (uint64_t)(sc.cb >> sc.fts) * (sc.cb >> sc.fts);
     const uint64_t tri_cap = std::max<uint64_t>(ntri_pad, 256);
     const uint64_t prims = ntri_pad + ntri_pad / 4 + 4096 + CHUNK;
     const uint64_t clip = ntri_pad / 8 + 1024;
@@ -873,6 +996,7 @@ bool noalloc_layout(unsigned char* base, uint32_t ntri_pad, const SceneCfg& sc, 
     b.tchunk_draw = cv.take<uint32_t>(tri_cap / TRI_ALIGN + 1);
     b.geom = cv.take<PrimGeom>(prims);
     b.attrs = cv.take<float>(prims * ATTR_STRIDE);
+    b.uvat = cv.take<float>(prims * UV_STRIDE);
     b.pbin = cv.take<uint4>(prims);
     b.pchunk_scene = cv.take<uint32_t>(prims / CHUNK + 1);
     b.clip_tri = cv.take<uint32_t>(clip);
@@ -882,7 +1006,7 @@ bool noalloc_layout(unsigned char* base, uint32_t ntri_pad, const SceneCfg& sc, 
     b.bin_counts = cv.take<uint32_t>(mat);
     b.bin_offsets = cv.take<uint32_t>(mat);
     b.bin_rng = cv.take<uint2>(nbins);
-    b.tile_rec = cv.take<uint4>(ntiles);
+    b.tile_rec = cv.take<uint4>(4 * (size_t)ntiles);
     b.fpack = cv.take<unsigned long long>(nbins);
     b.fpre = cv.take<unsigned long long>(nbins);
     b.parts64 = cv.take<unsigned long long>(nbins / SCAN_TILE + 2);
@@ -977,7 +1101,7 @@ extern "C" int rz_render_noalloc(void* stream, uint32_t width, uint32_t height, 
     sc->W = width;
     sc->H = height;
     sc->clear_depth = clear_depth;
-    sc->cb = coarse_side(width, height, 0, MAX_BINS);
+    scene_bins(width, height, 0, MAX_BINS, 16, sc->cb, sc->fts);
     sc->nbx = (width + sc->cb - 1) / sc->cb;
     sc->nby = (height + sc->cb - 1) / sc->cb;
     sc->tri_begin = 0;
@@ -1094,6 +1218,7 @@ int rz_destroy(rz_ctx* c) {
     release(c->clip_out);
     release(c->geom);
     release(c->attrs);
+    release(c->uvat);
     release(c->pbin);
     release(c->bin_counts);
     release(c->bin_offsets);
@@ -1172,6 +1297,7 @@ int rz_bind_texture(rz_ctx* c, uint32_t index, const uint8_t* rgba8, uint32_t tw
         return fail(c, RZ_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
     }
     c->pool->textures.push_back(t);
+    c->pool->resident_bytes += bytes;
     return RZ_OK;
 }
 
@@ -1197,6 +1323,7 @@ int rz_mesh_destroy(rz_ctx* c, rz_mesh* m) {
     if (m->col) (void)hipFree(m->col);
     if (m->uv) (void)hipFree(m->uv);
     if (m->idx) (void)hipFree(m->idx);
+    c->pool->resident_bytes -= m->bytes;
     delete m;
     return RZ_OK;
 }
@@ -1254,6 +1381,113 @@ int rz_set_coarse_bin(rz_ctx* c, uint32_t pixels) {
     c->coarse_req = pixels;
     return RZ_OK;
 }
+
+// Bytes ctx would hold after running a job of plan pl (its buffers only grow).
+uint64_t job_bytes_after(const rz_ctx* c, const JobPlan& pl) {
+    rz_ctx* m = const_cast<rz_ctx*>(c);  // the dry sizing pass reads capacities only
+    const std::string err = m->err;
+    DryGrow g;
+    const int rc = size_buffers(m, pl, Needs{}, g);
+    m->err = err;
+    const uint64_t now = ctx_bytes(c);
+    if (rc != RZ_OK) return now;
+    // the pipeline buffers as the job sizes them, plus what sizing leaves alone
+    const uint64_t rest = held(c->out) + held(c->msaa_color) + held(c->msaa_depth) +
+                          (c->raster_stats ? 0 : held(c->rstat_parts));
+    uint64_t dcap = std::max<uint32_t>(c->dcap, 16), scap = std::max<uint32_t>(c->scap, 4);
+    while (dcap < pl.ndraws) dcap *= 2;
+    while (scap < pl.nscenes) scap *= 2;
+    auto al = [](uint64_t v) { return (v + 255) & ~(uint64_t)255; };
+    const uint64_t tables = al(sizeof(JobHead)) + al(sizeof(DrawDev) * dcap) + al(sizeof(Mat3) * dcap) +
+                            al(sizeof(SceneCfg) * scap) + sizeof(SceneRun) * scap + sizeof(FrameCounters);
+    return std::max(now, g.bytes + rest + tables);
+}
+
+int rz_memory(const rz_ctx* c, uint64_t out[3]) {
+    if (!c || !out) return RZ_ERR_ARG;
+    out[0] = ctx_bytes(c);
+    out[1] = c->pool->resident_bytes;
+    out[2] = out[0];
+    if (c->recording != JobKind::None && !c->in_flight && !c->scenes.empty()) {
+        JobPlan pl;
+        plan_job(c, pl);
+        out[2] = job_bytes_after(c, pl);
+    }
+    return RZ_OK;
+}
+
+int rz_trim(rz_ctx* c) {
+    if (!c) return RZ_ERR_ARG;
+    if (c->recording != JobKind::None || c->in_flight || !c->draws.empty())
+        return fail(c, RZ_ERR_STATE, "not between jobs");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    release(c->cls);
+    release(c->nsub);
+    release(c->suboff);
+    release(c->parts);
+    release(c->parts64);
+    release(c->clip_tri);
+    release(c->clip_redo);
+    release(c->clip_draw);
+    release(c->clip_out);
+    release(c->geom);
+    release(c->attrs);
+    release(c->uvat);
+    release(c->pbin);
+    release(c->bin_counts);
+    release(c->bin_offsets);
+    release(c->list);
+    release(c->bin_rng);
+    release(c->tile_rec);
+    release(c->fchunk_rec);
+    release(c->tchunk_draw);
+    release(c->pchunk_scene);
+    release(c->rstat_parts);
+    release(c->fpack);
+    release(c->fpre);
+    release(c->fine_counts);
+    release(c->fine_off);
+    release(c->flist);
+    release(c->out);
+    release(c->msaa_color);
+    release(c->msaa_depth);
+    c->tiles_cap = 0;
+    c->gen++;
+    return RZ_OK;
+}
+
+int rz_batch_need(const rz_ctx* c, const rz_batch_item* items, uint32_t n, uint64_t* bytes) {
+    if (!c || !bytes || (n && !items)) return RZ_ERR_ARG;
+    for (uint32_t i = 0; i < n; ++i)
+        if (!items[i].mesh || items[i].mesh->pool != c->pool || items[i].width == 0 || items[i].height == 0)
+            return RZ_ERR_ARG;
+    JobPlan pl;
+    plan_sizes(
+        c, false, n, n, [&](uint32_t i) { return items[i].mesh->ntri; },
+        [&](uint32_t i, uint32_t& W, uint32_t& H) {
+            W = items[i].width;
+            H = items[i].height;
+        },
+        pl);
+    *bytes = n ? job_bytes_after(c, pl) : ctx_bytes(c);
+    return RZ_OK;
+}
+
+int rz_set_fine_tile(rz_ctx* c, uint32_t side) {
+    if (!c) return RZ_ERR_ARG;
+    if (side != 0 && side != 8 && side != 16 && side != 32)
+        return fail(c, RZ_ERR_ARG, "fine tile side must be 8, 16 or 32 (0 = 16)");
+    if (c->recording != JobKind::None || c->in_flight || !c->draws.empty())
+        return fail(c, RZ_ERR_STATE, "not between jobs");
+    c->fine_req = side ? side : 16;
+    return RZ_OK;
+}
+
+uint32_t rz_get_fine_tile(const rz_ctx* c) { return c ? c->fine_req : 0u; }
 
 int rz_profile(rz_ctx* c, int enable) {
     if (!c) return RZ_ERR_ARG;

@@ -23,7 +23,8 @@ constexpr uint32_t CLIP_GROUPS = 8;   // clipped triangles per k_clip wave (8 la
 constexpr uint32_t CHUNK = 256;       // prims per binning chunk (one wave)
 constexpr uint32_t MAX_BINS = 256;    // coarse bins per scene (one binning thread each)
 constexpr uint32_t TRI_ALIGN = 256;   // draws start at multiples of this in the triangle index space
-constexpr uint32_t ATTR_STRIDE = 12;  // floats of attributes per prim
+constexpr uint32_t ATTR_STRIDE = 12;  // floats of colour attributes per prim (3 x RGBA)
+constexpr uint32_t UV_STRIDE = 6;     // floats of texture coordinates per prim (3 x uv)
 constexpr int SCAN_ITEMS = 8;
 constexpr uint32_t SCAN_TILE = 256 * SCAN_ITEMS;
 constexpr uint32_t NO_TILE = 0xFFFFFFFFu;
@@ -95,7 +96,8 @@ struct SceneCfg {
     uint32_t tile_base;       // first global 16x16 raster tile of the scene
     uint32_t bin_base;        // first global coarse bin of the scene
     uint32_t tri_begin, tri_end;  // padded triangle range of the scene's draws
-    uint32_t pad0, pad1;
+    uint32_t fts;             // fine-binning tile side = 1 << fts pixels (8, 16 or 32; divides cb)
+    uint32_t pad1;
     uint32_t* out;            // resolved ARGB, W*H words (device)
     uint32_t* msaa_color;     // optional W*H*4 pre-resolve colours
     float* msaa_depth;        // optional W*H*4 pre-resolve depths
@@ -151,7 +153,8 @@ struct JobBufs {
     uint32_t* clip_draw;  // its draw
     float* clip_out;
     PrimGeom* geom;
-    float* attrs;
+    float* attrs;  // colour prims: ATTR_STRIDE floats each
+    float* uvat;   // textured prims: UV_STRIDE floats each
     uint4* pbin;
     // coarse bins
     uint32_t* bin_counts;
@@ -164,7 +167,7 @@ struct JobBufs {
     uint32_t* fine_counts;
     uint32_t* fine_off;
     uint2* flist;
-    uint4* tile_rec;    // per raster tile: {scene, list begin, list end, 0}
+    uint4* tile_rec;    // per raster tile and 8x8 quadrant q, at 4 * tile + q: {scene, list begin, list end, 0}
     uint2* fchunk_rec;  // per fine-binning chunk: {global coarse bin, scene}
     uint32_t* tchunk_draw;   // per TRI_ALIGN-triangle chunk: its draw
     uint32_t* pchunk_scene;  // per CHUNK-prim binning chunk: its scene
@@ -173,7 +176,8 @@ struct JobBufs {
     uint32_t* parts;
     unsigned long long* parts64;
     // capacities (elements)
-    uint32_t tri_cap, clip_cap, prim_cap, pair_cap, mat_cap, fine_cap, fmat_cap, bins_cap, tile_cap, fchunk_cap;
+    uint32_t tri_cap, clip_cap, prim_cap, pair_cap, mat_cap, fine_cap, fmat_cap, bins_cap, fchunk_cap;
+    uint32_t tile_cap;  // raster tiles tile_rec holds (4 records each)
     uint32_t tchunk_cap, pchunk_cap, rstat_cap;  // rstat_cap in workgroups
     uint32_t clip_wide;    // clip every polygon with a full wave (k_clip_redo)
     uint32_t exact_paths;  // setup proves nothing: every prim takes the raster's exact paths

@@ -685,7 +685,7 @@ __device__ __forceinline__ bool finite_bary(const PrimGeom& o) {
 
 __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t H, uint32_t draw, uint32_t fs,
                                            bool tex_idx32, bool proofs, PrimGeom* g, float* __restrict__ attr,
-                                           uint4* __restrict__ pbin) {
+                                           float* __restrict__ uvat, uint4* __restrict__ pbin) {
     const float zmin = 0.0f, zmax = 1.0f;
     float sx[3], sy[3], sz[3], w[3];
     for (int k = 0; k < 3; ++k) {
@@ -736,17 +736,18 @@ __device__ __forceinline__ void setup_prim(const ClipV* tv, uint32_t W, uint32_t
     o.pad1 = 0;
     *g = o;
     *pbin = make_uint4(0u, o.bbx, o.bby, __float_as_uint(o.zlo));
-    // vector stores of the attributes the shader reads (the record is 48-byte aligned)
+    // vector stores of only the attributes the shader reads: the 3 colours
+    // (48-byte records) or the 3 uv pairs (24-byte records in their own array)
     if (fs == RZ_FS_COLOR) {
         float4* a4 = reinterpret_cast<float4*>(attr);
         a4[0] = make_float4(tv[0].a[0], tv[0].a[1], tv[0].a[2], tv[0].a[3]);
         a4[1] = make_float4(tv[1].a[0], tv[1].a[1], tv[1].a[2], tv[1].a[3]);
         a4[2] = make_float4(tv[2].a[0], tv[2].a[1], tv[2].a[2], tv[2].a[3]);
     } else if (fs == RZ_FS_TEXTURE) {
-        float4* a4 = reinterpret_cast<float4*>(attr);
-        a4[0] = make_float4(tv[0].a[4], tv[0].a[5], tv[1].a[4], tv[1].a[5]);
-        a4[1] = make_float4(tv[2].a[4], tv[2].a[5], 0.0f, 0.0f);
-        a4[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float2* a2 = reinterpret_cast<float2*>(uvat);
+        a2[0] = make_float2(tv[0].a[4], tv[0].a[5]);
+        a2[1] = make_float2(tv[1].a[4], tv[1].a[5]);
+        a2[2] = make_float2(tv[2].a[4], tv[2].a[5]);
     }
 }
 
@@ -836,7 +837,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
             }
             if (prim < prim_cap)
                 setup_prim(tv, W, H, d, fs, tex_idx32, proofs, staged ? stage + pp : b.geom + prim,
-                           b.attrs + (size_t)prim * ATTR_STRIDE, b.pbin + prim);
+                           b.attrs + (size_t)prim * ATTR_STRIDE, b.uvat + (size_t)prim * UV_STRIDE, b.pbin + prim);
             else
                 ovf = true;
         }
@@ -851,6 +852,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         if (__ballot(ovf) && lane_id() == 0) atomicOr(&b.cnt->overflow, OVF_PRIMS);
     }
 }
+
+// ---------------------------------------------------------------------------
+// XCD-aware work split: workgroups are dispatched round-robin over the 8
+// XCDs (workgroup i runs on XCD i % 8), each with its own L2. Binning chunk
+// c writes the cells of count matrices and the list entries next to those
+// of chunks c - 1 and c + 1, so neighbouring chunks share cache lines. The
+// chunks are dealt to the XCDs in runs of BAND_RUN consecutive chunks, so
+// that a line written by several chunks is mostly merged in one L2 instead
+// of being written back in parts from several, while every XCD still gets
+// chunks from all over the job (uneven chunk costs stay balanced).
+// ChunkBand walks this workgroup's chunks of [0, n).
+// ---------------------------------------------------------------------------
+constexpr uint32_t BAND_RUN = 16;
+
+struct ChunkBand {
+    uint32_t j, step, x, n;
+    bool plain;
+    __device__ ChunkBand(uint32_t n_) : n(n_) {
+        const uint32_t G = gridDim.x;
+        plain = G < 8;  // too few workgroups for one per XCD: plain grid stride
+        x = blockIdx.x & 7u;
+        j = plain ? blockIdx.x : blockIdx.x >> 3;
+        step = plain ? G : (G - x + 7u) / 8u;  // workgroups on XCD x
+    }
+    // the j-th chunk of XCD x: run j / BAND_RUN of that XCD, offset j % BAND_RUN
+    __device__ uint32_t chunk() const {
+        return plain ? j : ((j / BAND_RUN) * 8u + x) * BAND_RUN + j % BAND_RUN;
+    }
+    __device__ bool more() const {
+        return plain ? j < n : ((j / BAND_RUN) * 8u + x) * BAND_RUN < n;
+    }
+};
 
 // ---------------------------------------------------------------------------
 // k_bin: ordered coarse binning, one workgroup per chunk of CHUNK prims of one
@@ -879,7 +912,9 @@ __global__ __launch_bounds__(256) void k_bin(JobBufs b, const uint32_t* __restri
     const uint32_t NC = uniform(b.cnt->nchunks);
     const uint32_t t = threadIdx.x, w = t >> 5, below = (1u << (t & 31)) - 1u;
     bool ovf = false;
-    for (uint32_t chunk = blockIdx.x; chunk < NC; chunk += gridDim.x) {
+    for (ChunkBand cb_(NC); cb_.more(); cb_.j += cb_.step) {
+        const uint32_t chunk = cb_.chunk();
+        if (chunk >= NC) continue;  // past the end inside this XCD's last run
         const uint32_t s = uniform(pchunk_scene[chunk]);
         const SceneCfg& sc = scenes[s];
         const SceneRun& r = runs[s];
@@ -961,7 +996,7 @@ __global__ __launch_bounds__(256) void k_fine_prep(JobBufs b) {
             beg = coarse_ofs(b, i0, mat_len, tot);
             end = coarse_ofs(b, i0 + r.nch, mat_len, tot);
             if (end < beg) end = beg;
-            T = (sc.cb / 16) * (sc.cb / 16);
+            T = (sc.cb >> sc.fts) * (sc.cb >> sc.fts);
         }
         const uint32_t n = (end - beg + CHUNK - 1) / CHUNK;
         b.bin_rng[g] = make_uint2(beg, end);
@@ -985,6 +1020,18 @@ __device__ __forceinline__ uint32_t quad_mask(uint32_t x0, uint32_t x1, uint32_t
                                               uint32_t oy) {
     const uint32_t cx = (x0 < ox + 8 ? 1u : 0u) | (x1 > ox + 8 ? 2u : 0u);
     return ((y0 < oy + 8) ? cx : 0u) | ((y1 > oy + 8) ? cx << 2 : 0u);
+}
+
+// The entry's quadrant bits for a fine tile of side 1 << fts at (ox, oy):
+// which 8x8 quadrant positions of the raster's 16x16 tiles it may touch.
+// 16: the quadrants of that tile; 8: the fine tile is one quadrant of a
+// raster tile; 32: the tile spans four raster tiles, all bits (the raster's
+// bbox test then decides).
+__device__ __forceinline__ uint32_t fine_quad_bits(uint32_t fts, uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1,
+                                                   uint32_t ox, uint32_t oy) {
+    if (fts == 4u) return quad_mask(x0, x1, y0, y1, ox, oy);
+    if (fts == 3u) return 1u << (((oy >> 2) & 2u) | ((ox >> 3) & 1u));
+    return 0xFu;
 }
 
 // k_fchunk_map, one thread per global coarse bin: writes {bin, scene} for
@@ -1020,11 +1067,13 @@ __global__ __launch_bounds__(256) void k_fine_bin(JobBufs b, const uint2* __rest
     }
     const uint32_t t = threadIdx.x, w = t >> 5, below = (1u << (t & 31)) - 1u;
     bool ovf = false;
-    for (uint32_t gc = blockIdx.x; gc < NCH; gc += gridDim.x) {
+    for (ChunkBand cb_(NCH); cb_.more(); cb_.j += cb_.step) {
+        const uint32_t gc = cb_.chunk();
+        if (gc >= NCH) continue;  // past the end inside this XCD's last run
         const uint2 fr = fchunk_rec[gc];
         const uint32_t g = uniform(fr.x), s = uniform(fr.y);
         const SceneCfg& sc = scenes[s];
-        const uint32_t cb = sc.cb, TS = cb / 16, T = TS * TS;
+        const uint32_t fts = sc.fts, cb = sc.cb, TS = cb >> fts, T = TS * TS;
         if (T > MAX_BINS) continue;  // the host never configures more tiles per coarse bin
         const unsigned long long fp = fpre[g];
         const uint2 rng = bin_rng[g];
@@ -1045,10 +1094,10 @@ __global__ __launch_bounds__(256) void k_fine_bin(JobBufs b, const uint2* __rest
             for (uint32_t k = 0; k < BIN_WORDS; ++k) bits[k * MAX_BINS + t] = 0u;
         const uint32_t x0 = en.y & 0xFFFFu, x1 = en.y >> 16, y0 = en.z & 0xFFFFu, y1 = en.z >> 16;
         // local tile range inside this coarse bin (the entry overlaps it)
-        const uint32_t lx0 = (max(x0, binx0) - binx0) >> 4;
-        const uint32_t lx1 = valid ? (min(x1, binx0 + cb) - 1 - binx0) >> 4 : 0u;
-        const uint32_t ly0 = (max(y0, biny0) - biny0) >> 4;
-        const uint32_t ly1 = valid ? (min(y1, biny0 + cb) - 1 - biny0) >> 4 : 0u;
+        const uint32_t lx0 = (max(x0, binx0) - binx0) >> fts;
+        const uint32_t lx1 = valid ? (min(x1, binx0 + cb) - 1 - binx0) >> fts : 0u;
+        const uint32_t ly0 = (max(y0, biny0) - biny0) >> fts;
+        const uint32_t ly1 = valid ? (min(y1, biny0 + cb) - 1 - biny0) >> fts : 0u;
         uint32_t* row = bits + w * MAX_BINS;
         __syncthreads();
         if (valid)
@@ -1072,7 +1121,8 @@ __global__ __launch_bounds__(256) void k_fine_bin(JobBufs b, const uint2* __rest
                     for (uint32_t tx = lx0; tx <= lx1; ++tx) {
                         const uint32_t lt = ty * TS + tx;
                         const uint32_t pos = wb[lt] + (uint32_t)__popc(row[lt] & below);
-                        const uint32_t qm = quad_mask(x0, x1, y0, y1, binx0 + tx * 16, biny0 + ty * 16);
+                        const uint32_t qm =
+                            fine_quad_bits(fts, x0, x1, y0, y1, binx0 + (tx << fts), biny0 + (ty << fts));
                         if (pos < b.fine_cap) b.flist[pos] = make_uint2(en.x, key | qm);
                         else ovf = true;
                     }
@@ -1083,31 +1133,35 @@ __global__ __launch_bounds__(256) void k_fine_bin(JobBufs b, const uint2* __rest
 }
 
 // ---------------------------------------------------------------------------
-// k_tile_prep: one thread per raster tile: its scene and the range of its
-// fine list (the fine matrix entry of (its coarse bin, its local tile, chunk
-// 0) and of the next tile), so the raster starts a tile with one load
-// instead of a chain of searches and lookups. Ranges are clamped to the list
-// buffer: a bad range can never read past it.
+// k_tile_prep: one thread per raster tile and 8x8 quadrant: the tile's scene
+// and the range of the fine list that quadrant's wave walks -- the list of
+// the fine tile holding the quadrant (the tile itself for 16x16 fine tiles,
+// the quadrant for 8x8 ones, the enclosing 32x32 tile for 32x32 ones): the
+// fine matrix entry of (its coarse bin, its local tile, chunk 0) and of the
+// next tile, so the raster starts with one load instead of a chain of
+// searches and lookups. Ranges are clamped to the list buffer: a bad range
+// can never read past it.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_tile_prep(JobBufs b) {
     const uint32_t nt = min(b.head->ntiles, b.tile_cap), ns = b.head->nscenes;
     if (b.cnt->overflow != 0) return;
     const uint32_t fmat_len = b.cnt->fmat_len, ftot = b.cnt->fine_total;
-    for (uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x; gt < nt; gt += gridDim.x * blockDim.x) {
+    for (uint32_t gq = blockIdx.x * blockDim.x + threadIdx.x; gq < 4u * nt; gq += gridDim.x * blockDim.x) {
+        const uint32_t gt = gq >> 2, q = gq & 3u;
         const uint32_t s = find_scene_tile(b.scenes, ns, gt);
         const SceneCfg& sc = b.scenes[s];
         const uint32_t ntx = (sc.W + 15) / 16, tile = gt - sc.tile_base;
-        const uint32_t tx = tile % ntx, ty = tile / ntx;
-        const uint32_t cbs = sc.cb, TS = cbs / 16;
-        const uint32_t cbx = (tx * 16) / cbs, cby = (ty * 16) / cbs;
+        const uint32_t px = (tile % ntx) * 16 + (q & 1u) * 8, py = (tile / ntx) * 16 + (q >> 1) * 8;
+        const uint32_t cbs = sc.cb, fts = sc.fts, TS = cbs >> fts;
+        const uint32_t cbx = px / cbs, cby = py / cbs;
         const uint32_t g = sc.bin_base + cby * sc.nbx + cbx;
-        const uint32_t lt = ((ty * 16 - cby * cbs) >> 4) * TS + ((tx * 16 - cbx * cbs) >> 4);
+        const uint32_t lt = ((py - cby * cbs) >> fts) * TS + ((px - cbx * cbs) >> fts);
         const uint2 rng = b.bin_rng[g];
         const uint32_t nchb = (rng.y - rng.x + CHUNK - 1) / CHUNK;
         const size_t s_idx = (size_t)hi32(b.fpre[g]) + (size_t)lt * nchb, e_idx = s_idx + nchb;
         const uint32_t lend = min(e_idx < fmat_len ? b.fine_off[e_idx] : ftot, b.fine_cap);
         const uint32_t lbeg = min(s_idx < fmat_len ? b.fine_off[s_idx] : ftot, lend);
-        b.tile_rec[gt] = make_uint4(s, lbeg, lend, 0u);
+        b.tile_rec[gq] = make_uint4(s, lbeg, lend, 0u);
     }
 }
 
@@ -1297,8 +1351,8 @@ __device__ __forceinline__ float sample0_depth(const PrimGeom& g, float fx, floa
 // main.rs:67-77 fragment shaders, then mod.rs:380-397 (to ARGB): the colour
 // one fragment writes to its passing samples
 __device__ __forceinline__ uint32_t shade_fragment(const PrimGeom& g, const float* __restrict__ at,
-                                                   const DrawDev* __restrict__ draws, float fx, float fy,
-                                                   uint32_t pass, const float* lut, bool& oob) {
+                                                   const float* __restrict__ uv, const DrawDev* __restrict__ draws,
+                                                   float fx, float fy, uint32_t pass, const float* lut, bool& oob) {
     float u, v, w;
     frag_bary(g, fx, fy, pass, u, v, w);
     const uint32_t fs = (g.flags >> 8) & 3u;
@@ -1310,8 +1364,8 @@ __device__ __forceinline__ uint32_t shade_fragment(const PrimGeom& g, const floa
         ca = at[3] * u + at[7] * v + at[11] * w;
     } else if (fs == RZ_FS_TEXTURE) {
         const DrawDev& dd = draws[g.flags >> 16];
-        const float tu = at[0] * u + at[2] * v + at[4] * w;
-        const float tv = at[1] * u + at[3] * v + at[5] * w;
+        const float tu = uv[0] * u + uv[2] * v + uv[4] * w;
+        const float tv = uv[1] * u + uv[3] * v + uv[5] * w;
         const RGBA c = texture_sample(dd.tex, dd.tw, dd.th, tu, tv, lut, (g.flags & PRIM_UV_CHECK) != 0, oob);
         cr = c.r; cg = c.g; cbl = c.b; ca = c.a;
     } else {  // Debug: Color::grayscale(depths[0]) (color.rs:65-72)
@@ -1327,6 +1381,7 @@ __device__ __forceinline__ uint32_t shade_fragment(const PrimGeom& g, const floa
 struct RasterIn {
     const PrimGeom* __restrict__ geom;
     const float* __restrict__ attrs;
+    const float* __restrict__ uvat;
     const uint2* __restrict__ flist;
     const uint4* __restrict__ tile_rec;
     const SceneCfg* __restrict__ scenes;
@@ -1347,8 +1402,9 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
                                             ShadeLds& sh,
                                             uint32_t& st_entries, uint32_t& st_cand, uint32_t& st_hits,
                                             uint32_t& st_cov, uint32_t& st_shaded) {
-    // the tile's scene and list range (k_tile_prep)
-    const uint4 tr = in.tile_rec[gtile];
+    // the tile's scene and this wave's list range (k_tile_prep)
+    const uint32_t wave = uniform(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const uint4 tr = in.tile_rec[4u * gtile + wave];
     const uint32_t s = uniform(tr.x);
     const SceneCfg& sc = in.scenes[s];
     const uint32_t W = sc.W, H = sc.H;
@@ -1356,7 +1412,6 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
     const uint32_t ntx = (W + 15) / 16;
     const uint32_t tile = gtile - sc.tile_base;
     const uint32_t tx = tile % ntx, ty = tile / ntx;
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t qx0 = tx * 16 + (wave & 1) * 8, qy0 = ty * 16 + (wave >> 1) * 8;
     const uint32_t px = qx0 + (lane & 7), py = qy0 + (lane >> 3);
     const float fx = (float)px, fy = (float)py;
@@ -1533,7 +1588,7 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
                             // fragment, also ones overwritten later)
                             float u, v, w;
                             frag_bary(gm, fx, fy, pass, u, v, w);
-                            const float* at = in.attrs + (size_t)prim * ATTR_STRIDE;
+                            const float* at = in.uvat + (size_t)prim * UV_STRIDE;
                             const DrawDev& dd = in.draws[gm.flags >> 16];
                             oob |= texel_oob(dd.tw, dd.th, at[0] * u + at[2] * v + at[4] * w,
                                              at[1] * u + at[3] * v + at[5] * w);
@@ -1615,7 +1670,8 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
                 const float ofy = (float)(ty * 16 + (r >> 1) * 8 + (ol >> 3));
                 const PrimGeom gp = geom[P];
                 sh.argb[r][i] =
-                    shade_fragment(gp, in.attrs + (size_t)P * ATTR_STRIDE, in.draws, ofx, ofy, pass, lut, oob);
+                    shade_fragment(gp, in.attrs + (size_t)P * ATTR_STRIDE, in.uvat + (size_t)P * UV_STRIDE, in.draws,
+                                   ofx, ofy, pass, lut, oob);
                 if (STATS) st_shaded += 1;  // shading tasks run by this lane
             }
         }
@@ -1652,9 +1708,10 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
 // arithmetic), which only textures with a side past 2^24 texels take.
 template <bool STATS, bool PERSIST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STATS || PERSIST ? 1 : 8, 8))) void k_raster(
-    JobBufs b, const PrimGeom* __restrict__ geom, const float* __restrict__ attrs, const uint2* __restrict__ flist,
+    JobBufs b, const PrimGeom* __restrict__ geom, const float* __restrict__ attrs, const float* __restrict__ uvat,
+    const uint2* __restrict__ flist,
     const uint4* __restrict__ tile_rec, const SceneCfg* __restrict__ scenes, const DrawDev* __restrict__ draws) {
-    const RasterIn in{geom, attrs, flist, tile_rec, scenes, draws};
+    const RasterIn in{geom, attrs, uvat, flist, tile_rec, scenes, draws};
     __shared__ float lut[256];
     __shared__ uint32_t next_tile;
     __shared__ ShadeLds sh;
@@ -1812,11 +1869,11 @@ void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_s
     launch_scan<uint32_t>(s, b.fine_counts, b.fine_off, b.fmat_cap, &b.cnt->fmat_len, b.parts, &b.cnt->fine_total);
     k_fine_bin<true><<<at_least1(g.fine), 256, 0, s>>>(b, b.fchunk_rec, b.scenes, b.fpre, b.bin_rng, b.list,
                                                 b.fine_off);
-    k_tile_prep<<<at_least1(std::min<uint32_t>((g.ntiles + 255) / 256, 4096u)), 256, 0, s>>>(b);
+    k_tile_prep<<<at_least1(std::min<uint32_t>((4u * g.ntiles + 255) / 256, 4096u)), 256, 0, s>>>(b);
     if (ev_raster) (void)hipEventRecord(ev_raster, s);
     const uint32_t rgrid = g.raster ? g.raster : at_least1((g.ntiles + 7) / 8 * 8);
 #define RZ_RASTER(ST, PE) \
-    k_raster<ST, PE><<<rgrid, 256, 0, s>>>(b, b.geom, b.attrs, b.flist, b.tile_rec, b.scenes, b.draws)
+    k_raster<ST, PE><<<rgrid, 256, 0, s>>>(b, b.geom, b.attrs, b.uvat, b.flist, b.tile_rec, b.scenes, b.draws)
     if (raster_stats) {
         if (g.raster) RZ_RASTER(true, true);
         else RZ_RASTER(true, false);

@@ -130,6 +130,33 @@ class Context:
     def texture_count(self) -> int:
         return int(self._L.rz_texture_count(self.handle))
 
+    def memory(self) -> dict:
+        """rz_memory: device bytes held by this context, resident in its
+        group, and what it will hold once the recorded job has run."""
+        a = (C.c_uint64 * 3)()
+        self._check(self._L.rz_memory(self.handle, a))
+        return dict(held=int(a[0]), resident=int(a[1]), job=int(a[2]))
+
+    def batch_need(self, items) -> int:
+        """rz_batch_need: bytes this context would hold after a batch of the
+        given rz_batch_item records (nothing is recorded)."""
+        items = np.ascontiguousarray(items, dtype=L.batch_item_dtype())
+        out = C.c_uint64(0)
+        self._check(self._L.rz_batch_need(self.handle, items.ctypes.data_as(C.c_void_p), len(items),
+                                          C.byref(out)))
+        return int(out.value)
+
+    def trim(self):
+        """rz_trim: free this context's pipeline buffers (between jobs)."""
+        self._check(self._L.rz_trim(self.handle))
+
+    def set_fine_tile(self, side: int):
+        """Side (8, 16 or 32) of the tiles whose lists binning builds."""
+        self._check(self._L.rz_set_fine_tile(self.handle, int(side)))
+
+    def fine_tile(self) -> int:
+        return int(self._L.rz_get_fine_tile(self.handle))
+
     def set_coarse_bin(self, pixels: int):
         self._check(self._L.rz_set_coarse_bin(self.handle, pixels))
 
@@ -234,14 +261,19 @@ class Context:
         self._check(self._L.rz_batch_scene_flags(self.handle, int(scene), C.byref(f)))
         return f.value
 
-    def batch_tuning(self, bin_target: int = 256, raster_per_cu: int = 0, use_graph: bool = True):
+    def batch_tuning(self, bin_target: int = 256, raster_per_cu: int = 0, use_graph: bool = True,
+                     fine_tile: int | None = None):
+        """rz_batch_tuning, and rz_set_fine_tile when fine_tile is given."""
         self._check(self._L.rz_batch_tuning(self.handle, int(bin_target), int(raster_per_cu),
                                             1 if use_graph else 0))
+        if fine_tile is not None:
+            self.set_fine_tile(fine_tile)
 
     def get_batch_tuning(self) -> dict:
         a = (C.c_uint32 * 3)()
         self._check(self._L.rz_batch_get_tuning(self.handle, a))
-        return dict(bin_target=int(a[0]), raster_per_cu=int(a[1]), use_graph=bool(a[2]))
+        return dict(bin_target=int(a[0]), raster_per_cu=int(a[1]), use_graph=bool(a[2]),
+                    fine_tile=self.fine_tile())
 
     def profile(self, enable: bool):
         self._check(self._L.rz_profile(self.handle, 1 if enable else 0))

@@ -11,7 +11,7 @@ CLEAR_COLOR = 0xFF191919
 CPU_THREADS = max(1, min(16, os.cpu_count() or 1))
 
 
-def run_gpu(scenes, coarse_bin: int | None = None, ctx=None, msaa: bool = True):
+def run_gpu(scenes, coarse_bin: int | None = None, ctx=None, msaa: bool = True, fine_tile: int | None = None):
     """Render one frame made of the given scenes' draws (same size) on the GPU."""
     from rusterizer_amd.render import Context
 
@@ -25,6 +25,8 @@ def run_gpu(scenes, coarse_bin: int | None = None, ctx=None, msaa: bool = True):
         ctx.set_clear_depth(s0.clear_depth)
         if coarse_bin is not None:
             ctx.set_coarse_bin(coarse_bin)
+        if fine_tile is not None:
+            ctx.set_fine_tile(fine_tile)
         tex_bound = False
         meshes = []
         for sc in scenes:

@@ -90,11 +90,15 @@ def test_batch_equals_plain_path():
 
 
 TUNINGS = [
-    dict(bin_target=256, raster_per_cu=0, use_graph=False),
-    dict(bin_target=16, raster_per_cu=0, use_graph=False),
-    dict(bin_target=4, raster_per_cu=2, use_graph=False),
-    dict(bin_target=64, raster_per_cu=1, use_graph=True),
-    dict(bin_target=256, raster_per_cu=8, use_graph=True),
+    dict(bin_target=256, raster_per_cu=0, use_graph=False, fine_tile=16),
+    dict(bin_target=16, raster_per_cu=0, use_graph=False, fine_tile=16),
+    dict(bin_target=4, raster_per_cu=2, use_graph=False, fine_tile=16),
+    dict(bin_target=64, raster_per_cu=1, use_graph=True, fine_tile=16),
+    dict(bin_target=256, raster_per_cu=8, use_graph=True, fine_tile=16),
+    dict(bin_target=64, raster_per_cu=0, use_graph=True, fine_tile=8),
+    dict(bin_target=16, raster_per_cu=4, use_graph=False, fine_tile=8),
+    dict(bin_target=256, raster_per_cu=0, use_graph=True, fine_tile=32),
+    dict(bin_target=4, raster_per_cu=0, use_graph=False, fine_tile=32),
 ]
 
 
@@ -248,3 +252,45 @@ def test_batch_add_array_records_like_single_calls():
         assert table.dtype == L.batch_item_dtype()
     finally:
         ctx.close()
+
+
+def test_memory_report_and_trim():
+    """rz_memory / rz_batch_need report the bytes a context holds and will
+    hold; rz_trim gives the pipeline buffers back and the next batch (which
+    grows them again and re-captures its graph) renders the same images."""
+    from rusterizer_amd import _lib as L
+    from rusterizer_amd import batch
+    from rusterizer_amd.render import Context
+
+    scs = scenes.batch_scenes(n_scenes=5, seed=131, tri_lo=2000, tri_hi=30000, resolutions=(256, 512),
+                              tex_size=64)
+    ctx = Context(64, 64)
+    try:
+        items = batch.upload(ctx, scs)
+        table = batch.item_table(items)
+        m0 = ctx.memory()
+        assert m0["resident"] >= sum(sc.pos.nbytes + sc.idx.nbytes + sc.uv.nbytes + sc.texture.nbytes for sc in scs)
+        need = ctx.batch_need(table)
+        assert need > m0["held"]
+        batch.record(ctx, items, table)
+        assert ctx.memory()["job"] == need  # the recorded batch, the same estimate
+        ctx.batch_submit()
+        ctx.batch_wait()
+        first = [batch.image(r).copy() for r in items]
+        m1 = ctx.memory()
+        assert m1["held"] >= need  # the buffers the job sized (grown further only on overflow)
+        assert ctx.batch_need(table) == m1["held"]  # the same batch needs no more
+        ctx.trim()
+        m2 = ctx.memory()
+        assert m2["held"] < m1["held"] // 10 and m2["resident"] == m1["resident"]
+        batch.run(ctx, items, table)
+        for r, img in zip(items, first):
+            assert np.array_equal(batch.image(r), img), r.scene.name
+        for r in items:
+            r.mesh.destroy()
+        assert ctx.memory()["resident"] == m0["resident"] - sum(
+            sc.pos.nbytes + sc.idx.nbytes + sc.uv.nbytes + (sc.color.nbytes if sc.color is not None else 0)
+            for sc in scs)
+    finally:
+        ctx.close()
+    _ = L

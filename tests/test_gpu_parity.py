@@ -76,6 +76,19 @@ def test_clear_depth_one():
     check(sc)
 
 
+@pytest.mark.parametrize("fine_tile", [8, 32])
+def test_fine_tile_sides(fine_tile):
+    """Lists of 8x8 and 32x32 screen tiles (the raster still works on 16x16
+    tiles): same frames, on uneven screens and every coarse bin side, on the
+    terrain and on a screen at the side limit."""
+    sc = scenes.soup_scene(5000, 333, 217, seed=12, textured=True)
+    for cb in (0, 32, 48, 64, 128, 256):
+        check(sc, coarse_bin=cb, fine_tile=fine_tile)
+    check(scenes.terrain_scene(quads=200, width=640, height=360, tex_size=512), fine_tile=fine_tile)
+    wide = scenes.soup_scene(3000, 16384, 24, seed=13)
+    check(wide, fine_tile=fine_tile)
+
+
 def test_uneven_screen_and_bins():
     sc = scenes.soup_scene(5000, 333, 217, seed=12, textured=True)
     for cb in (32, 48, 64, 128, 256):

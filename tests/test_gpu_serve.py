@@ -134,6 +134,12 @@ def test_serving_full_size_catalog():
 
 
 def test_serving_budget_admission():
+    """Admission counts the device bytes the library reports (rz_memory,
+    rz_batch_need): with a budget that holds the real need of the largest
+    request but not the fixed-rate estimate of it, every request is served
+    within the budget (batches are cut to fit and idle slots give memory
+    back), and its images are right; a budget below the resident data
+    rejects every request instead of waiting forever."""
     import torch
 
     from rusterizer_amd.serve import Server, make_requests
@@ -141,24 +147,40 @@ def test_serving_budget_admission():
     root, cat = _catalog()
     try:
         alone = _alone(cat)
-        big = max(cat.templates, key=lambda t: t.scene.ntri)
-        srv0 = Server(cat, slots=1)
-        need_big = srv0.est_bytes(big)
-        srv0.close()
-        budget = cat.resident_bytes + 1.5 * need_big
-        reqs = make_requests(cat, 120, rate=40_000.0, weights={"batch": 0.6, "soup": 0.2, "terrain": 0.2}, seed=8)
+        probe = Server(cat, slots=3)
+        try:
+            fresh = probe.ctxs[0].memory()["held"]
+            est = {i: probe.est_bytes(t) for i, t in enumerate(cat.templates)}
+            dry = {i: probe.ctxs[0].batch_need(np.array([t.row], dtype=L.batch_item_dtype())) - fresh
+                   for i, t in enumerate(cat.templates)}
+        finally:
+            probe.close()
+        # what each template really took alone, against the library's up-front
+        # estimate and the fixed-rate guess: they differ
+        big = max(range(len(cat.templates)), key=lambda i: cat.templates[i].need)
+        t_big = cat.templates[big]
+        print("budget test: measured / library estimate / fixed guess of the largest:",
+              t_big.need, dry[big], est[big], flush=True)
+        assert any(abs(t.need - est[i]) > 0.2 * t.need for i, t in enumerate(cat.templates))
+        rm = root.memory()
+        base = rm["resident"] + rm["held"] + 3 * fresh
+        headroom = cat.empty_bytes + t_big.need - fresh + 4 * t_big.pixels + (8 << 20)
+        budget = base + headroom
+        print(f"budget test: base {base}, headroom {headroom}, budget {budget}", flush=True)
+        reqs = make_requests(cat, 150, rate=40_000.0, weights={"batch": 0.6, "soup": 0.2, "terrain": 0.2}, seed=8)
         srv = Server(cat, slots=3, budget_bytes=budget, max_batch=12)
         try:
-            rep = srv.run(reqs, keep_outputs=True)
-            for r in reqs:
-                assert torch.equal(r.out, alone[r.tmpl]), r.rid
+            rep = srv.run(reqs, copy_back=True)
         finally:
             srv.close()
-        assert rep["peak_admitted_bytes"] <= budget
-        assert rep["requests"] == 120
+        assert rep["requests"] == 150 and rep["rejected"] == 0
+        assert rep["peak_device_bytes"] <= budget
+        assert rep["budget_cuts"] > 0  # some batches were cut down to fit
+        for r in reqs:
+            assert torch.equal(r.image, alone[r.tmpl].cpu()), r.rid
         # a request larger than the whole budget is rejected, not waited on forever
         reqs2 = make_requests(cat, 10, rate=40_000.0, weights={"terrain": 1.0}, seed=9)
-        srv = Server(cat, slots=2, budget_bytes=cat.resident_bytes + 1000)
+        srv = Server(cat, slots=2, budget_bytes=base + 1000)
         try:
             rep2 = srv.run(reqs2)
         finally:

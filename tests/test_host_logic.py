@@ -37,8 +37,11 @@ def test_autotune_candidates_signature_and_cache(tmp_path):
     from rusterizer_amd import autotune
 
     cands = list(autotune.candidates())
-    assert len(cands) == len(autotune.BIN_TARGETS) * len(autotune.RASTER_PER_CU)
+    assert len(cands) == len(autotune.FINE_TILES) * len(autotune.BIN_TARGETS) * len(autotune.RASTER_PER_CU)
     assert all(c["use_graph"] for c in cands)
+    # the tile sides asked for (8x8 / 16x16 / 32x32) and the bin capacities are all searched
+    assert {c["fine_tile"] for c in cands} == {8, 16, 32}
+    assert {c["bin_target"] for c in cands} == set(autotune.BIN_TARGETS)
     # the signature depends on the resolution histogram and the triangle count's magnitude only
     a = autotune.signature([(256, 256), (512, 512), (256, 256)], 100_000)
     b = autotune.signature([(512, 512), (256, 256), (256, 256)], 120_000)
@@ -56,26 +59,27 @@ def test_autotune_candidates_signature_and_cache(tmp_path):
 
 def test_autotune_picks_the_fastest_candidate(monkeypatch):
     """tune() times every candidate on the caller's batch and leaves the
-    context on the fastest (a fake clock makes (64 bins, 0 per CU) fastest)."""
+    context on the fastest (a fake clock makes (8x8 tiles, 64 bins, 0 per
+    CU) fastest)."""
     from rusterizer_amd import autotune
 
     class FakeCtx:
         t = None
 
-        def batch_tuning(self, bin_target, raster_per_cu, use_graph):
-            self.t = (bin_target, raster_per_cu, use_graph)
+        def batch_tuning(self, bin_target, raster_per_cu, use_graph, fine_tile=None):
+            self.t = (bin_target, raster_per_cu, use_graph, fine_tile)
 
     ctx = FakeCtx()
     clock = [0.0]
     monkeypatch.setattr(autotune.time, "perf_counter", lambda: clock[0])
 
     def run_batch():
-        bt, rp, _ = ctx.t
-        clock[0] += 1e-3 if (bt, rp) == (64, 0) else (bt + rp + 1) * 1e-3
+        bt, rp, _, ft = ctx.t
+        clock[0] += 1e-3 if (bt, rp, ft) == (64, 0, 8) else (bt + rp + ft + 1) * 1e-3
 
     best, results = autotune.tune(ctx, run_batch, reps=3, warmup=1)
-    assert (best["bin_target"], best["raster_per_cu"]) == (64, 0)
-    assert ctx.t == (64, 0, True)  # left on the best
+    assert (best["bin_target"], best["raster_per_cu"], best["fine_tile"]) == (64, 0, 8)
+    assert ctx.t == (64, 0, True, 8)  # left on the best
     assert len(results) == len(list(autotune.candidates()))
     assert min(results, key=lambda r: r[1])[0] == best
 
