@@ -11,7 +11,9 @@ and whether the batch replays its captured HIP graph (rz_batch_tuning).
 Bin lists have no fixed capacity: binning counts, scans and then writes
 each list at its exact length. Every setting renders bit-identical images;
 only the time changes. The tuner times each candidate on the caller's real
-batch and keeps the fastest.
+batch and keeps the fastest; a caller may name a preferred candidate that
+wins unless another is faster by more than a tolerance, so that timing noise
+between near-equal settings does not flip the choice from run to run.
 """
 from __future__ import annotations
 
@@ -22,7 +24,11 @@ import time
 
 FINE_TILES = (8, 16, 32)
 BIN_TARGETS = (16, 64, 256)
-RASTER_PER_CU = (0, 4)
+RASTER_PER_CU = (0, 8)  # 8 persistent workgroups per CU fill it (8 waves per SIMD)
+
+# the usual winner on mixed-resolution batches of textured soups: 16x16 fine
+# tiles, up to 64 coarse bins per scene, one raster workgroup per tile
+DEFAULT = dict(fine_tile=16, bin_target=64, raster_per_cu=0, use_graph=True)
 
 
 def candidates(fine_tiles=FINE_TILES, bin_targets=BIN_TARGETS, raster_per_cu=RASTER_PER_CU, graph=(True,)):
@@ -40,9 +46,11 @@ def signature(scene_sizes, tris: int) -> str:
                        "tris_log2": max(0, int(tris).bit_length())}, sort_keys=True)
 
 
-def tune(ctx, run_batch, cands=None, reps: int = 3, warmup: int = 1):
+def tune(ctx, run_batch, cands=None, reps: int = 3, warmup: int = 1, prefer=None, tol: float = 0.0):
     """run_batch(): records, submits and waits one batch on ctx. Returns
-    (best tuning, [(tuning, median seconds), ...]); leaves ctx on the best."""
+    (best tuning, [(tuning, median seconds), ...]); leaves ctx on the best.
+    prefer: a candidate kept as the best while its median is within
+    (1 + tol) of the fastest one's."""
     results = []
     for t in (cands if cands is not None else candidates()):
         ctx.batch_tuning(**t)
@@ -55,7 +63,10 @@ def tune(ctx, run_batch, cands=None, reps: int = 3, warmup: int = 1):
             ts.append(time.perf_counter() - t0)
         ts.sort()
         results.append((t, ts[len(ts) // 2]))
-    best = min(results, key=lambda r: r[1])[0]
+    best, fastest = min(results, key=lambda r: r[1])
+    for t, sec in results:
+        if t == prefer and sec <= fastest * (1.0 + tol):
+            best = t
     ctx.batch_tuning(**best)
     return best, results
 

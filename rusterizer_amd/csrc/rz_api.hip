@@ -428,7 +428,7 @@ int size_buffers(rz_ctx* c, const JobPlan& pl, const Needs& n_in, G& g) {
     g(c->pbin, geom);
     g(c->tchunk_draw, cls / TRI_ALIGN + 1);
     g(c->pchunk_scene, geom / CHUNK + 1);
-    if (c->raster_stats) g(c->rstat_parts, 5ull * ((uint64_t)pl.ntiles + 8 + 32ull * (uint32_t)c->n_cu));
+    if (c->raster_stats) g(c->rstat_parts, 5ull * ((uint64_t)pl.ntiles + 128 + 32ull * (uint32_t)c->n_cu));
     const size_t clip = g(c->clip_tri, n.clip);
     g(c->clip_redo, clip);
     g(c->clip_draw, clip);

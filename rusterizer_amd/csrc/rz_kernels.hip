@@ -1133,6 +1133,17 @@ __global__ __launch_bounds__(256) void k_fine_bin(JobBufs b, const uint2* __rest
 }
 
 // ---------------------------------------------------------------------------
+// Rank of raster tile (tx, ty) of a scene of ntx x nty tiles in blocked
+// order: 4x4-tile blocks in row-major order, tiles row-major inside a block;
+// the blocks of the last column and row are narrower or shorter. A bijection
+// of [0, ntx * nty): block row by holds rows * ntx tiles, every block left of
+// column bx is 4 wide.
+__device__ __forceinline__ uint32_t blocked_rank(uint32_t tx, uint32_t ty, uint32_t ntx, uint32_t nty) {
+    const uint32_t bx = tx / 4u, by = ty / 4u;
+    const uint32_t rows = min(4u, nty - by * 4u), cols = min(4u, ntx - bx * 4u);
+    return by * 4u * ntx + bx * 4u * rows + (ty % 4u) * cols + tx % 4u;
+}
+
 // k_tile_prep: one thread per raster tile and 8x8 quadrant: the tile's scene
 // and the range of the fine list that quadrant's wave walks -- the list of
 // the fine tile holding the quadrant (the tile itself for 16x16 fine tiles,
@@ -1140,7 +1151,9 @@ __global__ __launch_bounds__(256) void k_fine_bin(JobBufs b, const uint2* __rest
 // fine matrix entry of (its coarse bin, its local tile, chunk 0) and of the
 // next tile, so the raster starts with one load instead of a chain of
 // searches and lookups. Ranges are clamped to the list buffer: a bad range
-// can never read past it.
+// can never read past it. Records are stored in raster order: the tile of
+// scene rank r (blocked_rank) at tile_base + r, with the tile's own index,
+// so that consecutive raster slots are 4x4 squares of the screen.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_tile_prep(JobBufs b) {
     const uint32_t nt = min(b.head->ntiles, b.tile_cap), ns = b.head->nscenes;
@@ -1150,8 +1163,9 @@ __global__ __launch_bounds__(256) void k_tile_prep(JobBufs b) {
         const uint32_t gt = gq >> 2, q = gq & 3u;
         const uint32_t s = find_scene_tile(b.scenes, ns, gt);
         const SceneCfg& sc = b.scenes[s];
-        const uint32_t ntx = (sc.W + 15) / 16, tile = gt - sc.tile_base;
-        const uint32_t px = (tile % ntx) * 16 + (q & 1u) * 8, py = (tile / ntx) * 16 + (q >> 1) * 8;
+        const uint32_t ntx = (sc.W + 15) / 16, nty = (sc.H + 15) / 16, tile = gt - sc.tile_base;
+        const uint32_t tx = tile % ntx, ty = tile / ntx;
+        const uint32_t px = tx * 16 + (q & 1u) * 8, py = ty * 16 + (q >> 1) * 8;
         const uint32_t cbs = sc.cb, fts = sc.fts, TS = cbs >> fts;
         const uint32_t cbx = px / cbs, cby = py / cbs;
         const uint32_t g = sc.bin_base + cby * sc.nbx + cbx;
@@ -1161,7 +1175,8 @@ __global__ __launch_bounds__(256) void k_tile_prep(JobBufs b) {
         const size_t s_idx = (size_t)hi32(b.fpre[g]) + (size_t)lt * nchb, e_idx = s_idx + nchb;
         const uint32_t lend = min(e_idx < fmat_len ? b.fine_off[e_idx] : ftot, b.fine_cap);
         const uint32_t lbeg = min(s_idx < fmat_len ? b.fine_off[s_idx] : ftot, lend);
-        b.tile_rec[gq] = make_uint4(s, lbeg, lend, 0u);
+        const uint32_t slot = sc.tile_base + blocked_rank(tx, ty, ntx, nty);
+        if (slot < b.tile_cap) b.tile_rec[4u * slot + q] = make_uint4(s, lbeg, lend, gt);
     }
 }
 
@@ -1173,8 +1188,9 @@ __global__ __launch_bounds__(256) void k_tile_prep(JobBufs b) {
 // triangle that can touch the quadrant, then resolves (buffers.rs:111-125)
 // and writes the ARGB word once.
 //
-// Tiles of all scenes form one index space. With a grid of one workgroup
-// per tile the order is static; with a smaller (persistent) grid the
+// Tiles of all scenes form one index space of raster slots (each scene's
+// tiles in 4x4 blocks, k_tile_prep). With a grid of one workgroup per tile
+// the order is static; with a smaller (persistent) grid the
 // workgroups pull tiles from 8 queues, one band of tiles per XCD, and move
 // on to the other bands when theirs is empty.
 //
@@ -1290,6 +1306,16 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+// raster slots of the one-workgroup-per-tile raster are dealt to the XCDs
+// in runs of this many consecutive slots (one 4x4 block of tiles)
+constexpr uint32_t RASTER_RUN = 16;
+
+// workgroups that cover nt tiles in runs of RASTER_RUN dealt to 8 XCDs
+static uint32_t raster_grid(uint32_t nt) {
+    const uint32_t runs = (nt + RASTER_RUN - 1) / RASTER_RUN;
+    return std::max<uint32_t>(1u, (runs + 7) / 8 * 8 * RASTER_RUN);
+}
+
 // band x of the persistent queue: tiles [band_lo(x), band_lo(x + 1))
 __device__ __forceinline__ uint32_t band_lo(uint32_t nt, uint32_t x) {
     const uint32_t q = nt / 8u, r = nt % 8u;
@@ -1398,14 +1424,14 @@ struct ShadeLds {
 };
 
 template <bool STATS>
-__device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in, uint32_t gtile, const float* lut,
+__device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in, uint32_t slot, const float* lut,
                                             ShadeLds& sh,
                                             uint32_t& st_entries, uint32_t& st_cand, uint32_t& st_hits,
                                             uint32_t& st_cov, uint32_t& st_shaded) {
     // the tile's scene and this wave's list range (k_tile_prep)
     const uint32_t wave = uniform(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const uint4 tr = in.tile_rec[4u * gtile + wave];
-    const uint32_t s = uniform(tr.x);
+    const uint4 tr = in.tile_rec[4u * slot + wave];
+    const uint32_t s = uniform(tr.x), gtile = uniform(tr.w);
     const SceneCfg& sc = in.scenes[s];
     const uint32_t W = sc.W, H = sc.H;
     const float clear_depth = sc.clear_depth;
@@ -1702,12 +1728,13 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
     }
 }
 
-// The production variant is held to 64 VGPRs (8 waves per SIMD): the extra
-// waves hide more of the walk's load latency than anything the limit costs.
-// Its few spills sit in the shader's exact texel path (64-bit texel index
-// arithmetic), which only textures with a side past 2^24 texels take.
+// The production variants are held to 64 VGPRs (8 waves per SIMD): the
+// extra waves hide more of the walk's load latency than anything the limit
+// costs. The one-workgroup-per-tile variant fits without spilling; the
+// persistent one keeps 64 bytes of its loop state per lane in scratch, and
+// at 8 workgroups per CU runs a quarter faster than unconstrained (6 waves).
 template <bool STATS, bool PERSIST>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STATS || PERSIST ? 1 : 8, 8))) void k_raster(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STATS ? 1 : 8, 8))) void k_raster(
     JobBufs b, const PrimGeom* __restrict__ geom, const float* __restrict__ attrs, const float* __restrict__ uvat,
     const uint2* __restrict__ flist,
     const uint4* __restrict__ tile_rec, const SceneCfg* __restrict__ scenes, const DrawDev* __restrict__ draws) {
@@ -1721,15 +1748,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STATS || PE
     uint32_t st_entries = 0, st_cand = 0, st_hits = 0, st_cov = 0, st_shaded = 0;
     const uint32_t nt = b.head->ntiles;
     const uint32_t blk = blockIdx.x, xcd = blk & 7u;
-    // one workgroup per tile: workgroups go round-robin to the 8 XCDs, so
-    // workgroup 8k + x takes tile k of band x -- each XCD renders one
-    // contiguous band of tiles (shared lists stay in its L2); any grid of at
-    // least 8 * ceil(nt / 8) workgroups covers every tile, extra ones exit.
-    // persistent: the same bands become 8 queues, drained XCD-local first
+    // one workgroup per tile: workgroups go round-robin to the 8 XCDs, and
+    // the slots are dealt to them in runs of RASTER_RUN consecutive slots,
+    // a 4x4 square of tiles (workgroup 8k + x takes the k-th slot of XCD x's
+    // runs), so each XCD renders neighbouring tiles (the prims they share
+    // stay in its L2) while every XCD gets work from all over the job at the
+    // same time: the dispatcher hands out workgroups in order, so an XCD
+    // that lags behind (a band of dense tiles next to bands of empty sky)
+    // would hold up all of them. Any grid of at least raster_grid(nt)
+    // workgroups covers every slot; extra ones exit.
+    // persistent: contiguous bands of tiles become 8 queues, drained
+    // XCD-local first
     uint32_t tile = NO_TILE;
     if (!PERSIST) {
-        const uint32_t k = blk >> 3, lo = band_lo(nt, xcd);
-        if (k < band_lo(nt, xcd + 1) - lo) tile = lo + k;
+        const uint32_t k = blk >> 3;
+        const uint32_t t = ((k / RASTER_RUN) * 8u + xcd) * RASTER_RUN + k % RASTER_RUN;
+        if (t < nt) tile = t;
     }
     for (bool first = true;; first = false) {
         if (PERSIST) {
@@ -1871,7 +1905,7 @@ void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_s
                                                 b.fine_off);
     k_tile_prep<<<at_least1(std::min<uint32_t>((4u * g.ntiles + 255) / 256, 4096u)), 256, 0, s>>>(b);
     if (ev_raster) (void)hipEventRecord(ev_raster, s);
-    const uint32_t rgrid = g.raster ? g.raster : at_least1((g.ntiles + 7) / 8 * 8);
+    const uint32_t rgrid = g.raster ? g.raster : raster_grid(g.ntiles);
 #define RZ_RASTER(ST, PE) \
     k_raster<ST, PE><<<rgrid, 256, 0, s>>>(b, b.geom, b.attrs, b.uvat, b.flist, b.tile_rec, b.scenes, b.draws)
     if (raster_stats) {

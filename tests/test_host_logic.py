@@ -23,14 +23,15 @@ def test_bench_roofline_fields_and_committed_traffic():
     import bench
 
     for name in ("traffic_raster_batch.json", "traffic_raster_soup1m.json", "traffic_raster_terrain.json"):
-        t = bench.load_traffic(name)
+        t, src = bench.load_traffic(name)
         assert isinstance(t, int) and t > 0, name
+        assert src.startswith("profiles/") and os.path.exists(os.path.join(ROOT, src)), src
     rf = bench.roofline(1_442_252_848, 8.0, bench.load_traffic("traffic_raster_batch.json"))
-    assert set(rf) >= {"bound", "achieved", "peak", "unit", "frac", "traffic"}
+    assert set(rf) >= {"bound", "achieved", "peak", "unit", "frac", "traffic", "traffic_source"}
     assert rf["unit"] == "GB/s" and rf["peak"] == 8000.0
     assert abs(rf["achieved"] - 1_442_252_848 / 8e-3 / 1e9) < 1e-6
     assert abs(rf["frac"] - rf["achieved"] / rf["peak"]) < 1e-12
-    assert bench.load_traffic("no_such_profile.json") is None
+    assert bench.load_traffic("no_such_profile.json") == (None, None)
 
 
 def test_autotune_candidates_signature_and_cache(tmp_path):
@@ -82,6 +83,32 @@ def test_autotune_picks_the_fastest_candidate(monkeypatch):
     assert ctx.t == (64, 0, True, 8)  # left on the best
     assert len(results) == len(list(autotune.candidates()))
     assert min(results, key=lambda r: r[1])[0] == best
+
+
+def test_autotune_keeps_preferred_within_tolerance(monkeypatch):
+    """A preferred candidate within the tolerance of the fastest wins; one
+    slower by more than the tolerance does not."""
+    from rusterizer_amd import autotune
+
+    class FakeCtx:
+        t = None
+
+        def batch_tuning(self, bin_target, raster_per_cu, use_graph, fine_tile=None):
+            self.t = dict(fine_tile=fine_tile, bin_target=bin_target, raster_per_cu=raster_per_cu,
+                          use_graph=use_graph)
+
+    clock = [0.0]
+    monkeypatch.setattr(autotune.time, "perf_counter", lambda: clock[0])
+    fast = dict(fine_tile=32, bin_target=16, raster_per_cu=0, use_graph=True)
+    for pref_cost, want in ((1.015, autotune.DEFAULT), (1.05, fast)):
+        ctx = FakeCtx()
+
+        def run_batch():
+            clock[0] += 1.0 if ctx.t == fast else pref_cost if ctx.t == autotune.DEFAULT else 2.0
+
+        best, _ = autotune.tune(ctx, run_batch, reps=3, warmup=1, prefer=autotune.DEFAULT, tol=0.02)
+        assert best == want and ctx.t == want
+    assert autotune.DEFAULT in list(autotune.candidates())
 
 
 def test_poisson_arrivals_rate_and_determinism():
