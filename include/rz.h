@@ -210,6 +210,45 @@ int rz_batch_tuning(rz_ctx *ctx, uint32_t bin_target, uint32_t raster_per_cu, in
 /* The current tuning: [bin_target, raster_per_cu, use_graph]. */
 int rz_batch_get_tuning(const rz_ctx *ctx, uint32_t out[3]);
 
+/* ------------------------------------------------------------ raster service
+ * No counterpart in the reference (it renders one frame at a time on the
+ * calling thread). For servers that keep several contexts of one group busy
+ * at once: a persistent raster kernel stays resident on a stream of its own
+ * and renders the tiles of every batch that an attached context submits.
+ * The context's own stream runs the batch up to binning and then posts the
+ * job to the service through device memory; the service's workgroups take
+ * tiles from the posted job with the fewest tiles left, so a small job is
+ * not held up behind a long one, and jobs of several contexts render at
+ * the same time in one kernel. Results are identical to the context's own
+ * raster. rz_batch_query / rz_batch_wait report a batch done once the
+ * service has finished it.
+ *
+ * rz_service_start launches the service of ctx's group with wg_per_cu
+ * (1..4) workgroups per CU -- the rest of each CU stays free for the
+ * contexts' front ends -- that exit after idle_us (1 us .. 5 s) in which no
+ * tile was claimed, and after 4 s in any case; the library relaunches the
+ * service when work is posted and it has exited. RZ_ERR_STATE if it is
+ * running. rz_service_stop asks it to stop and waits (RZ_ERR_STATE while an
+ * attached context has a batch in flight). rz_service_attach(ctx, 1) routes
+ * ctx's batches (not its plain frames) through the running service (at most
+ * 16 contexts per group; RZ_ERR_LIMIT beyond; not with RZ_RASTER_STATS);
+ * rz_service_attach(ctx, 0) detaches it. Both between jobs. A batch of more
+ * than 16,769,023 16x16 tiles cannot be posted (RZ_ERR_LIMIT).
+ * rz_service_stats: out[0] started, out[1] a launch of it is running now,
+ * out[2] launches so far, out[3] workgroups per launch.
+ * The service runs on a CU-masked stream (a hardware queue of its own),
+ * which HIP creates as a blocking stream: work on the legacy null stream and
+ * device-wide synchronisation (hipDeviceSynchronize) wait until the service
+ * exits, up to idle_us after its last tile. Keep other work on non-blocking
+ * streams while it runs; the library's own copies from host memory and
+ * to host memory use the context's (non-blocking) stream, while
+ * rz_mesh_create_device copies on the null stream, after the caller's work
+ * there. */
+int rz_service_start(rz_ctx *ctx, uint32_t wg_per_cu, uint32_t idle_us);
+int rz_service_stop(rz_ctx *ctx);
+int rz_service_attach(rz_ctx *ctx, int on);
+int rz_service_stats(const rz_ctx *ctx, uint64_t out[4]);
+
 /* ------------------------------------------------------- allocation-free frame
  * No counterpart in the reference. For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

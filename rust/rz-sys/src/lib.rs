@@ -133,6 +133,13 @@ extern "C" {
     pub fn rz_batch_tuning(ctx: *mut rz_ctx, bin_target: u32, raster_per_cu: u32, use_graph: c_int) -> c_int;
     pub fn rz_batch_get_tuning(ctx: *const rz_ctx, out: *mut u32) -> c_int;
 
+    // persistent raster service (no reference counterpart; for servers
+    // that keep several contexts of one group busy)
+    pub fn rz_service_start(ctx: *mut rz_ctx, wg_per_cu: u32, idle_us: u32) -> c_int;
+    pub fn rz_service_stop(ctx: *mut rz_ctx) -> c_int;
+    pub fn rz_service_attach(ctx: *mut rz_ctx, on: c_int) -> c_int;
+    pub fn rz_service_stats(ctx: *const rz_ctx, out: *mut u64) -> c_int;
+
     // allocation-free frame (no reference counterpart; for callers that
     // cannot allocate device memory)
     pub fn rz_render_noalloc(

@@ -76,6 +76,10 @@ SIGNATURES = {
     "rz_trim": (C.c_int, [_vp]),
     "rz_set_fine_tile": (C.c_int, [_vp, C.c_uint32]),
     "rz_get_fine_tile": (C.c_uint32, [_vp]),
+    "rz_service_start": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),
+    "rz_service_stop": (C.c_int, [_vp]),
+    "rz_service_attach": (C.c_int, [_vp, C.c_int]),
+    "rz_service_stats": (C.c_int, [_vp, C.POINTER(C.c_uint64)]),
     "rz_profile": (C.c_int, [_vp, C.c_int]),
     "rz_profile_read": (C.c_int, [_vp, _f, _u32]),
     "rz_render_noalloc": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_float, _f, _f, _f, _vp, _vp, _vp,

@@ -22,6 +22,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "rz_kernels.h"
@@ -54,12 +55,27 @@ struct DBuf {
 
 }  // namespace
 
+// the persistent raster service of a context group (rz_service_start)
+struct SvcState {
+    bool on = false;                  // between rz_service_start and rz_service_stop
+    hipStream_t stream = nullptr;     // CU-masked: a hardware queue of its own
+    SvcEntry* ents = nullptr;         // SVC_MAX_ENTRIES entries
+    SvcCtl* ctl = nullptr;
+    SvcHost* host = nullptr;          // pinned
+    uint32_t gen[SVC_MAX_ENTRIES] = {};  // last job number posted per entry
+    uint32_t used = 0;                // entries held by attached contexts (bits)
+    uint32_t grid = 0;
+    unsigned long long idle_ticks = 0, life_ticks = 0;
+    uint64_t launches = 0;
+};
+
 // resources shared by the contexts of one root (same device)
 struct rz_pool {
     int device = 0;
     std::vector<Texture> textures;
     std::vector<rz_ctx*> ctxs;
     uint64_t resident_bytes = 0;  // device bytes of the group's meshes and textures
+    SvcState svc;
 };
 
 struct rz_mesh {
@@ -134,6 +150,7 @@ struct rz_ctx {
     size_t off_draws = 0, off_mats = 0, off_scenes = 0;
     SceneRun* d_runs = nullptr;
     SceneRun* h_runs = nullptr;  // pinned
+    uint32_t* h_sflags = nullptr;  // pinned: scene flags the raster service writes
     FrameCounters* d_cnt = nullptr;
     FrameCounters* h_cnt = nullptr;  // pinned
 
@@ -172,6 +189,12 @@ struct rz_ctx {
     hipEvent_t done = nullptr;
     uint32_t replays = 0, ovf_seen = 0;
     bool job_graph = false;  // the job in flight was launched from the graph
+
+    // raster service (rz_service_attach): this context's entry, or -1
+    int svc_entry = -1;
+    int graph_svc = -1;    // the entry the captured graph posts to
+    bool job_svc = false;  // the job in flight was posted to the service
+    uint32_t job_gen = 0;  // its number
 
     uint32_t frame_flags = 0;
     uint64_t frame_stats[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -275,15 +298,18 @@ int ensure_tables(rz_ctx* c, uint32_t nd, uint32_t ns) {
     if (c->d_job) (void)hipFree(c->d_job);
     if (c->h_runs) (void)hipHostFree(c->h_runs);
     if (c->d_runs) (void)hipFree(c->d_runs);
+    if (c->h_sflags) (void)hipHostFree(c->h_sflags);
     c->h_job = nullptr;
     c->d_job = nullptr;
     c->h_runs = nullptr;
     c->d_runs = nullptr;
+    c->h_sflags = nullptr;
     c->dcap = c->scap = 0;
     HIP_TRY(c, hipHostMalloc(&c->h_job, bytes, hipHostMallocDefault));
     HIP_TRY(c, hipMalloc(&c->d_job, bytes));
     HIP_TRY(c, hipHostMalloc(&c->h_runs, sizeof(SceneRun) * scap, hipHostMallocDefault));
     HIP_TRY(c, hipMalloc(&c->d_runs, sizeof(SceneRun) * scap));
+    HIP_TRY(c, hipHostMalloc(&c->h_sflags, sizeof(uint32_t) * scap, hipHostMallocDefault));
     memset(c->h_job, 0, bytes);
     c->job_bytes = bytes;
     c->off_draws = off_d;
@@ -532,6 +558,7 @@ JobBufs job_bufs(const rz_ctx* c) {
     b.scenes = reinterpret_cast<const SceneCfg*>(c->d_job + c->off_scenes);
     b.runs = c->d_runs;
     b.cnt = c->d_cnt;
+    b.svc_flags = c->h_sflags;
     b.cls = c->cls.p;
     b.nsub = c->nsub.p;
     b.suboff = c->suboff.p;
@@ -579,6 +606,87 @@ JobBufs job_bufs(const rz_ctx* c) {
 
 // grids: exact where the host knows the work (plain launches), capacity
 // bounds otherwise (graph launches serve any job that fits the buffers)
+// ---- raster service ------------------------------------------------------
+constexpr double SVC_LIFE_S = 4.0;  // a service launch exits after this long (the host relaunches it); job
+                                   // numbers (16 bits) cannot wrap within one launch
+constexpr double SVC_WAIT_S = 30.0;    // longest host wait for a posted job
+
+hipError_t svc_launch(rz_pool* p) {
+    SvcState& v = p->svc;
+    __atomic_store_n(&v.host->stop, 0u, __ATOMIC_RELEASE);
+    hipError_t e = hipMemsetAsync(v.ctl, 0, sizeof(SvcCtl), v.stream);
+    if (e != hipSuccess) return e;
+    launch_raster_service(v.stream, v.grid, v.ents, SVC_MAX_ENTRIES, v.ctl, v.host, v.idle_ticks, v.life_ticks);
+    e = hipGetLastError();
+    if (e == hipSuccess) v.launches++;
+    return e;
+}
+
+// the service is running, or is relaunched (it exits when idle or old)
+int svc_ensure(rz_ctx* c) {
+    SvcState& v = c->pool->svc;
+    if (!v.on) return fail(c, RZ_ERR_STATE, "the raster service is not running (rz_service_start)");
+    const hipError_t q = hipStreamQuery(v.stream);
+    if (q == hipErrorNotReady) return RZ_OK;
+    if (q != hipSuccess) return fail(c, RZ_ERR_HIP, std::string("raster service: ") + hipGetErrorString(q));
+    const hipError_t e = svc_launch(c->pool);
+    if (e != hipSuccess) return fail(c, RZ_ERR_HIP, std::string("raster service launch: ") + hipGetErrorString(e));
+    return RZ_OK;
+}
+
+bool svc_job_done(const rz_ctx* c) {
+    const uint32_t d = __atomic_load_n(&c->pool->svc.host->done[c->svc_entry], __ATOMIC_ACQUIRE);
+    return d == (c->job_gen & 0xFFFFu);
+}
+
+// wait for the service to finish the posted job (relaunching it if it
+// exited meanwhile)
+int svc_wait(rz_ctx* c) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t i = 0;; ++i) {
+        if (svc_job_done(c)) return RZ_OK;
+        if ((i & 255u) == 255u) {
+            int rc = svc_ensure(c);
+            if (rc) return rc;
+            if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > SVC_WAIT_S)
+                return fail(c, RZ_ERR_HIP, "the raster service did not finish a posted job in time");
+        }
+        std::this_thread::yield();
+    }
+}
+
+void svc_free(rz_pool* p) {
+    SvcState& v = p->svc;
+    if (v.on && v.stream) {
+        __atomic_store_n(&v.host->stop, 1u, __ATOMIC_RELEASE);
+        (void)hipStreamSynchronize(v.stream);
+        v.on = false;
+    }
+    if (v.ents) (void)hipFree(v.ents);
+    if (v.ctl) (void)hipFree(v.ctl);
+    if (v.host) (void)hipHostFree(v.host);
+    if (v.stream) (void)hipStreamDestroy(v.stream);
+    v = SvcState{};
+}
+
+// posts the job in flight to the service instead of running the raster
+bool posts_to_service(const rz_ctx* c) { return c->svc_entry >= 0 && c->recording == JobKind::Batch; }
+
+// before each launch of a posting job: its number in the job table, and a
+// running service
+int svc_prepare(rz_ctx* c, const JobPlan& pl) {
+    c->job_svc = posts_to_service(c);
+    if (!c->job_svc) return RZ_OK;
+    if (pl.ntiles > SVC_MAX_TILES) return fail(c, RZ_ERR_LIMIT, "too many tiles for one raster service job");
+    SvcState& v = c->pool->svc;
+    c->job_gen = (v.gen[c->svc_entry] + 1u) & 0xFFFFu;
+    if (c->job_gen == (__atomic_load_n(&v.host->done[c->svc_entry], __ATOMIC_ACQUIRE) & 0xFFFFu))
+        c->job_gen = (c->job_gen + 1u) & 0xFFFFu;  // never the number already marked done
+    v.gen[c->svc_entry] = c->job_gen;
+    reinterpret_cast<JobHead*>(c->h_job)->svc_gen = c->job_gen;
+    return svc_ensure(c);
+}
+
 JobGrid job_grid(const rz_ctx* c, const JobPlan& pl, bool for_graph) {
     JobGrid g{};
     const uint32_t nd = for_graph ? c->dcap : pl.ndraws;
@@ -598,10 +706,14 @@ JobGrid job_grid(const rz_ctx* c, const JobPlan& pl, bool for_graph) {
 
 int enqueue_eager(rz_ctx* c, const JobPlan& pl) {
     hipStream_t s = c->stream;
+    int rc = svc_prepare(c, pl);
+    if (rc) return rc;
+    SvcState& v = c->pool->svc;
     if (c->prof) HIP_TRY(c, hipEventRecord(c->ev[0], s));
     HIP_TRY(c, hipMemcpyAsync(c->d_job, c->h_job, c->job_bytes, hipMemcpyHostToDevice, s));
     launch_job(s, job_bufs(c), job_grid(c, pl, false), c->raster_stats, c->prof ? c->ev[1] : nullptr,
-               c->prof ? c->ev[2] : nullptr);
+               c->prof ? c->ev[2] : nullptr, c->job_svc ? v.ents + c->svc_entry : nullptr, v.host, v.ctl,
+               (uint32_t)std::max(c->svc_entry, 0));
     if (c->prof) HIP_TRY(c, hipEventRecord(c->ev[3], s));
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipMemcpyAsync(c->h_cnt, c->d_cnt, sizeof(FrameCounters), hipMemcpyDeviceToHost, s));
@@ -615,10 +727,15 @@ int enqueue_eager(rz_ctx* c, const JobPlan& pl) {
 // capture (once per buffer generation / table capacity / tuning) and launch
 int enqueue_graph(rz_ctx* c, const JobPlan& pl) {
     hipStream_t s = c->stream;
+    int rc = svc_prepare(c, pl);
+    if (rc) return rc;
+    SvcState& v = c->pool->svc;
+    const int svc = c->job_svc ? c->svc_entry : -1;
     if (pl.ntiles > c->tiles_cap) c->tiles_cap = std::max<uint32_t>(pl.ntiles, c->tiles_cap + c->tiles_cap / 4);
     const JobGrid g = job_grid(c, pl, true);
     if (!c->graph || c->graph_gen != c->gen || c->graph_dcap != c->dcap || c->graph_scap != c->scap ||
-        c->graph_raster != g.raster || c->graph_tiles != g.ntiles || c->graph_stats != c->raster_stats) {
+        c->graph_raster != g.raster || c->graph_tiles != g.ntiles || c->graph_stats != c->raster_stats ||
+        c->graph_svc != svc) {
         if (c->graph) {
             HIP_TRY(c, hipStreamSynchronize(s));
             (void)hipGraphExecDestroy(c->graph);
@@ -628,7 +745,8 @@ int enqueue_graph(rz_ctx* c, const JobPlan& pl) {
         HIP_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         hipError_t e = hipMemcpyAsync(c->d_job, c->h_job, c->job_bytes, hipMemcpyHostToDevice, s);
         if (e == hipSuccess) {
-            launch_job(s, job_bufs(c), g, c->raster_stats, nullptr, nullptr);
+            launch_job(s, job_bufs(c), g, c->raster_stats, nullptr, nullptr, svc >= 0 ? v.ents + svc : nullptr,
+                       v.host, v.ctl, (uint32_t)std::max(svc, 0));
             e = hipGetLastError();
         }
         if (e == hipSuccess)
@@ -652,6 +770,7 @@ int enqueue_graph(rz_ctx* c, const JobPlan& pl) {
         c->graph_raster = g.raster;
         c->graph_tiles = g.ntiles;
         c->graph_stats = c->raster_stats;
+        c->graph_svc = svc;
     }
     if (c->prof) HIP_TRY(c, hipEventRecord(c->ev[0], s));
     HIP_TRY(c, hipGraphLaunch(c->graph, s));
@@ -678,7 +797,8 @@ void release_job(rz_ctx* c) {
 
 // After the job's completion event: on overflow grow and replay
 // (asynchronously); else record the results. Returns 1 when finished,
-// 0 when a replay was queued, < 0 on error.
+// 0 when a replay was queued, 2 while the raster service has not finished
+// the posted job, < 0 on error.
 int complete_job(rz_ctx* c) {
     const FrameCounters fc = *c->h_cnt;
     c->ovf_seen |= fc.overflow;
@@ -705,6 +825,13 @@ int complete_job(rz_ctx* c) {
         return rc ? rc : 0;
     }
     const uint32_t ns = c->plan.nscenes;
+    if (c->job_svc) {
+        // the front end is done and the job posted; the service writes the
+        // scenes' final flags to host memory before it marks the job done
+        if (!svc_job_done(c)) return 2;
+        for (uint32_t s = 0; s < ns; ++s) c->h_runs[s].flags = __atomic_load_n(&c->h_sflags[s], __ATOMIC_ACQUIRE);
+        c->job_svc = false;
+    }
     c->scene_flags.assign(ns, 0);
     uint32_t fl = 0;
     for (uint32_t s = 0; s < ns; ++s) {
@@ -750,6 +877,10 @@ int wait_job(rz_ctx* c) {
     for (;;) {
         HIP_TRY(c, hipEventSynchronize(c->done));
         int rc = complete_job(c);
+        if (rc == 2) {
+            if ((rc = svc_wait(c))) return rc;
+            continue;
+        }
         if (rc < 0) return rc;
         if (rc == 1) return RZ_OK;
     }
@@ -779,11 +910,17 @@ int frame_impl(rz_ctx* c, uint32_t* host_out, uint32_t* dev_out, uint32_t* host_
     release_job(c);
     c->clear_depth = c->next_clear_depth;
     if (rc) return rc;
-    if (host_out) HIP_TRY(c, hipMemcpy(host_out, target, npx * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    // on the context's stream (non-blocking), so that a running raster
+    // service, whose stream the null stream waits for, does not hold them up
+    if (host_out)
+        HIP_TRY(c, hipMemcpyAsync(host_out, target, npx * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     if (host_color4)
-        HIP_TRY(c, hipMemcpy(host_color4, c->msaa_color.p, npx * 4 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpyAsync(host_color4, c->msaa_color.p, npx * 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                  c->stream));
     if (host_depth4)
-        HIP_TRY(c, hipMemcpy(host_depth4, c->msaa_depth.p, npx * 4 * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpyAsync(host_depth4, c->msaa_depth.p, npx * 4 * sizeof(float), hipMemcpyDeviceToHost,
+                                  c->stream));
+    if (host_out || host_color4 || host_depth4) HIP_TRY(c, hipStreamSynchronize(c->stream));
     return RZ_OK;
 }
 
@@ -807,7 +944,12 @@ int mesh_create(rz_ctx* c, const float* pos, const float* color, const float* uv
         hipError_t e = hipMalloc(dst, bytes);
         if (e != hipSuccess) return fail(c, RZ_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
         m->bytes += bytes;
-        e = hipMemcpy(*dst, src, bytes, kind);
+        // host data: on the context's stream (not held up by a running
+        // raster service); device data: on the null stream, ordered after
+        // the caller's work there that produced it
+        e = kind == hipMemcpyHostToDevice ? hipMemcpyAsync(*dst, src, bytes, kind, c->stream)
+                                          : hipMemcpy(*dst, src, bytes, kind);
+        if (e == hipSuccess && kind == hipMemcpyHostToDevice) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) return fail(c, RZ_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
         return RZ_OK;
     };
@@ -1204,6 +1346,10 @@ int rz_create_shared(rz_ctx* parent, uint32_t width, uint32_t height, rz_ctx** o
 int rz_destroy(rz_ctx* c) {
     if (!c) return RZ_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    // a job posted to the raster service still reads this context's buffers
+    if (c->in_flight && c->job_svc && c->h_cnt && c->h_cnt->overflow == 0 && c->pool->svc.on && !svc_job_done(c))
+        (void)svc_wait(c);
+    if (c->svc_entry >= 0) c->pool->svc.used &= ~(1u << c->svc_entry);
     for (PendingDraw& d : c->draws) d.mesh->pending--;
     c->draws.clear();
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
@@ -1241,6 +1387,7 @@ int rz_destroy(rz_ctx* c) {
     if (c->d_job) (void)hipFree(c->d_job);
     if (c->h_runs) (void)hipHostFree(c->h_runs);
     if (c->d_runs) (void)hipFree(c->d_runs);
+    if (c->h_sflags) (void)hipHostFree(c->h_sflags);
     if (c->d_cnt) (void)hipFree(c->d_cnt);
     if (c->h_cnt) (void)hipHostFree(c->h_cnt);
     for (int i = 0; i < 4; ++i)
@@ -1251,6 +1398,7 @@ int rz_destroy(rz_ctx* c) {
     if (pool) {
         pool->ctxs.erase(std::remove(pool->ctxs.begin(), pool->ctxs.end(), c), pool->ctxs.end());
         if (pool->ctxs.empty()) {
+            svc_free(pool);
             for (Texture& t : pool->textures)
                 if (t.data) (void)hipFree(t.data);
             delete pool;
@@ -1291,7 +1439,8 @@ int rz_bind_texture(rz_ctx* c, uint32_t index, const uint8_t* rgba8, uint32_t tw
     t.th = th;
     size_t bytes = (size_t)tw * th * 4;
     HIP_TRY(c, hipMalloc(&t.data, bytes));
-    hipError_t e = hipMemcpy(t.data, rgba8, bytes, hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpyAsync(t.data, rgba8, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         (void)hipFree(t.data);
         return fail(c, RZ_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
@@ -1623,6 +1772,10 @@ int rz_batch_query(rz_ctx* c) {
         return fail(c, RZ_ERR_HIP, std::string("hipEventQuery: ") + hipGetErrorString(e));
     }
     int rc = complete_job(c);
+    if (rc == 2) {
+        rc = svc_ensure(c);  // relaunch a service that exited with the job posted
+        if (rc == RZ_OK) return 0;
+    }
     if (rc < 0 || rc == 1) release_job(c);
     return rc;
 }
@@ -1679,6 +1832,86 @@ int rz_batch_get_tuning(const rz_ctx* c, uint32_t out[3]) {
     out[0] = c->bin_target;
     out[1] = c->raster_per_cu;
     out[2] = c->use_graph ? 1u : 0u;
+    return RZ_OK;
+}
+
+int rz_service_start(rz_ctx* c, uint32_t wg_per_cu, uint32_t idle_us) {
+    if (!c) return RZ_ERR_ARG;
+    SvcState& v = c->pool->svc;
+    if (v.on) return fail(c, RZ_ERR_STATE, "the raster service is already running");
+    if (wg_per_cu < 1 || wg_per_cu > 4) return fail(c, RZ_ERR_ARG, "service workgroups per CU must be in [1, 4]");
+    if (idle_us < 1 || idle_us > 5000000) return fail(c, RZ_ERR_ARG, "service idle time must be in [1 us, 5 s]");
+    int khz = 0;
+    HIP_TRY(c, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->pool->device));
+    if (khz <= 0) return fail(c, RZ_ERR_HIP, "no wall clock rate");
+    if (!v.ents) {
+        hipError_t e = hipMalloc(&v.ents, sizeof(SvcEntry) * SVC_MAX_ENTRIES);
+        if (e == hipSuccess) e = hipMemset(v.ents, 0, sizeof(SvcEntry) * SVC_MAX_ENTRIES);
+        if (e == hipSuccess) e = hipMalloc(&v.ctl, sizeof(SvcCtl));
+        if (e == hipSuccess) e = hipHostMalloc(&v.host, sizeof(SvcHost), hipHostMallocDefault);
+        if (e == hipSuccess) {
+            memset(v.host, 0, sizeof(SvcHost));
+            std::vector<uint32_t> mask(((uint32_t)c->n_cu + 31) / 32, 0u);
+            for (uint32_t i = 0; i < (uint32_t)c->n_cu; ++i) mask[i / 32] |= 1u << (i % 32);
+            e = hipExtStreamCreateWithCUMask(&v.stream, (uint32_t)mask.size(), mask.data());
+        }
+        if (e != hipSuccess) {
+            svc_free(c->pool);
+            return fail(c, RZ_ERR_HIP, std::string("raster service set-up: ") + hipGetErrorString(e));
+        }
+    }
+    v.grid = std::min<uint32_t>(wg_per_cu * (uint32_t)c->n_cu, SVC_MAX_GRID);
+    v.idle_ticks = (unsigned long long)idle_us * (unsigned long long)khz / 1000ull;
+    v.life_ticks = (unsigned long long)(SVC_LIFE_S * 1000.0) * (unsigned long long)khz;
+    v.on = true;
+    const hipError_t e = svc_launch(c->pool);
+    if (e != hipSuccess) {
+        v.on = false;
+        return fail(c, RZ_ERR_HIP, std::string("raster service launch: ") + hipGetErrorString(e));
+    }
+    return RZ_OK;
+}
+
+int rz_service_stop(rz_ctx* c) {
+    if (!c) return RZ_ERR_ARG;
+    SvcState& v = c->pool->svc;
+    if (!v.on) return RZ_OK;
+    for (rz_ctx* o : c->pool->ctxs)
+        if (o->svc_entry >= 0 && o->in_flight) return fail(c, RZ_ERR_STATE, "a context has a job in flight");
+    __atomic_store_n(&v.host->stop, 1u, __ATOMIC_RELEASE);
+    HIP_TRY(c, hipStreamSynchronize(v.stream));
+    v.on = false;
+    return RZ_OK;
+}
+
+int rz_service_attach(rz_ctx* c, int on) {
+    if (!c) return RZ_ERR_ARG;
+    if (c->recording != JobKind::None || c->in_flight) return fail(c, RZ_ERR_STATE, "not between jobs");
+    SvcState& v = c->pool->svc;
+    if (!on) {
+        if (c->svc_entry >= 0) v.used &= ~(1u << c->svc_entry);
+        c->svc_entry = -1;
+        return RZ_OK;
+    }
+    if (c->svc_entry >= 0) return RZ_OK;
+    if (!v.on) return fail(c, RZ_ERR_STATE, "the raster service is not running (rz_service_start)");
+    if (c->raster_stats) return fail(c, RZ_ERR_STATE, "the counting raster (RZ_RASTER_STATS) cannot use the service");
+    for (uint32_t i = 0; i < SVC_MAX_ENTRIES; ++i)
+        if (!(v.used & (1u << i))) {
+            v.used |= 1u << i;
+            c->svc_entry = (int)i;
+            return RZ_OK;
+        }
+    return fail(c, RZ_ERR_LIMIT, "every raster service entry is taken");
+}
+
+int rz_service_stats(const rz_ctx* c, uint64_t out[4]) {
+    if (!c || !out) return RZ_ERR_ARG;
+    const SvcState& v = c->pool->svc;
+    out[0] = v.on ? 1u : 0u;
+    out[1] = v.on && hipStreamQuery(v.stream) == hipErrorNotReady ? 1u : 0u;
+    out[2] = v.launches;
+    out[3] = v.grid;
     return RZ_OK;
 }
 

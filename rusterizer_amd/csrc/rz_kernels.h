@@ -66,7 +66,8 @@ struct JobHead {
     uint32_t ntri;    // padded triangle index space (sum of draws rounded up to TRI_ALIGN)
     uint32_t ntiles;  // 16x16 raster tiles of all scenes
     uint32_t nbins;   // coarse bins of all scenes
-    uint32_t pad[3];
+    uint32_t svc_gen;  // raster service: number of this job in its context (low 16 bits are posted)
+    uint32_t pad[2];
 };
 
 // Per-draw state on the device (mesh streams, fragment shader, MVP).
@@ -181,6 +182,7 @@ struct JobBufs {
     uint32_t tchunk_cap, pchunk_cap, rstat_cap;  // rstat_cap in workgroups
     uint32_t clip_wide;    // clip every polygon with a full wave (k_clip_redo)
     uint32_t exact_paths;  // setup proves nothing: every prim takes the raster's exact paths
+    uint32_t* svc_flags;   // raster service: pinned host array of the scenes' final flags
 };
 
 // Grid sizes of one launch sequence. Kernels loop over their work with the
@@ -204,14 +206,85 @@ struct WordBlock {
     uint32_t w[WORD_BLOCK];
 };
 
+// ---- raster service ------------------------------------------------------
+// A persistent raster kernel (k_raster_service) stays resident on a stream of
+// its own and renders the tiles of jobs that the front ends of up to
+// SVC_MAX_ENTRIES contexts post to it: each context owns one entry; its
+// front end (every stage up to k_tile_prep) ends with k_svc_post, which
+// publishes the job in the entry. A job's tiles are split into 8 contiguous
+// bands, one per XCD, each with its own claim word (so claims stay in one
+// XCD's L2 instead of bouncing one line between all of them); workgroups
+// claim tiles one at a time from their XCD's band of the posted job with
+// the fewest tiles left there, then from other XCDs' bands, and the
+// workgroup that finishes a job's last tile writes the job's number to host
+// memory.
+constexpr uint32_t SVC_MAX_ENTRIES = 16;
+constexpr uint32_t SVC_TILE_BITS = 24;
+constexpr uint32_t SVC_TILE_MASK = (1u << SVC_TILE_BITS) - 1u;
+// claims past a job's last tile (at most one per workgroup) must not carry
+// into the tile-count field
+constexpr uint32_t SVC_MAX_TILES = SVC_TILE_MASK - 8192u;
+constexpr uint32_t SVC_MAX_GRID = 8192;
+
+// what the raster of a posted job reads
+struct SvcJob {
+    uint32_t ntiles;
+    const PrimGeom* geom;
+    const float* attrs;
+    const float* uvat;
+    const uint2* flist;
+    const uint4* tile_rec;
+    const SceneCfg* scenes;
+    const DrawDev* draws;
+    SceneRun* runs;
+    uint32_t* host_flags;  // pinned: each scene's RZ_FLAG_* bits once the job is done
+    uint32_t nscenes;
+};
+
+// one XCD's claim word, alone in its cache line: job number (16 bits) << 48
+// | the band's tiles << 24 | next unclaimed tile of the band. k_svc_post
+// stores it; workgroups claim with one atomic add, and a claim returns the
+// word as it was, so whoever adds owns the tile it names.
+struct alignas(128) SvcBand {
+    unsigned long long claim;
+};
+
+struct alignas(256) SvcEntry {
+    SvcBand band[8];
+    uint32_t finished;  // tiles of the posted job rendered
+    uint32_t pad;
+    SvcJob job;
+};
+
+struct alignas(256) SvcCtl {
+    unsigned long long last_claim;  // wall clock of a recent claim by any workgroup
+    uint32_t stop;                  // raised by workgroup 0 when the host asks the service to stop
+    uint32_t posted;                // jobs posted so far (idle workgroups watch this one word)
+};
+
+// pinned host memory shared with the service
+struct alignas(256) SvcHost {
+    uint32_t done[SVC_MAX_ENTRIES];  // number of the last job each entry finished
+    uint32_t stop;                   // the host asks the service to stop once idle
+};
+
 // ---- launchers (all asynchronous on `s`) ---------------------------------
 void launch_max_index(hipStream_t s, const uint32_t* idx, uint64_t n, uint32_t* out);  // *out must start at 0
 // dst[0, n) = blk.w[0, n), n <= WORD_BLOCK (one small kernel)
 void launch_write_words(hipStream_t s, const WordBlock& blk, uint32_t n, uint32_t* dst);
 // p[0, n) = v (one small kernel; used where a memset node would be captured)
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v);
-// the whole pipeline of one job, from counter reset to resolved outputs
+// the whole pipeline of one job, from counter reset to resolved outputs;
+// with svc (the context's raster service entry) the raster is left to the
+// service: the sequence ends by posting the job there
 void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_stats, hipEvent_t ev_bin,
-                hipEvent_t ev_raster);
+                hipEvent_t ev_raster, SvcEntry* svc = nullptr, SvcHost* svc_host = nullptr, SvcCtl* svc_ctl = nullptr,
+                uint32_t svc_index = 0);
+// the persistent raster service: grid workgroups (each CU's share of which
+// must fit beside the front ends' kernels) that exit when the host raises
+// svc_host->stop and they find no work, when no workgroup has claimed a tile
+// for idle_ticks of the wall clock, or life_ticks after the launch
+void launch_raster_service(hipStream_t s, uint32_t grid, SvcEntry* ents, uint32_t nent, SvcCtl* ctl,
+                           SvcHost* host, unsigned long long idle_ticks, unsigned long long life_ticks);
 
 }  // namespace rzk

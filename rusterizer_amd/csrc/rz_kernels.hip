@@ -1424,7 +1424,7 @@ struct ShadeLds {
 };
 
 template <bool STATS>
-__device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in, uint32_t slot, const float* lut,
+__device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, uint32_t slot, const float* lut,
                                             ShadeLds& sh,
                                             uint32_t& st_entries, uint32_t& st_cand, uint32_t& st_hits,
                                             uint32_t& st_cov, uint32_t& st_shaded) {
@@ -1708,7 +1708,7 @@ __device__ __forceinline__ void raster_tile(const JobBufs& b, const RasterIn& in
     const uint32_t cb1 = (vmask & 2u) ? wargb[(slots >> 8) & 0xFFu] : CLEAR_COLOR;
     const uint32_t cb2 = (vmask & 4u) ? wargb[(slots >> 16) & 0xFFu] : CLEAR_COLOR;
     const uint32_t cb3 = (vmask & 8u) ? wargb[slots >> 24] : CLEAR_COLOR;
-    if (__ballot(oob) && lane == 0) atomicOr(&b.runs[s].flags, RZ_FLAG_TEXEL_OOB);
+    if (__ballot(oob) && lane == 0) atomicOr(&runs[s].flags, RZ_FLAG_TEXEL_OOB);
     uint32_t l3 = lane;
     asm volatile("" : "+v"(l3));
     const uint32_t px3 = qx0 + (l3 & 7), py3 = qy0 + (l3 >> 3);
@@ -1785,7 +1785,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STATS ? 1 :
             tile = NO_TILE;
         }
         if (tile == NO_TILE) break;
-        raster_tile<STATS>(b, in, tile, lut, sh, st_entries, st_cand, st_hits, st_cov, st_shaded);
+        raster_tile<STATS>(b.runs, in, tile, lut, sh, st_entries, st_cand, st_hits, st_cov, st_shaded);
     }
     if (STATS) {  // the last two counts are per lane
         for (int o = 32; o >= 1; o >>= 1) {
@@ -1874,8 +1874,258 @@ void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v) {
 
 static uint32_t at_least1(uint32_t v) { return v ? v : 1u; }
 
+// ---------------------------------------------------------------------------
+// Raster service (see SvcEntry). k_svc_post ends a posting context's front
+// end: unless the job overflowed a buffer (the host then grows and replays
+// it), it stores the job's buffers in the context's entry and then, after a
+// release fence at device scope, the 8 band claim words that make the job
+// visible. A job without tiles is finished at once.
+// ---------------------------------------------------------------------------
+__global__ void k_svc_post(JobBufs b, SvcEntry* e, SvcHost* host, SvcCtl* ctl, uint32_t index) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (b.cnt->overflow != 0) return;
+    const uint32_t nt = b.head->ntiles;
+    const uint32_t gen = b.head->svc_gen & 0xFFFFu;
+    if (nt == 0) {
+        for (uint32_t s = 0; s < b.head->nscenes; ++s) b.svc_flags[s] = b.runs[s].flags;
+        __threadfence_system();
+        __hip_atomic_store(&host->done[index], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        return;
+    }
+    e->job = SvcJob{nt, b.geom, b.attrs, b.uvat, b.flist, b.tile_rec, b.scenes, b.draws, b.runs, b.svc_flags,
+                    b.head->nscenes};
+    e->finished = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    for (uint32_t x = 0; x < 8; ++x)
+        __hip_atomic_store(&e->band[x].claim,
+                           (unsigned long long)gen << 48 |
+                               (unsigned long long)(band_lo(nt, x + 1) - band_lo(nt, x)) << SVC_TILE_BITS,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(&ctl->posted, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// a pointer of a posted job: an atomic (vector) load, marked as global
+// memory so that the loads through it are global rather than flat loads
+template <class T, class P>
+__device__ __forceinline__ const T* svc_ptr(P* field) {
+    const unsigned long long v =
+        __hip_atomic_load(reinterpret_cast<unsigned long long*>(field), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return (const T*)(const __attribute__((address_space(1))) T*)v;
+}
+
+// a pointer field of a posted job as a plain (generic) pointer
+template <class T, class P>
+__device__ __forceinline__ T* svc_ptr_mut(P* field) {
+    return (T*)__hip_atomic_load(reinterpret_cast<unsigned long long*>(field), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// claim the next tile of band x of entry e: true with (index in the band,
+// job number) when the add landed inside the band
+__device__ __forceinline__ bool svc_claim(SvcEntry* ents, uint32_t e, uint32_t x, uint32_t& c, uint32_t& g) {
+    const unsigned long long w =
+        __hip_atomic_fetch_add(&ents[e].band[x].claim, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    c = (uint32_t)w & SVC_TILE_MASK;
+    g = (uint32_t)(w >> 48);
+    return c < ((uint32_t)(w >> SVC_TILE_BITS) & SVC_TILE_MASK);
+}
+
+// tiles left in band x of entry e (0 when claimed out)
+__device__ __forceinline__ uint32_t svc_left(const SvcEntry* ents, uint32_t e, uint32_t x) {
+    const unsigned long long w =
+        __hip_atomic_load(&ents[e].band[x].claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t n = (uint32_t)(w >> SVC_TILE_BITS) & SVC_TILE_MASK, c = (uint32_t)w & SVC_TILE_MASK;
+    return c < n ? n - c : 0u;
+}
+
+// The service loop. Thread 0 of a workgroup claims tiles (one atomic add on
+// a band's claim word each): from the band it claimed from last while no
+// new job has been posted, else from its own XCD's band of the posted job
+// with the fewest unclaimed tiles there (short jobs are not held up behind
+// long ones), else from another XCD's band (stealing balances the end of
+// a job). The
+// workgroup renders each tile with the raster's own tile routine.
+// Synchronisation is paid once per workgroup and job, not per tile: the
+// first claim of a job is followed by an acquire fence at device scope (the
+// job's buffers and tables, written by the front end's kernels before the
+// post's release, are visible after it), and the tiles a workgroup renders
+// of one job are released together -- every wave fences at device scope,
+// then one add to the job's finished count -- when its next claim is of
+// another job or finds nothing. The workgroup whose add completes a job
+// writes the scenes' flags and then the job number to host memory at
+// system scope. The job's data changes between jobs while this kernel runs,
+// so every read of it is a vector load through the coherent caches: the
+// descriptor is read with atomic loads, and the tile routine's reads follow
+// stores of the loop's earlier tiles that may alias them, so the compiler
+// cannot turn them into scalar loads (whose cache would keep an earlier
+// job's lines).
+// Exit: when the host's stop flag is seen (workgroup 0 reads host memory
+// and relays it through ctl) and no work is found, when no workgroup has
+// claimed a tile for idle_ticks (all workgroups go idle together, so the
+// service does not dwindle to a few), or, before claiming anything, once
+// life_ticks have passed since the launch -- every path of the loop reaches
+// one of these, releasing its finished tiles first, and the host relaunches
+// the service when it finds it gone with work posted.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_raster_service(
+    SvcEntry* ents, uint32_t nent, SvcCtl* ctl, SvcHost* host, unsigned long long idle_ticks,
+    unsigned long long life_ticks) {
+    __shared__ float lut[256];
+    __shared__ ShadeLds sh;
+    __shared__ uint32_t pick[6];
+    __shared__ uint32_t acq_gen[SVC_MAX_ENTRIES];  // thread 0: the job of each entry it has acquired
+    __shared__ uint32_t acq_nt[SVC_MAX_ENTRIES];   // and that job's tile count
+    constexpr uint32_t NONE = 0xFFFFFFFFu, EXIT = 0xFFFFFFFEu, RETRY = 0xFFFFFFFDu;
+    lut[threadIdx.x] = (float)threadIdx.x / 255.0f;  // color.rs:24-27
+    if (threadIdx.x < SVC_MAX_ENTRIES) acq_gen[threadIdx.x] = NONE;
+    const uint32_t xcd = blockIdx.x & 7u;  // workgroups are dealt to the 8 XCDs in turn
+    const unsigned long long t0 = (unsigned long long)wall_clock64();
+    // thread 0's claim state
+    unsigned long long noted = t0;  // the last claim time this workgroup stored
+    uint32_t polls = 0, cur_e = NONE, cur_x = 0, seen = 0;
+    bool quiet = false;  // the last scan found no work; nothing new until `posted` moves past `seen`
+    // finished tiles not yet released (uniform: every thread keeps them)
+    uint32_t pend_e = NONE, pend_g = 0, pend_nt = 0, pend_n = 0;
+    uint32_t st0 = 0, st1 = 0, st2 = 0, st3 = 0, st4 = 0;
+    __syncthreads();
+    for (;;) {
+        if (threadIdx.x == 0) {
+            uint32_t e_out = NONE, t_out = 0, g_out = 0, n_out = 0;
+            const unsigned long long now = (unsigned long long)wall_clock64();
+            if (now - t0 > life_ticks) {
+                e_out = EXIT;
+            } else {
+                const uint32_t posted = __hip_atomic_load(&ctl->posted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                uint32_t c = 0, g = 0, x = 0;
+                bool got = false;
+                if (cur_e != NONE && posted == seen) {
+                    got = svc_claim(ents, cur_e, cur_x, c, g);
+                    if (got) {
+                        e_out = cur_e;
+                        x = cur_x;
+                    } else {
+                        cur_e = NONE;
+                    }
+                }
+                if (!got && (!quiet || posted != seen)) {
+                    // own band first (shortest there), then the other bands
+                    uint32_t best = NONE, bx = 0, left = 0xFFFFFFFFu;
+                    for (uint32_t k = 0; k < 8 && best == NONE; ++k) {
+                        const uint32_t xb = (xcd + k) & 7u;
+                        for (uint32_t e = 0; e < nent; ++e) {
+                            const uint32_t l = svc_left(ents, e, xb);
+                            if (l != 0 && l < left) {
+                                left = l;
+                                best = e;
+                                bx = xb;
+                            }
+                        }
+                    }
+                    seen = posted;
+                    quiet = best == NONE;
+                    if (best != NONE) {
+                        got = svc_claim(ents, best, bx, c, g);
+                        cur_e = got ? best : NONE;
+                        cur_x = bx;
+                        if (got) {
+                            e_out = best;
+                            x = bx;
+                        } else {
+                            e_out = RETRY;  // others took the tiles meanwhile
+                        }
+                    }
+                }
+                if (got) {
+                    if (acq_gen[e_out] != g) {
+                        // first tile of this job here: see what the post published
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                        acq_gen[e_out] = g;
+                        acq_nt[e_out] = __hip_atomic_load(&ents[e_out].job.ntiles, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    n_out = acq_nt[e_out];
+                    t_out = band_lo(n_out, x) + c;
+                    g_out = g;
+                    // idleness is the whole service's: claims refresh a shared
+                    // time stamp (a store every idle/16 ticks at most)
+                    if (now - noted > idle_ticks / 16) {
+                        __hip_atomic_store(&ctl->last_claim, now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        noted = now;
+                    }
+                } else if (e_out == NONE) {
+                    if (blockIdx.x == 0 && (++polls & 15u) == 0 &&
+                        __hip_atomic_load(&host->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0)
+                        __hip_atomic_store(&ctl->stop, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    unsigned long long last =
+                        __hip_atomic_load(&ctl->last_claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    last = last > t0 ? last : t0;
+                    if (__hip_atomic_load(&ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+                        (now > last && now - last > idle_ticks))
+                        e_out = EXIT;
+                }
+            }
+            pick[0] = e_out;
+            pick[1] = t_out;
+            pick[2] = g_out;
+            pick[3] = n_out;
+        }
+        __syncthreads();
+        const uint32_t e = pick[0], t = pick[1], g = pick[2], nt = pick[3];
+        __syncthreads();
+        const bool tile = e < SVC_MAX_ENTRIES;
+        if (pend_n != 0 && !(tile && e == pend_e && g == pend_g)) {
+            // release this workgroup's finished tiles of its last job
+            __threadfence();
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const uint32_t f =
+                    __hip_atomic_fetch_add(&ents[pend_e].finished, pend_n, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                pick[4] = f + pend_n == pend_nt ? 1u : 0u;
+            }
+            __syncthreads();
+            if (pick[4] != 0) {
+                // the job's last tiles: the scenes' flags (raised by every
+                // stage and tile) to host memory, then the job number
+                SvcJob* pj = &ents[pend_e].job;
+                uint32_t* hf = svc_ptr_mut<uint32_t>(&pj->host_flags);
+                SceneRun* pr = svc_ptr_mut<SceneRun>(&pj->runs);
+                const uint32_t ns = __hip_atomic_load(&pj->nscenes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (uint32_t s = threadIdx.x; s < ns; s += blockDim.x)
+                    __hip_atomic_store(&hf[s],
+                                       __hip_atomic_load(&pr[s].flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __threadfence_system();
+                __syncthreads();
+                if (threadIdx.x == 0)
+                    __hip_atomic_store(&host->done[pend_e], pend_g, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+            __syncthreads();
+            pend_n = 0;
+        }
+        if (e == EXIT) break;
+        if (!tile) {
+            if (e == NONE) __builtin_amdgcn_s_sleep(32);
+            continue;
+        }
+        SvcJob* j = &ents[e].job;
+        const RasterIn in{svc_ptr<PrimGeom>(&j->geom), svc_ptr<float>(&j->attrs), svc_ptr<float>(&j->uvat),
+                          svc_ptr<uint2>(&j->flist),   svc_ptr<uint4>(&j->tile_rec), svc_ptr<SceneCfg>(&j->scenes),
+                          svc_ptr<DrawDev>(&j->draws)};
+        SceneRun* runs = svc_ptr_mut<SceneRun>(&j->runs);
+        raster_tile<false>(runs, in, t, lut, sh, st0, st1, st2, st3, st4);
+        pend_e = e;
+        pend_g = g;
+        pend_nt = nt;
+        pend_n++;
+    }
+}
+
+void launch_raster_service(hipStream_t s, uint32_t grid, SvcEntry* ents, uint32_t nent, SvcCtl* ctl,
+                           SvcHost* host, unsigned long long idle_ticks, unsigned long long life_ticks) {
+    k_raster_service<<<grid, 256, 0, s>>>(ents, nent, ctl, host, idle_ticks, life_ticks);
+}
+
 void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_stats, hipEvent_t ev_bin,
-                hipEvent_t ev_raster) {
+                hipEvent_t ev_raster, SvcEntry* svc, SvcHost* svc_host, SvcCtl* svc_ctl, uint32_t svc_index) {
     const uint32_t scene_blocks = at_least1((g.scenes + 255) / 256);
     k_frame_reset<<<scene_blocks, 256, 0, s>>>(b);
     k_mvp<<<at_least1(g.draws), 256, 0, s>>>(b);
@@ -1905,6 +2155,10 @@ void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_s
                                                 b.fine_off);
     k_tile_prep<<<at_least1(std::min<uint32_t>((4u * g.ntiles + 255) / 256, 4096u)), 256, 0, s>>>(b);
     if (ev_raster) (void)hipEventRecord(ev_raster, s);
+    if (svc) {
+        k_svc_post<<<1, 64, 0, s>>>(b, svc, svc_host, svc_ctl, svc_index);
+        return;
+    }
     const uint32_t rgrid = g.raster ? g.raster : raster_grid(g.ntiles);
 #define RZ_RASTER(ST, PE) \
     k_raster<ST, PE><<<rgrid, 256, 0, s>>>(b, b.geom, b.attrs, b.uvat, b.flist, b.tile_rec, b.scenes, b.draws)

@@ -157,6 +157,27 @@ class Context:
     def fine_tile(self) -> int:
         return int(self._L.rz_get_fine_tile(self.handle))
 
+    def service_start(self, wg_per_cu: int = 2, idle_us: int = 50_000):
+        """rz_service_start: launch the group's persistent raster service
+        (wg_per_cu resident workgroups per CU; it exits after idle_us
+        without work and is relaunched when work is posted)."""
+        self._check(self._L.rz_service_start(self.handle, int(wg_per_cu), int(idle_us)))
+
+    def service_stop(self):
+        """rz_service_stop: stop the group's raster service (no attached
+        context may have a job in flight)."""
+        self._check(self._L.rz_service_stop(self.handle))
+
+    def service_attach(self, on: bool = True):
+        """rz_service_attach: this context's batches post their raster to the
+        group's service (on) or run it themselves (off)."""
+        self._check(self._L.rz_service_attach(self.handle, 1 if on else 0))
+
+    def service_stats(self) -> dict:
+        a = (C.c_uint64 * 4)()
+        self._check(self._L.rz_service_stats(self.handle, a))
+        return dict(on=bool(a[0]), running=bool(a[1]), launches=int(a[2]), grid=int(a[3]))
+
     def set_coarse_bin(self, pixels: int):
         self._check(self._L.rz_set_coarse_bin(self.handle, pixels))
 

@@ -192,16 +192,26 @@ class Server:
 
     def __init__(self, catalog: Catalog, slots: int = 3, budget_bytes: float = 200e9, max_batch: int = 64,
                  max_batch_tris: int = 12_000_000, tuning: dict | None = None, small_cus: int = 0,
-                 small_tris: int = 250_000, small_px: int = 1 << 20):
+                 small_tris: int = 250_000, small_px: int = 1 << 20, service: bool = False,
+                 service_wg_per_cu: int = 2, service_idle_us: int = 50_000):
         """small_cus > 0 (with slots >= 2): slot 0 serves only small requests
         (at most small_tris triangles and small_px pixels) on CUs
         [0, small_cus); the other slots serve any request on the remaining
-        CUs, so a small request never waits behind a long job's workgroups."""
+        CUs, so a small request never waits behind a long job's workgroups.
+        service: the slots post their batches' raster to the group's
+        persistent raster service (rz_service_*), which renders the tiles of
+        all slots' batches in one resident kernel, the batch with the fewest
+        tiles left first."""
         self.catalog = catalog
         self.ctxs = [Context(64, 64, share=catalog.root) for _ in range(slots)]
         for c in self.ctxs:
             if tuning:
                 c.batch_tuning(**tuning)
+        self.service = service
+        if service:
+            catalog.root.service_start(service_wg_per_cu, service_idle_us)
+            for c in self.ctxs:
+                c.service_attach(True)
         self.small_slot = small_cus > 0 and slots >= 2
         self.small_tris, self.small_px = small_tris, small_px
         if self.small_slot:
@@ -222,6 +232,9 @@ class Server:
         for c in self.ctxs:
             c.close()
         self.ctxs = []
+        if self.service:
+            self.catalog.root.service_stop()
+            self.service = False
 
     def is_small(self, tmpl: Template) -> bool:
         return tmpl.scene.ntri <= self.small_tris and tmpl.pixels <= self.small_px
@@ -414,6 +427,7 @@ class Server:
             "batch_size": {"mean": float(np.mean(batches)), "max": int(max(batches))},
             "slots": len(self.ctxs),
             "small_slot": self.small_slot,
+            "service": self.service,
             "max_batches_in_flight": max_concurrent,
             "overlap_share": overlap_polls / max(busy_polls, 1),
             "budget_bytes": self.budget,

@@ -44,7 +44,10 @@ def _alone(cat):
     return out
 
 
-def test_serving_mix_equals_alone():
+@pytest.mark.parametrize("service", [False, True], ids=["own_raster", "raster_service"])
+def test_serving_mix_equals_alone(service):
+    """service: the slots post their rasters to the group's persistent raster
+    service instead of launching them."""
     import torch
 
     from rusterizer_amd.serve import Server, make_requests
@@ -53,15 +56,17 @@ def test_serving_mix_equals_alone():
     try:
         alone = _alone(cat)
         reqs = make_requests(cat, 400, rate=40_000.0, weights={"batch": 0.8, "soup": 0.1, "terrain": 0.1}, seed=7)
-        srv = Server(cat, slots=3, max_batch=12)
+        srv = Server(cat, slots=3, max_batch=12, service=service)
         try:
             rep = srv.run(reqs, keep_outputs=True)
+            if service:
+                assert root.service_stats()["launches"] >= 1
         finally:
             for r in reqs:
                 if r.out is not None:
                     assert torch.equal(r.out, alone[r.tmpl]), (r.rid, cat.templates[r.tmpl].kind)
             srv.close()
-        assert rep["requests"] == 400 and rep["rejected"] == 0
+        assert rep["requests"] == 400 and rep["rejected"] == 0 and rep["service"] == service
         assert set(rep["kinds"]) == {"batch", "soup", "terrain"}
         by_kind = rep["latency_ms_by_kind"]
         assert {k: v["n"] for k, v in by_kind.items()} == rep["kinds"]
@@ -77,14 +82,16 @@ def test_serving_mix_equals_alone():
         root.close()
 
 
-def test_serving_full_size_catalog():
+@pytest.mark.parametrize("service", [False, True], ids=["own_raster", "raster_service"])
+def test_serving_full_size_catalog(service):
     """Serving at its real size: the 256 batched scenes (seed 1002), the
     1M-triangle 1080p soup and the 4M-triangle 4K terrain resident, Poisson
     arrivals at 2000/s (seed 1004, 90% / 7% / 3%), 300 requests with their
-    outputs kept. Every output equals its template rendered alone, and the
-    rendered-alone images of the terrain and of sampled batch scenes equal
-    the oracle's (the 1M soup frame is checked against the oracle by
-    test_soup_full_size)."""
+    outputs kept, with the slots' rasters launched by the slots or run by
+    the group's persistent raster service. Every output equals its template
+    rendered alone, and the rendered-alone images of the terrain and of
+    sampled batch scenes equal the oracle's (the 1M soup frame is checked
+    against the oracle by test_soup_full_size)."""
     import torch
 
     from harness import run_oracle
@@ -97,7 +104,7 @@ def test_serving_full_size_catalog():
     try:
         reqs = make_requests(cat, 300, rate=2000.0, weights={"batch": 0.90, "soup": 0.07, "terrain": 0.03},
                              seed=scenes.SEEDS["serve"])
-        srv = Server(cat, slots=3, max_batch=64)
+        srv = Server(cat, slots=3, max_batch=64, service=service)
         try:
             rep = srv.run(reqs, keep_outputs=True)
         finally:
