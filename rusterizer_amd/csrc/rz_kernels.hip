@@ -1306,6 +1306,26 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+// The largest value over a full wave's 64 lanes, as a wave-uniform value:
+// DPP steps (no LDS round trips) -- pairs and quads within each row of 16
+// lanes, the mirrored half-rows and rows, then row 0's result broadcast into
+// row 1 and rows 0-1's into rows 2-3 -- leave the maximum in lane 63. Every
+// lane must be active.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_max_step(float x) {
+    const int t = __builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), CTRL, ROW_MASK, 0xF, false);
+    return fmaxf(x, __int_as_float(t));
+}
+__device__ __forceinline__ float wave_max_uniform(float v) {
+    v = dpp_max_step<0xB1, 0xF>(v);   // quad_perm [1, 0, 3, 2]
+    v = dpp_max_step<0x4E, 0xF>(v);   // quad_perm [2, 3, 0, 1]
+    v = dpp_max_step<0x141, 0xF>(v);  // row_half_mirror
+    v = dpp_max_step<0x140, 0xF>(v);  // row_mirror
+    v = dpp_max_step<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v = dpp_max_step<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 // raster slots of the one-workgroup-per-tile raster are dealt to the XCDs
 // in runs of this many consecutive slots (one 4x4 block of tiles)
 constexpr uint32_t RASTER_RUN = 16;
@@ -1456,9 +1476,11 @@ __device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, 
 #pragma unroll
     for (int k = 0; k < 4; ++k) win[k * 64 + lane] = make_uint2(NO_PRIM, 0u);
     bool oob = false;
-    // qzmax: largest depth stored in the quadrant's on-screen samples
-    float qzmax = __int_as_float(
-        __builtin_amdgcn_readfirstlane(__float_as_int(wave_max(onscreen ? clear_depth : -FLT_MAX_F))));
+    // qzmax: largest depth stored in the quadrant's on-screen samples (all
+    // cleared to the same depth): the maximum of clear_depth over the
+    // on-screen lanes and -FLT_MAX over the others
+    const uint64_t on = __ballot(onscreen);
+    float qzmax = on == 0 ? -FLT_MAX_F : (on == ~0ull ? clear_depth : fmaxf(clear_depth, -FLT_MAX_F));
     // lane_zbmax: largest depth stored in this pixel's 4 samples
     float lane_zbmax = onscreen ? clear_depth : -FLT_MAX_F;
 
@@ -1638,7 +1660,7 @@ __device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, 
         // stored depths only fall, so a stale qzmax is still an upper bound;
         // it is refreshed once per batch of entries
         if (__ballot(dirty)) {
-            qzmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(wave_max(lane_zbmax))));
+            qzmax = wave_max_uniform(lane_zbmax);
         }
     }
     {
