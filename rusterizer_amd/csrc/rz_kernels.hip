@@ -1753,7 +1753,7 @@ __device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, 
 // The production variants are held to 64 VGPRs (8 waves per SIMD): the
 // extra waves hide more of the walk's load latency than anything the limit
 // costs. The one-workgroup-per-tile variant fits without spilling; the
-// persistent one keeps 64 bytes of its loop state per lane in scratch, and
+// persistent one keeps 40 bytes of its loop state per lane in scratch, and
 // at 8 workgroups per CU runs a quarter faster than unconstrained (6 waves).
 template <bool STATS, bool PERSIST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STATS ? 1 : 8, 8))) void k_raster(
