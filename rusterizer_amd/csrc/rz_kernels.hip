@@ -1487,22 +1487,46 @@ __device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, 
     const uint32_t lbeg = uniform(tr.y), lend = uniform(tr.z);
 
     const PrimGeom* __restrict__ geom = in.geom;
+    // The list is scanned 64 entries at a time with the entries' own bits
+    // (the quadrant's bit, the depth key against qzmax); the entries that
+    // pass queue up in list order (in this wave's shading-colour row, which
+    // the shading stage fills only after the walk), and the record-based
+    // culls and the walk run on full batches of 64 queued candidates (the
+    // rest at the end of the list), so a long list with sparse candidates
+    // pays for them once per 64 candidates rather than once per 64 entries.
+    uint32_t* const cq = sh.argb[wave];
+    uint32_t qn = 0;  // queued candidates (at most 127)
     // each batch's entries are loaded one batch ahead
     uint2 en_next = make_uint2(0u, 0u);
     if (lbeg + lane < lend) en_next = in.flist[lbeg + lane];
-    for (uint32_t base = lbeg; base < lend; base += 64) {
-        const uint32_t i = base + lane;
-        const uint2 en = en_next;
-        if (i + 64 < lend) en_next = in.flist[i + 64];
-        uint32_t myprim = 0;
+    for (uint32_t base = lbeg;; base += 64) {
+        const bool scan = base < lend;
+        if (scan) {
+            const uint32_t i = base + lane;
+            const uint2 en = en_next;
+            if (i + 64 < lend) en_next = in.flist[i + 64];
+            if (STATS) st_entries += min(64u, lend - base);
+            const bool c0 = i < lend && ((en.y >> wave) & 1u) && __uint_as_float(en.y & ~0xFu) < qzmax;
+            const uint64_t mc = __ballot(c0);
+            if (c0) cq[qn + mbcnt(mc)] = en.x;
+            qn += (uint32_t)__popcll(mc);
+            if (qn < 64u) continue;
+        } else if (qn == 0) {
+            break;
+        }
+        // the next batch of queued candidates; the rest (< 64) move to the front
+        const uint32_t take = min(qn, 64u), left = qn - take;
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t myprim = lane < take ? cq[lane] : 0u;
+        if (left != 0) {
+            const uint32_t r = lane < left ? cq[take + lane] : 0u;
+            __builtin_amdgcn_wave_barrier();
+            if (lane < left) cq[lane] = r;
+        }
+        qn = left;
         float lane_zlb = 0.0f;
         uint64_t lane_box = 0;  // quadrant pixels (bit 8 * y + x) inside the prim's bbox
-        bool cand = false;
-        if (STATS) st_entries += min(64u, lend - base);
-        if (i < lend) {
-            myprim = en.x;
-            cand = ((en.y >> wave) & 1u) && __uint_as_float(en.y & ~0xFu) < qzmax;
-        }
+        bool cand = lane < take;
         if (cand) {
             const float4* gp = reinterpret_cast<const float4*>(geom + myprim);
             const float4 g0 = gp[0], g1 = gp[1], g2 = gp[2];  // a0 a1 | a2 n0 | n1 n2
