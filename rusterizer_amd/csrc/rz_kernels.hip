@@ -1491,11 +1491,16 @@ __device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, 
     // (the quadrant's bit, the depth key against qzmax); the entries that
     // pass queue up in list order (in this wave's shading-colour row, which
     // the shading stage fills only after the walk), and the record-based
-    // culls and the walk run on full batches of 64 queued candidates (the
-    // rest at the end of the list), so a long list with sparse candidates
-    // pays for them once per 64 candidates rather than once per 64 entries.
+    // culls and the walk run on batches of queued candidates (the rest at
+    // the end of the list), so a long list with sparse candidates pays for
+    // them once per batch of candidates rather than once per 64 entries.
+    // A batch runs once QUEUE_RUN candidates are queued (up to 64 at once):
+    // waiting for fewer than 64 lets each walk tighten qzmax for the next
+    // entries sooner (batched scenes 1.4% faster than at 64, the long soup
+    // lists 1% slower).
+    constexpr uint32_t QUEUE_RUN = 32;
     uint32_t* const cq = sh.argb[wave];
-    uint32_t qn = 0;  // queued candidates (at most 127)
+    uint32_t qn = 0;  // queued candidates (at most QUEUE_RUN + 63)
     // each batch's entries are loaded one batch ahead
     uint2 en_next = make_uint2(0u, 0u);
     if (lbeg + lane < lend) en_next = in.flist[lbeg + lane];
@@ -1510,7 +1515,7 @@ __device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, 
             const uint64_t mc = __ballot(c0);
             if (c0) cq[qn + mbcnt(mc)] = en.x;
             qn += (uint32_t)__popcll(mc);
-            if (qn < 64u) continue;
+            if (qn < QUEUE_RUN) continue;
         } else if (qn == 0) {
             break;
         }
