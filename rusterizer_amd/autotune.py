@@ -26,9 +26,11 @@ FINE_TILES = (8, 16, 32)
 BIN_TARGETS = (16, 64, 256)
 RASTER_PER_CU = (0, 8)  # 8 persistent workgroups per CU fill it (8 waves per SIMD)
 
-# the usual winner on mixed-resolution batches of textured soups: 16x16 fine
-# tiles, up to 64 coarse bins per scene, one raster workgroup per tile
-DEFAULT = dict(fine_tile=16, bin_target=64, raster_per_cu=0, use_graph=True)
+# the usual winner on mixed-resolution batches of textured soups: 32x32 fine
+# tiles (binning writes a quarter of the entries of 16x16 ones; the raster's
+# candidate queue makes the longer lists cheap to scan), up to 16 coarse bins
+# per scene, one raster workgroup per tile
+DEFAULT = dict(fine_tile=32, bin_target=16, raster_per_cu=0, use_graph=True)
 
 
 def candidates(fine_tiles=FINE_TILES, bin_targets=BIN_TARGETS, raster_per_cu=RASTER_PER_CU, graph=(True,)):

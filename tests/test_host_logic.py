@@ -99,7 +99,8 @@ def test_autotune_keeps_preferred_within_tolerance(monkeypatch):
 
     clock = [0.0]
     monkeypatch.setattr(autotune.time, "perf_counter", lambda: clock[0])
-    fast = dict(fine_tile=32, bin_target=16, raster_per_cu=0, use_graph=True)
+    fast = dict(fine_tile=16, bin_target=64, raster_per_cu=0, use_graph=True)
+    assert fast != autotune.DEFAULT
     for pref_cost, want in ((1.015, autotune.DEFAULT), (1.05, fast)):
         ctx = FakeCtx()
 
