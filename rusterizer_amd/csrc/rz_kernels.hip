@@ -1005,13 +1005,18 @@ __global__ __launch_bounds__(256) void k_fine_prep(JobBufs b) {
 }
 
 // A fine-list entry is 8 bytes: {prim, key}, key = the prim's depth lower
-// bound zlo rounded toward -inf to a multiple of 16 ulps, with the low 4 bits
-// holding which 8x8 quadrants of the tile the prim's bbox overlaps (bit
-// 2*qy + qx). The raster only compares the key against depths it stores, to
-// skip prims, so a rounded-down bound skips no prim it would have kept.
-__device__ __forceinline__ uint32_t zlo_key(uint32_t z) {
-    // negative floats: round the magnitude up; NaN stays NaN
-    return (z & 0x80000000u) ? ((min(z, 0xFFFFFFF0u) + 0xFu) & ~0xFu) : (z & ~0xFu);
+// bound zlo rounded toward -inf to a multiple of 2^QB ulps, with the low QB
+// bits holding which 8x8 quadrants of the fine tile the prim's bbox overlaps:
+// QB = 4 (bit 2*qy + qx of the 16x16 raster tile) for 8- and 16-pixel fine
+// tiles, QB = 16 (bit 4*qy + qx of the 32x32 fine tile) for 32-pixel ones.
+// The raster only compares the key against depths it stores, to skip prims,
+// so a rounded-down bound skips no prim it would have kept.
+__device__ __forceinline__ uint32_t quad_bits_of(uint32_t fts) { return fts == 5u ? 16u : 4u; }
+__device__ __forceinline__ uint32_t zlo_key(uint32_t z, uint32_t qb) {
+    // negative floats: round the magnitude up (a NaN stays a NaN, or -inf for
+    // the lowest negative NaNs: either way no stored depth is above it)
+    const uint32_t low = (1u << qb) - 1u;
+    return (z & 0x80000000u) ? ((min(z, ~low) + low) & ~low) : (z & ~low);
 }
 
 // quadrants of the 16x16 tile at (ox, oy) that the bbox [x0,x1) x [y0,y1)
@@ -1022,16 +1027,21 @@ __device__ __forceinline__ uint32_t quad_mask(uint32_t x0, uint32_t x1, uint32_t
     return ((y0 < oy + 8) ? cx : 0u) | ((y1 > oy + 8) ? cx << 2 : 0u);
 }
 
-// The entry's quadrant bits for a fine tile of side 1 << fts at (ox, oy):
-// which 8x8 quadrant positions of the raster's 16x16 tiles it may touch.
-// 16: the quadrants of that tile; 8: the fine tile is one quadrant of a
-// raster tile; 32: the tile spans four raster tiles, all bits (the raster's
-// bbox test then decides).
+// The entry's quadrant bits for a fine tile of side 1 << fts at (ox, oy),
+// given that the bbox [x0,x1) x [y0,y1) overlaps it. 16: the quadrants of
+// that raster tile; 8: the fine tile is one quadrant of a raster tile; 32:
+// the 8x8 quadrants of the fine tile (a 4x4 grid) that the bbox overlaps.
 __device__ __forceinline__ uint32_t fine_quad_bits(uint32_t fts, uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1,
                                                    uint32_t ox, uint32_t oy) {
     if (fts == 4u) return quad_mask(x0, x1, y0, y1, ox, oy);
     if (fts == 3u) return 1u << (((oy >> 2) & 2u) | ((ox >> 3) & 1u));
-    return 0xFu;
+    const uint32_t qx0 = (max(x0, ox) - ox) >> 3, qx1 = (min(x1, ox + 32u) - 1u - ox) >> 3;
+    const uint32_t qy0 = (max(y0, oy) - oy) >> 3, qy1 = (min(y1, oy + 32u) - 1u - oy) >> 3;
+    const uint32_t row = (2u << qx1) - (1u << qx0);   // columns qx0..qx1 of one row (< 16)
+    const uint32_t rows = (2u << qy1) - (1u << qy0);  // rows qy0..qy1
+    // one bit per selected row at nibble 4*row, times the row's columns
+    const uint32_t spread = (rows & 1u) | ((rows & 2u) << 3) | ((rows & 4u) << 6) | ((rows & 8u) << 9);
+    return spread * row;
 }
 
 // k_fchunk_map, one thread per global coarse bin: writes {bin, scene} for
@@ -1115,7 +1125,7 @@ __global__ __launch_bounds__(256) void k_fine_bin(JobBufs b, const uint2* __rest
         if (SCATTER) {
             __syncthreads();
             const uint32_t* wb = wbase + w * MAX_BINS;
-            const uint32_t key = zlo_key(en.w);
+            const uint32_t key = zlo_key(en.w, quad_bits_of(fts));
             if (valid)
                 for (uint32_t ty = ly0; ty <= ly1; ++ty)
                     for (uint32_t tx = lx0; tx <= lx1; ++tx) {
@@ -1501,6 +1511,10 @@ __device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, 
     constexpr uint32_t QUEUE_RUN = 32;
     uint32_t* const cq = sh.argb[wave];
     uint32_t qn = 0;  // queued candidates (at most QUEUE_RUN + 63)
+    // this quadrant's bit in its fine tile's entries (its position in the
+    // 32x32 fine tile, or in the 16x16 raster tile), and the key's bits
+    const uint32_t qbit = sc.fts == 5u ? ((((qy0 >> 3) & 3u) << 2) | ((qx0 >> 3) & 3u)) : wave;
+    const uint32_t kmask = sc.fts == 5u ? ~0xFFFFu : ~0xFu;
     // each batch's entries are loaded one batch ahead
     uint2 en_next = make_uint2(0u, 0u);
     if (lbeg + lane < lend) en_next = in.flist[lbeg + lane];
@@ -1511,7 +1525,7 @@ __device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, 
             const uint2 en = en_next;
             if (i + 64 < lend) en_next = in.flist[i + 64];
             if (STATS) st_entries += min(64u, lend - base);
-            const bool c0 = i < lend && ((en.y >> wave) & 1u) && __uint_as_float(en.y & ~0xFu) < qzmax;
+            const bool c0 = i < lend && ((en.y >> qbit) & 1u) && __uint_as_float(en.y & kmask) < qzmax;
             const uint64_t mc = __ballot(c0);
             if (c0) cq[qn + mbcnt(mc)] = en.x;
             qn += (uint32_t)__popcll(mc);
