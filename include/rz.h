@@ -304,7 +304,7 @@ int rz_batch_need(const rz_ctx *ctx, const rz_batch_item *items, uint32_t n, uin
  * then re-captures its graph): lets a server give memory back under
  * pressure. Between jobs; meshes, textures and tables stay. */
 int rz_trim(rz_ctx *ctx);
-/* Side in pixels (8, 16 or 32; 0 = 16) of the screen tiles whose primitive
+/* Side in pixels (8, 16 or 32; 0 = 32) of the screen tiles whose primitive
  * lists binning builds and the raster walks (no reference counterpart: the
  * reference walks each triangle's bounding box; it tunes work per list
  * against list count). The raster still works on 16x16 tiles, one wave per
@@ -314,7 +314,7 @@ int rz_trim(rz_ctx *ctx);
  * bin) uses 16. Results do not depend on it. Between jobs; frames and
  * batches. */
 int rz_set_fine_tile(rz_ctx *ctx, uint32_t side);
-/* The side last set (16 by default). */
+/* The side last set (32 by default). */
 uint32_t rz_get_fine_tile(const rz_ctx *ctx);
 /* Enable (1) / disable (0) per-stage HIP event timing; enabling resets sums. */
 int rz_profile(rz_ctx *ctx, int enable);

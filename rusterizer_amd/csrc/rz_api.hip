@@ -128,7 +128,7 @@ struct rz_ctx {
     float next_clear_depth = FLT_MAX;
     Mat3 uni{};
     uint32_t coarse_req = 0;  // plain frame: coarse bin side, 0 = automatic
-    uint32_t fine_req = 16;   // fine-binning tile side asked for (8, 16 or 32; per scene where it fits)
+    uint32_t fine_req = 32;   // fine-binning tile side asked for (8, 16 or 32; per scene where it fits)
     // batch tuning
     uint32_t bin_target = MAX_BINS;  // automatic bins: smallest power-of-2 side with at most this many bins
     uint32_t raster_per_cu = 0;      // persistent raster workgroups per CU (0: one workgroup per tile)
@@ -1629,10 +1629,10 @@ int rz_batch_need(const rz_ctx* c, const rz_batch_item* items, uint32_t n, uint6
 int rz_set_fine_tile(rz_ctx* c, uint32_t side) {
     if (!c) return RZ_ERR_ARG;
     if (side != 0 && side != 8 && side != 16 && side != 32)
-        return fail(c, RZ_ERR_ARG, "fine tile side must be 8, 16 or 32 (0 = 16)");
+        return fail(c, RZ_ERR_ARG, "fine tile side must be 8, 16 or 32 (0 = 32)");
     if (c->recording != JobKind::None || c->in_flight || !c->draws.empty())
         return fail(c, RZ_ERR_STATE, "not between jobs");
-    c->fine_req = side ? side : 16;
+    c->fine_req = side ? side : 32;
     return RZ_OK;
 }
 
