@@ -10,7 +10,9 @@
  *  - Every call returns 0 on success or a negative RZ_ERR_* code; nothing
  *    aborts. rz_last_error() gives the message of the last failure.
  *  - One context per host thread; contexts are independent. All work of a
- *    context runs in order on its own HIP stream (rz_stream()).
+ *    context runs in order on its own HIP stream (rz_stream()). Contexts of
+ *    one group (rz_create_shared) share its meshes, textures and raster
+ *    service: calls on them are made from one host thread, or serialized.
  *  - Host arrays passed in are copied during the call; meshes and textures
  *    are device-resident and owned by the context until destroyed.
  *  - Draws accumulate into the frame until rz_resolve*, which resolves the
