@@ -125,8 +125,8 @@ struct PrimGeom {
     float nx0, ny0, nx1, ny1, nx2, ny2;  // edge normals
     float inv_area;                      // 1 / (2 * area)
     float z0, z1, z2;                    // screen-space depth
-    float w0, w1, w2;                    // clip-space w
     uint32_t flags;                      // tie-rule bit per edge (0..2) | PRIM_* bits | fs << 8 | draw << 16
+    float w0, w1, w2;                    // clip-space w
     uint32_t bbx, bby;                   // clamped pixel bbox: lo | hi << 16
     float zlo;                           // lower bound of every interpolated depth
     float pz0, pzx, pzy;                 // depth lower-bound plane: pz0 + pzx*(x-bx0) + pzy*(y-by0)

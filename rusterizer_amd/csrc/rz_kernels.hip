@@ -1444,6 +1444,23 @@ struct RasterIn {
     const DrawDev* __restrict__ draws;
 };
 
+// The words of a prim record that the list walk reads: the leading 17 of
+// PrimGeom (flags included), loaded with one 16-word and one 1-word scalar
+// load and held in scalar registers while the next record loads.
+struct WalkRec {
+    float ax0, ay0, ax1, ay1, ax2, ay2;
+    float nx0, ny0, nx1, ny1, nx2, ny2;
+    float inv_area;
+    float z0, z1, z2;
+    uint32_t flags;
+};
+static_assert(offsetof(PrimGeom, flags) == 16 * 4, "PrimGeom: the walk's words lead the record");
+
+__device__ __forceinline__ WalkRec walk_rec(const PrimGeom* __restrict__ g) {
+    return WalkRec{g->ax0, g->ay0, g->ax1, g->ay1, g->ax2, g->ay2, g->nx0,   g->ny0, g->nx1,
+                   g->ny1, g->nx2, g->ny2, g->inv_area, g->z0, g->z1, g->z2, g->flags};
+}
+
 // LDS of the deferred shading stage: per wave, its shading tasks {prim,
 // owner lane | pass mask << 6}, their colours, and its task count (a wave's
 // 64 pixels hold at most 256 distinct prims)
@@ -1578,7 +1595,7 @@ __device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, 
         // before their record is fetched
         uint64_t nbox = 0;
         float nzlb = 0.0f;
-        PrimGeom nxt;
+        WalkRec nxt;
         uint32_t nxt_prim = 0;
         auto next_live = [&]() -> bool {
             while (m != 0) {
@@ -1589,7 +1606,7 @@ __device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, 
                 nzlb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lane_zlb), (int)k));
                 if ((__ballot(!(nzlb >= lane_zbmax)) & nbox) != 0) {
                     const uint32_t nprim = (uint32_t)__builtin_amdgcn_readlane((int)myprim, (int)k);
-                    nxt = geom[nprim];  // start loading its record now
+                    nxt = walk_rec(geom + nprim);  // start loading its record now
                     nxt_prim = nprim;
                     return true;
                 }
@@ -1599,10 +1616,14 @@ __device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, 
         if (!next_live()) continue;
         bool dirty = false;
         for (;;) {
-            const PrimGeom gm = nxt;
+            const WalkRec gm = nxt;
             const uint32_t prim = nxt_prim;
             const float zlb = nzlb;  // the prim's depth lower bound over the quadrant
             const uint64_t box = nbox;
+            // this record is waited for before the next one is asked for:
+            // scalar loads return out of order, so a wait for this record
+            // inside the walk would also wait for the next one
+            asm volatile("" ::"s"(gm.ax0), "s"(gm.flags) : "memory");
             // pick the next live candidate now, so its record loads while this
             // prim is processed (chosen with the stored depths before this
             // prim's writes: it may turn out to have no live lane)
@@ -1678,7 +1699,7 @@ __device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, 
                             // fragment's texel reads exactly (the flag covers every
                             // fragment, also ones overwritten later)
                             float u, v, w;
-                            frag_bary(gm, fx, fy, pass, u, v, w);
+                            frag_bary(geom[prim], fx, fy, pass, u, v, w);
                             const float* at = in.uvat + (size_t)prim * UV_STRIDE;
                             const DrawDev& dd = in.draws[gm.flags >> 16];
                             oob |= texel_oob(dd.tw, dd.th, at[0] * u + at[2] * v + at[4] * w,
