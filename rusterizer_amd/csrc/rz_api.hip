@@ -199,6 +199,8 @@ struct rz_ctx {
     uint32_t frame_flags = 0;
     uint64_t frame_stats[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     std::vector<uint32_t> scene_flags;
+    std::vector<double> scene_density;   // build_job's working arrays (kept to avoid reallocation)
+    std::vector<uint32_t> scene_order;
     bool raster_stats = false;  // RZ_RASTER_STATS=1 at rz_create
     bool clip_wide = false;     // RZ_CLIP_WIDE=1 at rz_create: one wave per clipped polygon
     bool exact_paths = false;   // RZ_EXACT_PATHS=1 at rz_create: no per-prim fast-path proofs
@@ -336,6 +338,8 @@ int build_job(rz_ctx* c, JobPlan& pl) {
     pl.nscenes = ns;
     uint64_t tri = 0, tiles = 0, bins = 0;
     uint32_t d = 0;
+    std::vector<double>& dens = c->scene_density;  // input triangles per pixel, for the raster order
+    dens.assign(ns, 0.0);
     for (uint32_t s = 0; s < ns; ++s) {
         const PendingScene& ps = c->scenes[s];
         SceneCfg& g = sc[s];
@@ -378,6 +382,7 @@ int build_job(rz_ctx* c, JobPlan& pl) {
             mats[d] = pd.m;
             tri += ((uint64_t)pd.mesh->ntri + TRI_ALIGN - 1) / TRI_ALIGN * TRI_ALIGN;
             pl.tris += pd.mesh->ntri;
+            dens[s] += pd.mesh->ntri;
             if (tri > 0xFFFF0000ull) return fail(c, RZ_ERR_LIMIT, "more than 2^32 triangles in one job");
         }
         g.tri_end = (uint32_t)tri;
@@ -388,6 +393,20 @@ int build_job(rz_ctx* c, JobPlan& pl) {
     }
     if (d != nd) return fail(c, RZ_ERR_STATE, "draws are not grouped by scene");
     if (tiles > 0xFFFF0000ull || bins > 0xFFFF0000ull) return fail(c, RZ_ERR_LIMIT, "job too large");
+    // raster order: the scenes' tile ranges densest first (ties in scene
+    // order), so that the longest tiles do not trail at the end of the launch
+    std::vector<uint32_t>& order = c->scene_order;
+    order.resize(ns);
+    for (uint32_t s = 0; s < ns; ++s) {
+        order[s] = s;
+        dens[s] /= (double)sc[s].W * sc[s].H;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return dens[a] > dens[b]; });
+    uint32_t rb = 0;
+    for (uint32_t s : order) {
+        sc[s].raster_base = rb;
+        rb += ((sc[s].W + 15) / 16) * ((sc[s].H + 15) / 16);
+    }
     pl.ntri = (uint32_t)tri;
     pl.ntiles = (uint32_t)tiles;
     pl.nbins = (uint32_t)bins;

@@ -98,7 +98,7 @@ struct SceneCfg {
     uint32_t bin_base;        // first global coarse bin of the scene
     uint32_t tri_begin, tri_end;  // padded triangle range of the scene's draws
     uint32_t fts;             // fine-binning tile side = 1 << fts pixels (8, 16 or 32; divides cb)
-    uint32_t pad1;
+    uint32_t raster_base;     // first raster slot of the scene's tiles (the host orders scenes densest first)
     uint32_t* out;            // resolved ARGB, W*H words (device)
     uint32_t* msaa_color;     // optional W*H*4 pre-resolve colours
     float* msaa_depth;        // optional W*H*4 pre-resolve depths

@@ -1162,8 +1162,11 @@ __device__ __forceinline__ uint32_t blocked_rank(uint32_t tx, uint32_t ty, uint3
 // next tile, so the raster starts with one load instead of a chain of
 // searches and lookups. Ranges are clamped to the list buffer: a bad range
 // can never read past it. Records are stored in raster order: the tile of
-// scene rank r (blocked_rank) at tile_base + r, with the tile's own index,
-// so that consecutive raster slots are 4x4 squares of the screen.
+// scene rank r (blocked_rank) at raster_base + r, with the tile's own index,
+// so that consecutive raster slots are 4x4 squares of the screen. The host
+// gives the scenes of a batch their slot ranges densest first (triangles per
+// pixel), so the longest tiles start early rather than trailing at the end
+// of the launch.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_tile_prep(JobBufs b) {
     const uint32_t nt = min(b.head->ntiles, b.tile_cap), ns = b.head->nscenes;
@@ -1185,7 +1188,7 @@ __global__ __launch_bounds__(256) void k_tile_prep(JobBufs b) {
         const size_t s_idx = (size_t)hi32(b.fpre[g]) + (size_t)lt * nchb, e_idx = s_idx + nchb;
         const uint32_t lend = min(e_idx < fmat_len ? b.fine_off[e_idx] : ftot, b.fine_cap);
         const uint32_t lbeg = min(s_idx < fmat_len ? b.fine_off[s_idx] : ftot, lend);
-        const uint32_t slot = sc.tile_base + blocked_rank(tx, ty, ntx, nty);
+        const uint32_t slot = sc.raster_base + blocked_rank(tx, ty, ntx, nty);
         if (slot < b.tile_cap) b.tile_rec[4u * slot + q] = make_uint4(s, lbeg, lend, gt);
     }
 }
