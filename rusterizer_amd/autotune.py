@@ -6,7 +6,7 @@ capacity, i.e. the largest number of coarse bins per scene when the bin side
 is chosen automatically (the side is the smallest power of two >= 16 px
 giving at most that many bins, so it sets how much screen -- and how many
 primitives -- one coarse bin holds; 8x8 tiles cap the side at 128 px); the
-number of persistent raster workgroups per CU (0 = one workgroup per tile);
+number of persistent raster workgroups per CU (0 = one one-wave workgroup per 8x8 quadrant);
 and whether the batch replays its captured HIP graph (rz_batch_tuning).
 Bin lists have no fixed capacity: binning counts, scans and then writes
 each list at its exact length. Every setting renders bit-identical images;

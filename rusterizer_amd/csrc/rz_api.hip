@@ -131,7 +131,7 @@ struct rz_ctx {
     uint32_t fine_req = 32;   // fine-binning tile side asked for (8, 16 or 32; per scene where it fits)
     // batch tuning
     uint32_t bin_target = MAX_BINS;  // automatic bins: smallest power-of-2 side with at most this many bins
-    uint32_t raster_per_cu = 0;      // persistent raster workgroups per CU (0: one workgroup per tile)
+    uint32_t raster_per_cu = 0;      // persistent raster workgroups per CU (0: one workgroup per quadrant)
     bool use_graph = true;
     int n_cu = 256;
 
@@ -473,7 +473,7 @@ int size_buffers(rz_ctx* c, const JobPlan& pl, const Needs& n_in, G& g) {
     g(c->pbin, geom);
     g(c->tchunk_draw, cls / TRI_ALIGN + 1);
     g(c->pchunk_scene, geom / CHUNK + 1);
-    if (c->raster_stats) g(c->rstat_parts, 5ull * ((uint64_t)pl.ntiles + 128 + 32ull * (uint32_t)c->n_cu));
+    if (c->raster_stats) g(c->rstat_parts, 5ull * (4ull * pl.ntiles + 512 + 32ull * (uint32_t)c->n_cu));
     const size_t clip = g(c->clip_tri, n.clip);
     g(c->clip_redo, clip);
     g(c->clip_draw, clip);

@@ -194,7 +194,7 @@ struct JobGrid {
     uint32_t clip;       // k_clip waves
     uint32_t bins;       // k_bin blocks
     uint32_t fine;       // k_fine_bin blocks
-    uint32_t raster;     // persistent raster workgroups; 0 = one workgroup per tile
+    uint32_t raster;     // persistent raster workgroups; 0 = one one-wave workgroup per quadrant
     uint32_t ntiles;     // tiles the one-per-tile raster grid must cover (>= the job's)
     uint32_t scenes;     // scene count (k_frame_reset / k_scene_prep loops)
 };

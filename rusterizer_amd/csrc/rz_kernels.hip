@@ -1202,10 +1202,11 @@ __global__ __launch_bounds__(256) void k_tile_prep(JobBufs b) {
 // and writes the ARGB word once.
 //
 // Tiles of all scenes form one index space of raster slots (each scene's
-// tiles in 4x4 blocks, k_tile_prep). With a grid of one workgroup per tile
-// the order is static; with a smaller (persistent) grid the
-// workgroups pull tiles from 8 queues, one band of tiles per XCD, and move
-// on to the other bands when theirs is empty.
+// tiles in 4x4 blocks, k_tile_prep). With a grid of one single-wave
+// workgroup per quadrant (k_raster_q) the order is static; with a smaller
+// (persistent) grid of four-wave workgroups (k_raster) the workgroups pull
+// tiles from 8 queues, one band of tiles per XCD, and move on to the other
+// bands when theirs is empty.
 //
 // Work that provably cannot change a sample is skipped (exact culls):
 //  - list entries whose bbox misses the quadrant;
@@ -1339,15 +1340,9 @@ __device__ __forceinline__ float wave_max_uniform(float v) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
-// raster slots of the one-workgroup-per-tile raster are dealt to the XCDs
-// in runs of this many consecutive slots (one 4x4 block of tiles)
+// raster slots are dealt to the XCDs in runs of this many consecutive
+// slots (one 4x4 block of tiles; k_raster_q deals their quadrants)
 constexpr uint32_t RASTER_RUN = 16;
-
-// workgroups that cover nt tiles in runs of RASTER_RUN dealt to 8 XCDs
-static uint32_t raster_grid(uint32_t nt) {
-    const uint32_t runs = (nt + RASTER_RUN - 1) / RASTER_RUN;
-    return std::max<uint32_t>(1u, (runs + 7) / 8 * 8 * RASTER_RUN);
-}
 
 // band x of the persistent queue: tiles [band_lo(x), band_lo(x + 1))
 __device__ __forceinline__ uint32_t band_lo(uint32_t nt, uint32_t x) {
@@ -1464,22 +1459,28 @@ __device__ __forceinline__ WalkRec walk_rec(const PrimGeom* __restrict__ g) {
                    g->ny1, g->nx2, g->ny2, g->inv_area, g->z0, g->z1, g->z2, g->flags};
 }
 
-// LDS of the deferred shading stage: per wave, its shading tasks {prim,
-// owner lane | pass mask << 6}, their colours, and its task count (a wave's
-// 64 pixels hold at most 256 distinct prims)
+// LDS of the deferred shading stage: per wave (R waves), its shading tasks
+// {prim, owner lane | pass mask << 6}, their colours, and its task count (a
+// wave's 64 pixels hold at most 256 distinct prims)
+template <uint32_t R>
 struct ShadeLds {
-    uint2 task[4][256];
-    uint32_t argb[4][256];
-    uint32_t count[4];
+    uint2 task[R][256];
+    uint32_t argb[R][256];
+    uint32_t count[R];
 };
 
-template <bool STATS>
-__device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, uint32_t slot, const float* lut,
-                                            ShadeLds& sh,
+// R = 4: the workgroup's four waves take the tile's four quadrants and
+// share the shading tasks; R = 1: the workgroup is one wave and takes
+// quadrant quad of the tile alone.
+template <bool STATS, uint32_t R>
+__device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, uint32_t slot, uint32_t quad,
+                                            const float* lut, ShadeLds<R>& sh,
                                             uint32_t& st_entries, uint32_t& st_cand, uint32_t& st_hits,
                                             uint32_t& st_cov, uint32_t& st_shaded) {
     // the tile's scene and this wave's list range (k_tile_prep)
-    const uint32_t wave = uniform(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    // wave: the quadrant; row: this wave's rows of the shading LDS
+    const uint32_t wave = R == 1 ? quad : uniform(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const uint32_t row = R == 1 ? 0u : wave;
     const uint4 tr = in.tile_rec[4u * slot + wave];
     const uint32_t s = uniform(tr.x), gtile = uniform(tr.w);
     const SceneCfg& sc = in.scenes[s];
@@ -1502,7 +1503,7 @@ __device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, 
     // shading-task row (sample k of the lane at k * 64 + lane) rather than
     // in registers, which keeps the walk's register count down.
     float zb0 = clear_depth, zb1 = clear_depth, zb2 = clear_depth, zb3 = clear_depth;
-    uint2* const win = sh.task[wave];
+    uint2* const win = sh.task[row];
 #pragma unroll
     for (int k = 0; k < 4; ++k) win[k * 64 + lane] = make_uint2(NO_PRIM, 0u);
     bool oob = false;
@@ -1529,7 +1530,7 @@ __device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, 
     // entries sooner (batched scenes 1.4% faster than at 64, the long soup
     // lists 1% slower).
     constexpr uint32_t QUEUE_RUN = 32;
-    uint32_t* const cq = sh.argb[wave];
+    uint32_t* const cq = sh.argb[row];
     uint32_t qn = 0;  // queued candidates (at most QUEUE_RUN + 63)
     // this quadrant's bit in its fine tile's entries (its position in the
     // 32x32 fine tile, or in the 16x16 raster tile), and the key's bits
@@ -1757,19 +1758,39 @@ __device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, 
     const uint64_t np0 = __ballot(ntask & 1u), np1 = __ballot(ntask & 2u), np2 = __ballot(ntask & 4u);
     const uint32_t s0 = mbcnt(np0) + 2u * mbcnt(np1) + 4u * mbcnt(np2);
     const uint32_t s1 = s0 + (uint32_t)hd0, s2 = s1 + (uint32_t)hd1, s3 = s2 + (uint32_t)hd2;
-    uint2* const wtask = sh.task[wave];
+    uint2* const wtask = sh.task[row];
     if (hd0) wtask[s0] = make_uint2(pr0, lane | (pm & 0xFu) << 6);
     if (hd1) wtask[s1] = make_uint2(pr1, lane | ((pm >> 4) & 0xFu) << 6);
     if (hd2) wtask[s2] = make_uint2(pr2, lane | ((pm >> 8) & 0xFu) << 6);
     if (hd3) wtask[s3] = make_uint2(pr3, lane | ((pm >> 12) & 0xFu) << 6);
-    if (lane == 0)
-        sh.count[wave] = (uint32_t)__popcll(np0) + 2u * (uint32_t)__popcll(np1) + 4u * (uint32_t)__popcll(np2);
+    const uint32_t own = uniform((uint32_t)__popcll(np0) + 2u * (uint32_t)__popcll(np1) + 4u * (uint32_t)__popcll(np2));
+    if (R > 1 && lane == 0) sh.count[row] = own;
     // each sample's task: that of its prim's first sample (8-bit slots)
-    const uint32_t slots = s0 | (h1 ? s1 : s0) << 8 | (h2 ? s2 : (pr2 == pr0 ? s0 : s1)) << 16 |
+    uint32_t slots = s0 | (h1 ? s1 : s0) << 8 | (h2 ? s2 : (pr2 == pr0 ? s0 : s1)) << 16 |
                            (h3 ? s3 : (pr3 == pr0 ? s0 : (pr3 == pr1 ? s1 : s2))) << 24;
-    const uint32_t vmask = (uint32_t)v0 | (uint32_t)v1 << 1 | (uint32_t)v2 << 2 | (uint32_t)v3 << 3;
-    __syncthreads();
-    {
+    uint32_t vmask = (uint32_t)v0 | (uint32_t)v1 << 1 | (uint32_t)v2 << 2 | (uint32_t)v3 << 3;
+    // formed here, so that their inputs are dead during the shading loop
+    asm volatile("" : "+v"(slots), "+v"(vmask));
+    if (R == 1) {
+        // one wave: its own tasks, 64 at a time (LDS accesses of one wave
+        // stay in order; the fences keep the compiler from moving them)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (uint32_t g = lane; g < own; g += 64u) {
+            const uint2 t = sh.task[0][g];
+            const uint32_t P = t.x, ol = t.y & 63u, pass = t.y >> 6;
+            const float ofx = (float)(qx0 + (ol & 7u)), ofy = (float)(qy0 + (ol >> 3));
+            const PrimGeom gp = geom[P];
+            sh.argb[0][g] = shade_fragment(gp, in.attrs + (size_t)P * ATTR_STRIDE, in.uvat + (size_t)P * UV_STRIDE,
+                                           in.draws, ofx, ofy, pass, lut, oob);
+            if (STATS) st_shaded += 1;  // shading tasks run by this lane
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    } else {
+        __syncthreads();
         const uint32_t p1 = uniform(sh.count[0]), p2 = p1 + uniform(sh.count[1]);
         const uint32_t p3 = p2 + uniform(sh.count[2]), tot = p3 + uniform(sh.count[3]);
         for (uint32_t gb = wave * 64u; gb < tot; gb += 256u) {
@@ -1790,9 +1811,9 @@ __device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, 
                 if (STATS) st_shaded += 1;  // shading tasks run by this lane
             }
         }
+        __syncthreads();
     }
-    __syncthreads();
-    const uint32_t* const wargb = sh.argb[wave];
+    const uint32_t* const wargb = sh.argb[row];
     const uint32_t cb0 = (vmask & 1u) ? wargb[slots & 0xFFu] : CLEAR_COLOR;
     const uint32_t cb1 = (vmask & 2u) ? wargb[(slots >> 8) & 0xFFu] : CLEAR_COLOR;
     const uint32_t cb2 = (vmask & 4u) ? wargb[(slots >> 16) & 0xFFu] : CLEAR_COLOR;
@@ -1817,12 +1838,13 @@ __device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, 
     }
 }
 
-// The production variants are held to 64 VGPRs (8 waves per SIMD): the
-// extra waves hide more of the walk's load latency than anything the limit
-// costs. The one-workgroup-per-tile variant fits without spilling; the
-// persistent one keeps 40 bytes of its loop state per lane in scratch, and
-// at 8 workgroups per CU runs a quarter faster than unconstrained (6 waves).
-template <bool STATS, bool PERSIST>
+// Persistent raster (raster_per_cu > 0): raster_per_cu workgroups of four
+// waves per CU, each wave one quadrant of the tile its workgroup holds;
+// contiguous bands of tiles become 8 queues, drained XCD-local first. Held
+// to 64 VGPRs (8 waves per SIMD), it keeps 40 bytes of its loop state per
+// lane in scratch, and at 8 workgroups per CU runs a quarter faster than
+// unconstrained (6 waves).
+template <bool STATS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STATS ? 1 : 8, 8))) void k_raster(
     JobBufs b, const PrimGeom* __restrict__ geom, const float* __restrict__ attrs, const float* __restrict__ uvat,
     const uint2* __restrict__ flist,
@@ -1830,51 +1852,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STATS ? 1 :
     const RasterIn in{geom, attrs, uvat, flist, tile_rec, scenes, draws};
     __shared__ float lut[256];
     __shared__ uint32_t next_tile;
-    __shared__ ShadeLds sh;
+    __shared__ ShadeLds<4> sh;
     lut[threadIdx.x] = (float)threadIdx.x / 255.0f;  // color.rs:24-27
     __syncthreads();
     if (uniform(b.cnt->overflow) != 0) return;
     uint32_t st_entries = 0, st_cand = 0, st_hits = 0, st_cov = 0, st_shaded = 0;
     const uint32_t nt = b.head->ntiles;
-    const uint32_t blk = blockIdx.x, xcd = blk & 7u;
-    // one workgroup per tile: workgroups go round-robin to the 8 XCDs, and
-    // the slots are dealt to them in runs of RASTER_RUN consecutive slots,
-    // a 4x4 square of tiles (workgroup 8k + x takes the k-th slot of XCD x's
-    // runs), so each XCD renders neighbouring tiles (the prims they share
-    // stay in its L2) while every XCD gets work from all over the job at the
-    // same time: the dispatcher hands out workgroups in order, so an XCD
-    // that lags behind (a band of dense tiles next to bands of empty sky)
-    // would hold up all of them. Any grid of at least raster_grid(nt)
-    // workgroups covers every slot; extra ones exit.
-    // persistent: contiguous bands of tiles become 8 queues, drained
-    // XCD-local first
-    uint32_t tile = NO_TILE;
-    if (!PERSIST) {
-        const uint32_t k = blk >> 3;
-        const uint32_t t = ((k / RASTER_RUN) * 8u + xcd) * RASTER_RUN + k % RASTER_RUN;
-        if (t < nt) tile = t;
-    }
-    for (bool first = true;; first = false) {
-        if (PERSIST) {
-            if (threadIdx.x == 0) {
-                uint32_t got = NO_TILE;
-                for (uint32_t k = 0; k < 8 && got == NO_TILE; ++k) {
-                    const uint32_t x = (xcd + k) & 7u;
-                    const uint32_t lo = band_lo(nt, x), n = band_lo(nt, x + 1) - lo;
-                    if (n == 0) continue;
-                    const uint32_t i = atomicAdd(&b.cnt->q[x], 1u);
-                    if (i < n) got = lo + i;
-                }
-                next_tile = got;
+    const uint32_t xcd = blockIdx.x & 7u;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            uint32_t got = NO_TILE;
+            for (uint32_t k = 0; k < 8 && got == NO_TILE; ++k) {
+                const uint32_t x = (xcd + k) & 7u;
+                const uint32_t lo = band_lo(nt, x), n = band_lo(nt, x + 1) - lo;
+                if (n == 0) continue;
+                const uint32_t i = atomicAdd(&b.cnt->q[x], 1u);
+                if (i < n) got = lo + i;
             }
-            __syncthreads();
-            tile = next_tile;
-            __syncthreads();
-        } else if (!first) {
-            tile = NO_TILE;
+            next_tile = got;
         }
+        __syncthreads();
+        const uint32_t tile = next_tile;
+        __syncthreads();
         if (tile == NO_TILE) break;
-        raster_tile<STATS>(b.runs, in, tile, lut, sh, st_entries, st_cand, st_hits, st_cov, st_shaded);
+        raster_tile<STATS, 4>(b.runs, in, tile, 0u, lut, sh, st_entries, st_cand, st_hits, st_cov, st_shaded);
     }
     if (STATS) {  // the last two counts are per lane
         for (int o = 32; o >= 1; o >>= 1) {
@@ -1902,6 +1903,61 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STATS ? 1 :
             else atomicAdd(&b.cnt->rstat[threadIdx.x], v);
         }
     }
+}
+
+// The raster (raster_per_cu = 0): one workgroup of one wave per 8x8
+// quadrant. The quadrants of a tile are independent (each walks its own list
+// range and shades its own pixels), so a wave that finishes early frees its
+// slot at once instead of waiting at a barrier for the tile's slowest
+// quadrant. Quadrant slots (4 per raster slot, in the same order) are dealt
+// to the XCDs in runs of 4 * RASTER_RUN, a 4x4 square of tiles: workgroups
+// go round-robin to the 8 XCDs (workgroup 8k + x takes the k-th quadrant of
+// XCD x's runs), so each XCD renders neighbouring tiles (the prims they
+// share stay in its L2) while every XCD gets work from all over the job at
+// the same time -- the dispatcher hands out workgroups in order, so an XCD
+// that lags behind (a band of dense tiles next to bands of empty sky) would
+// hold up all of them. Any grid of at least raster_grid_q(nt) workgroups
+// covers every quadrant; extra ones exit. Held to 64 VGPRs (8 waves per
+// SIMD): the extra waves hide more of the walk's load latency than anything
+// the limit costs.
+template <bool STATS>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(STATS ? 1 : 8, 8))) void k_raster_q(
+    JobBufs b, const PrimGeom* __restrict__ geom, const float* __restrict__ attrs, const float* __restrict__ uvat,
+    const uint2* __restrict__ flist,
+    const uint4* __restrict__ tile_rec, const SceneCfg* __restrict__ scenes, const DrawDev* __restrict__ draws) {
+    const RasterIn in{geom, attrs, uvat, flist, tile_rec, scenes, draws};
+    __shared__ float lut[256];
+    __shared__ ShadeLds<1> sh;
+    for (uint32_t i = threadIdx.x; i < 256u; i += 64u) lut[i] = (float)i / 255.0f;  // color.rs:24-27
+    __syncthreads();
+    if (uniform(b.cnt->overflow) != 0) return;
+    constexpr uint32_t QRUN = 4u * RASTER_RUN;
+    const uint32_t nq = 4u * b.head->ntiles;
+    const uint32_t k = blockIdx.x >> 3, xcd = blockIdx.x & 7u;
+    const uint32_t qs = ((k / QRUN) * 8u + xcd) * QRUN + k % QRUN;
+    uint32_t st_entries = 0, st_cand = 0, st_hits = 0, st_cov = 0, st_shaded = 0;
+    if (qs < nq)
+        raster_tile<STATS, 1>(b.runs, in, qs >> 2, qs & 3u, lut, sh, st_entries, st_cand, st_hits, st_cov, st_shaded);
+    if (STATS) {  // every workgroup writes its counts, zeros past the last quadrant
+        for (int o = 32; o >= 1; o >>= 1) {  // the last two counts are per lane
+            st_cov += __shfl_xor(st_cov, o);
+            st_shaded += __shfl_xor(st_shaded, o);
+        }
+        if (threadIdx.x < 5) {
+            const unsigned long long v = threadIdx.x == 0 ? st_entries
+                                         : threadIdx.x == 1 ? st_cand
+                                         : threadIdx.x == 2 ? st_hits
+                                         : threadIdx.x == 3 ? st_cov
+                                                            : st_shaded;
+            if (blockIdx.x < b.rstat_cap) b.rstat_parts[(size_t)blockIdx.x * 5 + threadIdx.x] = v;
+            else atomicAdd(&b.cnt->rstat[threadIdx.x], v);
+        }
+    }
+}
+
+static uint32_t raster_grid_q(uint32_t nt) {
+    const uint32_t runs = (4u * nt + 4u * RASTER_RUN - 1) / (4u * RASTER_RUN);
+    return std::max<uint32_t>(1u, (runs + 7) / 8 * 8 * 4u * RASTER_RUN);
 }
 
 // stats variant: the raster workgroups' counts (nwg workgroups) into the counters
@@ -2059,7 +2115,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     SvcEntry* ents, uint32_t nent, SvcCtl* ctl, SvcHost* host, unsigned long long idle_ticks,
     unsigned long long life_ticks) {
     __shared__ float lut[256];
-    __shared__ ShadeLds sh;
+    __shared__ ShadeLds<4> sh;
     __shared__ uint32_t pick[6];
     __shared__ uint32_t acq_gen[SVC_MAX_ENTRIES];  // thread 0: the job of each entry it has acquired
     __shared__ uint32_t acq_nt[SVC_MAX_ENTRIES];   // and that job's tile count
@@ -2200,7 +2256,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                           svc_ptr<uint2>(&j->flist),   svc_ptr<uint4>(&j->tile_rec), svc_ptr<SceneCfg>(&j->scenes),
                           svc_ptr<DrawDev>(&j->draws)};
         SceneRun* runs = svc_ptr_mut<SceneRun>(&j->runs);
-        raster_tile<false>(runs, in, t, lut, sh, st0, st1, st2, st3, st4);
+        raster_tile<false, 4>(runs, in, t, 0u, lut, sh, st0, st1, st2, st3, st4);
         pend_e = e;
         pend_g = g;
         pend_nt = nt;
@@ -2248,16 +2304,19 @@ void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_s
         k_svc_post<<<1, 64, 0, s>>>(b, svc, svc_host, svc_ctl, svc_index);
         return;
     }
-    const uint32_t rgrid = g.raster ? g.raster : raster_grid(g.ntiles);
-#define RZ_RASTER(ST, PE) \
-    k_raster<ST, PE><<<rgrid, 256, 0, s>>>(b, b.geom, b.attrs, b.uvat, b.flist, b.tile_rec, b.scenes, b.draws)
+    // persistent: g.raster workgroups of four waves; else one one-wave
+    // workgroup per quadrant
+    const uint32_t rgrid = g.raster ? g.raster : raster_grid_q(g.ntiles);
+#define RZ_RASTER(ST)                                                                                              \
+    if (g.raster)                                                                                                  \
+        k_raster<ST><<<rgrid, 256, 0, s>>>(b, b.geom, b.attrs, b.uvat, b.flist, b.tile_rec, b.scenes, b.draws);    \
+    else                                                                                                           \
+        k_raster_q<ST><<<rgrid, 64, 0, s>>>(b, b.geom, b.attrs, b.uvat, b.flist, b.tile_rec, b.scenes, b.draws)
     if (raster_stats) {
-        if (g.raster) RZ_RASTER(true, true);
-        else RZ_RASTER(true, false);
+        RZ_RASTER(true);
         k_rstat_reduce<<<64, 256, 0, s>>>(b, rgrid);
     } else {
-        if (g.raster) RZ_RASTER(false, true);
-        else RZ_RASTER(false, false);
+        RZ_RASTER(false);
     }
 #undef RZ_RASTER
 }
