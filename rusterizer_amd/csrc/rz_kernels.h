@@ -168,7 +168,7 @@ struct JobBufs {
     uint32_t* fine_counts;
     uint32_t* fine_off;
     uint2* flist;
-    uint4* tile_rec;    // per raster tile and 8x8 quadrant q, at 4 * tile + q: {scene, list begin, list end, 0}
+    uint4* tile_rec;    // per raster slot and 8x8 quadrant q, at 4 * slot + q: {scene, list begin, list end, x | y << 16}
     uint2* fchunk_rec;  // per fine-binning chunk: {global coarse bin, scene}
     uint32_t* tchunk_draw;   // per TRI_ALIGN-triangle chunk: its draw
     uint32_t* pchunk_scene;  // per CHUNK-prim binning chunk: its scene

@@ -1162,8 +1162,9 @@ __device__ __forceinline__ uint32_t blocked_rank(uint32_t tx, uint32_t ty, uint3
 // next tile, so the raster starts with one load instead of a chain of
 // searches and lookups. Ranges are clamped to the list buffer: a bad range
 // can never read past it. Records are stored in raster order: the tile of
-// scene rank r (blocked_rank) at raster_base + r, with the tile's own index,
-// so that consecutive raster slots are 4x4 squares of the screen. The host
+// scene rank r (blocked_rank) at raster_base + r, each quadrant's record
+// {scene, list begin, list end, first pixel x | y << 16}, so that
+// consecutive raster slots are 4x4 squares of the screen. The host
 // gives the scenes of a batch their slot ranges densest first (triangles per
 // pixel), so the longest tiles start early rather than trailing at the end
 // of the launch.
@@ -1189,7 +1190,7 @@ __global__ __launch_bounds__(256) void k_tile_prep(JobBufs b) {
         const uint32_t lend = min(e_idx < fmat_len ? b.fine_off[e_idx] : ftot, b.fine_cap);
         const uint32_t lbeg = min(s_idx < fmat_len ? b.fine_off[s_idx] : ftot, lend);
         const uint32_t slot = sc.raster_base + blocked_rank(tx, ty, ntx, nty);
-        if (slot < b.tile_cap) b.tile_rec[4u * slot + q] = make_uint4(s, lbeg, lend, gt);
+        if (slot < b.tile_cap) b.tile_rec[4u * slot + q] = make_uint4(s, lbeg, lend, px | py << 16);
     }
 }
 
@@ -1482,14 +1483,13 @@ __device__ __forceinline__ void raster_tile(SceneRun* runs, const RasterIn& in, 
     const uint32_t wave = R == 1 ? quad : uniform(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const uint32_t row = R == 1 ? 0u : wave;
     const uint4 tr = in.tile_rec[4u * slot + wave];
-    const uint32_t s = uniform(tr.x), gtile = uniform(tr.w);
+    const uint32_t s = uniform(tr.x);
     const SceneCfg& sc = in.scenes[s];
     const uint32_t W = sc.W, H = sc.H;
     const float clear_depth = sc.clear_depth;
-    const uint32_t ntx = (W + 15) / 16;
-    const uint32_t tile = gtile - sc.tile_base;
-    const uint32_t tx = tile % ntx, ty = tile / ntx;
-    const uint32_t qx0 = tx * 16 + (wave & 1) * 8, qy0 = ty * 16 + (wave >> 1) * 8;
+    // the quadrant's first pixel (k_tile_prep), and its tile's
+    const uint32_t qx0 = uniform(tr.w) & 0xFFFFu, qy0 = uniform(tr.w) >> 16;
+    const uint32_t tx = qx0 >> 4, ty = qy0 >> 4;
     const uint32_t px = qx0 + (lane & 7), py = qy0 + (lane >> 3);
     const float fx = (float)px, fy = (float)py;
     const bool onscreen = px < W && py < H;
