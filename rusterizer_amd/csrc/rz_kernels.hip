@@ -1246,6 +1246,19 @@ struct RGBA {
     float r, g, b, a;
 };
 
+// color.rs:24-27: channel byte b as b / 255.0, correctly rounded (the
+// compiler folds the divisions with IEEE round-to-nearest, as the device's
+// division rounds). The raster copies it into LDS at start.
+struct alignas(16) ByteToFloat {
+    float v[256];
+};
+constexpr ByteToFloat make_byte_to_float() {
+    ByteToFloat t{};
+    for (int i = 0; i < 256; ++i) t.v[i] = (float)i / 255.0f;
+    return t;
+}
+__device__ constexpr ByteToFloat kByteToFloat = make_byte_to_float();
+
 // texture.rs:65-83 (channels via color.rs:22-29, byte / 255.0 from a table)
 // checked: bounds-checked texel reads (texture.rs:47-63); otherwise the prim's
 // coordinates are proven in bounds and the index x + y*tw fits 32 bits.
@@ -1854,7 +1867,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STATS ? 1 :
     __shared__ float lut[256];
     __shared__ uint32_t next_tile;
     __shared__ ShadeLds<4> sh;
-    lut[threadIdx.x] = (float)threadIdx.x / 255.0f;  // color.rs:24-27
+    lut[threadIdx.x] = kByteToFloat.v[threadIdx.x];
     __syncthreads();
     if (uniform(b.cnt->overflow) != 0) return;
     uint32_t st_entries = 0, st_cand = 0, st_hits = 0, st_cov = 0, st_shaded = 0;
@@ -1927,9 +1940,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(STATS ? 1 : 
     const uint2* __restrict__ flist,
     const uint4* __restrict__ tile_rec, const SceneCfg* __restrict__ scenes, const DrawDev* __restrict__ draws) {
     const RasterIn in{geom, attrs, uvat, flist, tile_rec, scenes, draws};
-    __shared__ float lut[256];
+    __shared__ __align__(16) float lut[256];
     __shared__ ShadeLds<1> sh;
-    for (uint32_t i = threadIdx.x; i < 256u; i += 64u) lut[i] = (float)i / 255.0f;  // color.rs:24-27
+    reinterpret_cast<float4*>(lut)[threadIdx.x] = reinterpret_cast<const float4*>(kByteToFloat.v)[threadIdx.x];
     __syncthreads();
     if (uniform(b.cnt->overflow) != 0) return;
     constexpr uint32_t QRUN = 4u * RASTER_RUN;
@@ -2121,7 +2134,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     __shared__ uint32_t acq_gen[SVC_MAX_ENTRIES];  // thread 0: the job of each entry it has acquired
     __shared__ uint32_t acq_nt[SVC_MAX_ENTRIES];   // and that job's tile count
     constexpr uint32_t NONE = 0xFFFFFFFFu, EXIT = 0xFFFFFFFEu, RETRY = 0xFFFFFFFDu;
-    lut[threadIdx.x] = (float)threadIdx.x / 255.0f;  // color.rs:24-27
+    lut[threadIdx.x] = kByteToFloat.v[threadIdx.x];
     if (threadIdx.x < SVC_MAX_ENTRIES) acq_gen[threadIdx.x] = NONE;
     const uint32_t xcd = blockIdx.x & 7u;  // workgroups are dealt to the 8 XCDs in turn
     const unsigned long long t0 = (unsigned long long)wall_clock64();
