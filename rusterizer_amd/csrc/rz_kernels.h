@@ -72,7 +72,7 @@ struct JobHead {
 
 // Per-draw state on the device (mesh streams, fragment shader, MVP).
 struct DrawDev {
-    float M[16];             // ((P*V)*W), written by k_mvp
+    float M[16];             // ((P*V)*W), written by k_job_init
     const float* pos;        // nv*3
     const float* col;        // nv*4 or null
     const float* uv;         // nv*2 or null
@@ -189,14 +189,14 @@ struct JobBufs {
 // device counts, so any grid >= 1 is correct; larger grids only add idle
 // workgroups.
 struct JobGrid {
-    uint32_t draws;      // k_mvp blocks
+    uint32_t draws;      // draw blocks (k_job_init)
     uint32_t tris;       // k_classify / k_setup blocks
     uint32_t clip;       // k_clip waves
     uint32_t bins;       // k_bin blocks
     uint32_t fine;       // k_fine_bin blocks
     uint32_t raster;     // persistent raster workgroups; 0 = one one-wave workgroup per quadrant
     uint32_t ntiles;     // tiles the one-per-tile raster grid must cover (>= the job's)
-    uint32_t scenes;     // scene count (k_frame_reset / k_scene_prep loops)
+    uint32_t scenes;     // scene count (k_job_init / k_scene_prep loops)
 };
 
 // A block of words passed to a kernel by value (kernel arguments hold up to

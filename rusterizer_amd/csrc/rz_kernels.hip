@@ -6,12 +6,12 @@
 // same association, Rust's saturating float->int casts).
 //
 // Pipeline of one job (S scenes, D draws; all on one stream):
-//   k_frame_reset -> k_mvp -> k_classify -> k_clip -> scan(nsub)
+//   k_job_init (counter reset, MVPs) -> k_classify -> k_clip -> scan(nsub)
 //   -> k_scene_prep -> k_setup                                  (geometry)
 //   -> k_bin<count> -> scan -> k_bin<scatter>                   (coarse bins)
 //   -> k_fine_prep -> scan64 -> k_fine_bin<count> -> scan
-//   -> k_fine_bin<scatter>                                      (16x16 tiles)
-//   -> k_raster (coverage, depth, shading, resolve)
+//   -> k_fine_bin<scatter>                      (8, 16 or 32 px fine tiles)
+//   -> k_tile_prep -> k_raster_q (coverage, depth, shading, resolve)
 // Rasterizer triangles ("prims") are stored per scene in submission order
 // (draw, input triangle, clip-fan index), so every later stage keeps the
 // reference's order per pixel without sorting. Each scene's prims start at a
@@ -114,13 +114,12 @@ __device__ __forceinline__ float4 mat_vec_p(const float* __restrict__ M, float x
     return r;
 }
 
-// main.rs:147-152: projection * view * world, evaluated left to right
-// (matrix.rs:56-79). One thread per draw; written into the draw descriptor.
 // per-chunk owner tables, one thread per chunk (the searches run in parallel
 // here instead of serially at the start of every workgroup that needs them):
-// k_tchunk_map: the draw of each TRI_ALIGN-triangle chunk (draws start at
-// chunk boundaries); k_pchunk_map: the scene of each CHUNK-prim binning chunk
-__global__ __launch_bounds__(256) void k_tchunk_map(JobBufs b) {
+// tchunk_map (in k_job_init): the draw of each TRI_ALIGN-triangle chunk (draws
+// start at chunk boundaries); k_pchunk_map: the scene of each CHUNK-prim
+// binning chunk
+__device__ __forceinline__ void tchunk_map(const JobBufs& b) {
     const uint32_t nc = min((b.head->ntri + TRI_ALIGN - 1) / TRI_ALIGN, b.tchunk_cap), nd = b.head->ndraws;
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
         b.tchunk_draw[c] = find_draw(b.draws, nd, c * TRI_ALIGN);
@@ -132,7 +131,9 @@ __global__ __launch_bounds__(256) void k_pchunk_map(JobBufs b) {
         b.pchunk_scene[c] = find_scene_chunk(b.runs, ns, c);
 }
 
-__global__ __launch_bounds__(256) void k_mvp(JobBufs b) {
+// main.rs:147-152: projection * view * world, evaluated left to right
+// (matrix.rs:56-79). One thread per draw; written into the draw descriptor.
+__device__ __forceinline__ void draw_mvp(const JobBufs& b) {
     const uint32_t nd = b.head->ndraws;
     for (uint32_t d = blockIdx.x * blockDim.x + threadIdx.x; d < nd; d += gridDim.x * blockDim.x) {
         const Mat3& m = b.mats[d];
@@ -174,7 +175,7 @@ __device__ __forceinline__ void load_vertex(const DrawDev& dd, uint32_t vi, floa
 // Job start: zero the counters and the per-scene flags (a kernel, not a
 // memset, so the sequence can be captured in a graph).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_frame_reset(JobBufs b) {
+__device__ __forceinline__ void frame_reset(const JobBufs& b) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         FrameCounters* c = b.cnt;
         c->nprims = 0;
@@ -194,6 +195,15 @@ __global__ __launch_bounds__(256) void k_frame_reset(JobBufs b) {
     const uint32_t ns = b.head->nscenes;
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x)
         b.runs[s].flags = 0;
+}
+
+// k_job_init: the job's three independent first passes in one launch (each a
+// grid-stride loop, so any grid serves): the counter and flag reset, the
+// draws' matrices and the triangle chunks' draws
+__global__ __launch_bounds__(256) void k_job_init(JobBufs b) {
+    frame_reset(b);
+    draw_mvp(b);
+    tchunk_map(b);
 }
 
 // ---------------------------------------------------------------------------
@@ -453,7 +463,7 @@ __global__ __launch_bounds__(64) void k_clip_redo(JobBufs b) {
 
 // ---------------------------------------------------------------------------
 // Exclusive scan of u32 or u64 (3 launches: tile sums, scan of sums, tile
-// scan). The length is *n_dev when given (bounded by n_host, which also sizes
+// scan; one launch for short arrays, k_scan_one). The length is *n_dev when given (bounded by n_host, which also sizes
 // the grid), else n_host. The grand total goes to *total (if given).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t scan_len(uint32_t n_host, const uint32_t* n_dev) {
@@ -543,12 +553,50 @@ __global__ __launch_bounds__(256) void k_scan_down(const T* __restrict__ in, uin
     }
 }
 
+// The same scan by one workgroup, SCAN_ONE_PASS elements per pass with the
+// running total carried between passes: for short arrays, whose scan would
+// otherwise be three launches of a few microseconds each (a small frame's
+// job is mostly such launches).
+constexpr uint32_t SCAN_ONE_PASS = 1024u * SCAN_ITEMS;
+constexpr uint32_t SCAN_ONE_MAX = 4u * SCAN_ONE_PASS;
+
+template <class T>
+__global__ __launch_bounds__(1024) void k_scan_one(const T* __restrict__ in, uint32_t n_host,
+                                                   const uint32_t* __restrict__ n_dev, T* __restrict__ out,
+                                                   T* __restrict__ total) {
+    __shared__ T lds[16];
+    const uint32_t n = scan_len(n_host, n_dev);
+    T run = 0;
+    for (uint32_t base = 0; base < n; base += SCAN_ONE_PASS) {
+        T v[SCAN_ITEMS];
+        T s = 0;
+        for (int i = 0; i < SCAN_ITEMS; ++i) {
+            const uint32_t k = base + threadIdx.x * SCAN_ITEMS + i;
+            v[i] = k < n ? in[k] : T(0);
+            s += v[i];
+        }
+        T tot;
+        T ex = run + block_exclusive_scan<T>(s, lds, &tot);
+        for (int i = 0; i < SCAN_ITEMS; ++i) {
+            const uint32_t k = base + threadIdx.x * SCAN_ITEMS + i;
+            if (k < n) out[k] = ex;
+            ex += v[i];
+        }
+        run += tot;
+    }
+    if (threadIdx.x == 0 && total) *total = run;
+}
+
 template <class T>
 void launch_scan(hipStream_t s, const T* in, T* out, uint32_t n_host, const uint32_t* n_dev, T* parts, T* total) {
+    if (n_host <= SCAN_ONE_MAX) {
+        k_scan_one<T><<<1, 1024, 0, s>>>(in, n_host, n_dev, out, total);
+        return;
+    }
     uint32_t nb = (n_host + SCAN_TILE - 1) / SCAN_TILE;
-    if (nb > 0) k_scan_reduce<T><<<nb, 256, 0, s>>>(in, n_host, n_dev, parts);
+    k_scan_reduce<T><<<nb, 256, 0, s>>>(in, n_host, n_dev, parts);
     k_scan_top<T><<<1, 1024, 0, s>>>(parts, n_host, n_dev, total);
-    if (nb > 0) k_scan_down<T><<<nb, 256, 0, s>>>(in, n_host, n_dev, parts, out);
+    k_scan_down<T><<<nb, 256, 0, s>>>(in, n_host, n_dev, parts, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -2285,10 +2333,9 @@ void launch_raster_service(hipStream_t s, uint32_t grid, SvcEntry* ents, uint32_
 
 void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_stats, hipEvent_t ev_bin,
                 hipEvent_t ev_raster, SvcEntry* svc, SvcHost* svc_host, SvcCtl* svc_ctl, uint32_t svc_index) {
-    const uint32_t scene_blocks = at_least1((g.scenes + 255) / 256);
-    k_frame_reset<<<scene_blocks, 256, 0, s>>>(b);
-    k_mvp<<<at_least1(g.draws), 256, 0, s>>>(b);
-    k_tchunk_map<<<at_least1(std::min<uint32_t>((g.tris + 255) / 256 + 1, 1024u)), 256, 0, s>>>(b);
+    const uint32_t scene_blocks = (g.scenes + 255) / 256;
+    const uint32_t tchunk_blocks = std::min<uint32_t>((g.tris + 255) / 256 + 1, 1024u);
+    k_job_init<<<at_least1(std::max({scene_blocks, g.draws, tchunk_blocks})), 256, 0, s>>>(b);
     k_classify<<<at_least1(std::min<uint32_t>(g.tris, 2048u)), 256, 0, s>>>(b, b.tchunk_draw, b.draws);
     k_clip<<<at_least1((g.clip + CLIP_GROUPS - 1) / CLIP_GROUPS), 64, 0, s>>>(b);
     k_clip_redo<<<64, 64, 0, s>>>(b);
