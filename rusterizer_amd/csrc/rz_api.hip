@@ -157,7 +157,7 @@ struct rz_ctx {
     // pipeline buffers
     DBuf<uint32_t> cls, nsub, suboff, parts;
     DBuf<unsigned long long> parts64;
-    DBuf<uint32_t> clip_tri, clip_redo, clip_draw;
+    DBuf<uint32_t> clip_tri, clip_draw;
     DBuf<float> clip_out;  // MAXV * 10 floats per slot
     DBuf<PrimGeom> geom;
     DBuf<float> attrs;  // ATTR_STRIDE per prim
@@ -475,7 +475,6 @@ int size_buffers(rz_ctx* c, const JobPlan& pl, const Needs& n_in, G& g) {
     g(c->pchunk_scene, geom / CHUNK + 1);
     if (c->raster_stats) g(c->rstat_parts, 5ull * (4ull * pl.ntiles + 512 + 32ull * (uint32_t)c->n_cu));
     const size_t clip = g(c->clip_tri, n.clip);
-    g(c->clip_redo, clip);
     g(c->clip_draw, clip);
     g(c->clip_out, clip * MAXV * 10);
     const size_t list = g(c->list, n.pairs);
@@ -557,7 +556,7 @@ uint64_t held(const DBuf<T>& b) {
 // device bytes of the context's pipeline buffers, outputs and tables
 uint64_t ctx_bytes(const rz_ctx* c) {
     uint64_t n = held(c->cls) + held(c->nsub) + held(c->suboff) + held(c->parts) + held(c->parts64) +
-                 held(c->clip_tri) + held(c->clip_redo) + held(c->clip_draw) + held(c->clip_out) + held(c->geom) +
+                 held(c->clip_tri) + held(c->clip_draw) + held(c->clip_out) + held(c->geom) +
                  held(c->attrs) + held(c->uvat) + held(c->pbin) + held(c->bin_counts) + held(c->bin_offsets) + held(c->list) +
                  held(c->bin_rng) + held(c->tile_rec) + held(c->fchunk_rec) + held(c->tchunk_draw) +
                  held(c->pchunk_scene) + held(c->rstat_parts) + held(c->fpack) + held(c->fpre) +
@@ -582,7 +581,6 @@ JobBufs job_bufs(const rz_ctx* c) {
     b.nsub = c->nsub.p;
     b.suboff = c->suboff.p;
     b.clip_tri = c->clip_tri.p;
-    b.clip_redo = c->clip_redo.p;
     b.clip_draw = c->clip_draw.p;
     b.clip_out = c->clip_out.p;
     b.geom = c->geom.p;
@@ -1161,7 +1159,6 @@ bool noalloc_layout(unsigned char* base, uint32_t ntri_pad, const SceneCfg& sc, 
     b.pbin = cv.take<uint4>(prims);
     b.pchunk_scene = cv.take<uint32_t>(prims / CHUNK + 1);
     b.clip_tri = cv.take<uint32_t>(clip);
-    b.clip_redo = cv.take<uint32_t>(clip);
     b.clip_draw = cv.take<uint32_t>(clip);
     b.clip_out = cv.take<float>(clip * MAXV * 10);
     b.bin_counts = cv.take<uint32_t>(mat);
@@ -1378,7 +1375,6 @@ int rz_destroy(rz_ctx* c) {
     release(c->parts);
     release(c->parts64);
     release(c->clip_tri);
-    release(c->clip_redo);
     release(c->clip_draw);
     release(c->clip_out);
     release(c->geom);
@@ -1599,7 +1595,6 @@ int rz_trim(rz_ctx* c) {
     release(c->parts);
     release(c->parts64);
     release(c->clip_tri);
-    release(c->clip_redo);
     release(c->clip_draw);
     release(c->clip_out);
     release(c->geom);

@@ -49,8 +49,7 @@ struct FrameCounters {
     uint32_t fine_chunks;   // chunks of the coarse lists (fine binning work items)
     uint32_t fmat_len;      // fine count matrix length (written with fine_chunks as one u64)
     uint32_t q[8];          // raster work queue heads, one per XCD band
-    uint32_t nredo;         // clipped polygons handed from k_clip to k_clip_redo
-    uint32_t pad;
+    uint32_t pad[2];
     // raster work counters (stats variant only), per wave: list entries
     // scanned, candidates after the list culls, prims processed (some pixel
     // could take them); per sample: samples covered by processed prims; per
@@ -150,7 +149,6 @@ struct JobBufs {
     uint32_t* nsub;
     uint32_t* suboff;
     uint32_t* clip_tri;   // clipped triangle of each clip slot
-    uint32_t* clip_redo;  // slots k_clip left to k_clip_redo
     uint32_t* clip_draw;  // its draw
     float* clip_out;
     PrimGeom* geom;
@@ -180,7 +178,7 @@ struct JobBufs {
     uint32_t tri_cap, clip_cap, prim_cap, pair_cap, mat_cap, fine_cap, fmat_cap, bins_cap, fchunk_cap;
     uint32_t tile_cap;  // raster tiles tile_rec holds (4 records each)
     uint32_t tchunk_cap, pchunk_cap, rstat_cap;  // rstat_cap in workgroups
-    uint32_t clip_wide;    // clip every polygon with a full wave (k_clip_redo)
+    uint32_t clip_wide;    // clip every polygon with a full wave
     uint32_t exact_paths;  // setup proves nothing: every prim takes the raster's exact paths
     uint32_t* svc_flags;   // raster service: pinned host array of the scenes' final flags
 };

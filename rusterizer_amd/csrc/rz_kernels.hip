@@ -186,7 +186,6 @@ __device__ __forceinline__ void frame_reset(const JobBufs& b) {
         c->mat_len = 0;
         c->pairs_binned = 0;
         c->fine_chunks = 0;
-        c->nredo = 0;
         c->fmat_len = 0;
         c->fine_total = 0;
         for (int i = 0; i < 8; ++i) c->q[i] = 0;
@@ -315,16 +314,18 @@ __global__ __launch_bounds__(256) void k_classify(JobBufs b, const uint32_t* __r
 // k_clip runs CLIP_GROUPS groups of 64 / CLIP_GROUPS lanes per wave. A
 // polygon that would need more lanes than its group has at some plane (a
 // triangle clipped by all six planes can reach 9 vertices; rounding can add
-// more) is handed to k_clip_redo, which clips it again from the start with
-// all 64 lanes.
+// more) is clipped again from the start by the whole wave, once its groups
+// are done.
 // ---------------------------------------------------------------------------
 struct ClipV {
     float x, y, z, w;
     float a[6];
 };
 
+// Returns true in the lanes of a group whose polygon needs more than G lanes
+// (nothing is written for it then).
 template <uint32_t G>
-__device__ __forceinline__ void clip_polygon(const JobBufs& b, uint32_t slot, bool have, float* __restrict__ sh) {
+__device__ __forceinline__ bool clip_polygon(const JobBufs& b, uint32_t slot, bool have, float* __restrict__ sh) {
     const uint32_t lane = threadIdx.x & 63u, gl = lane & (G - 1), gb = lane - gl;
     const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << gb);
     float* q = sh + gb * 10;  // this group's staging area: G vertices
@@ -410,17 +411,11 @@ __device__ __forceinline__ void clip_polygon(const JobBufs& b, uint32_t slot, bo
             }
         }
     }
-    if (!have) return;
+    if (!have) return false;
     if (big) {
-        if (gl == 0) {
-            if (G < 64) {
-                const uint32_t r = atomicAdd(&b.cnt->nredo, 1u);
-                b.clip_redo[r] = slot;  // r < clip slots: each slot is handed over at most once
-            } else {
-                atomicOr(&b.cnt->overflow, OVF_POLY);  // more vertices than lanes: cannot continue
-            }
-        }
-        return;
+        // more vertices than a wave's lanes: cannot continue
+        if (G == 64 && gl == 0) atomicOr(&b.cnt->overflow, OVF_POLY);
+        return G < 64;
     }
     uint32_t ntris = 0;
     if (n > MAXV) {
@@ -436,35 +431,38 @@ __device__ __forceinline__ void clip_polygon(const JobBufs& b, uint32_t slot, bo
         if (gl == 0) atomicOr(&b.runs[scene].flags, RZ_FLAG_CLIP_FEWER);
     }
     if (gl == 0) b.nsub[t] = ntris;
+    return false;
 }
 
-// CLIP_GROUPS clipped triangles per wave (with clip_wide set, every one goes to k_clip_redo)
+// CLIP_GROUPS clipped triangles per wave, then with all 64 lanes each of them
+// that needed more than its group (with clip_wide set, every one is clipped
+// with all 64 lanes)
 __global__ __launch_bounds__(64) void k_clip(JobBufs b) {
     __shared__ float sh[64 * 10];
     const uint32_t nslots = min(uniform(b.cnt->nclip), b.clip_cap);
     if (b.clip_wide) {
-        for (uint32_t i = blockIdx.x * 64u + threadIdx.x; i < nslots; i += gridDim.x * 64u)
-            b.clip_redo[atomicAdd(&b.cnt->nredo, 1u)] = i;
+        for (uint32_t i = blockIdx.x; i < nslots; i += gridDim.x) (void)clip_polygon<64>(b, i, true, sh);
         return;
     }
     constexpr uint32_t G = 64 / CLIP_GROUPS;
     for (uint32_t base = blockIdx.x * CLIP_GROUPS; base < nslots; base += gridDim.x * CLIP_GROUPS) {
         const uint32_t slot = base + threadIdx.x / G;
-        clip_polygon<G>(b, slot, slot < nslots, sh);
+        const bool big = clip_polygon<G>(b, slot, slot < nslots, sh);
+        // one bit per group whose polygon was too big: lane G * k of group k
+        uint64_t redo = __ballot(big && (threadIdx.x % G) == 0);
+        while (redo != 0) {
+            const uint32_t k = (uint32_t)__ffsll((unsigned long long)redo) - 1u;
+            redo &= redo - 1;
+            (void)clip_polygon<64>(b, base + k / G, true, sh);
+        }
     }
-}
-
-// polygons k_clip could not finish: one wave each
-__global__ __launch_bounds__(64) void k_clip_redo(JobBufs b) {
-    __shared__ float sh[64 * 10];
-    const uint32_t nr = uniform(b.cnt->nredo);
-    for (uint32_t i = blockIdx.x; i < nr; i += gridDim.x) clip_polygon<64>(b, uniform(b.clip_redo[i]), true, sh);
 }
 
 // ---------------------------------------------------------------------------
 // Exclusive scan of u32 or u64 (3 launches: tile sums, scan of sums, tile
-// scan; one launch for short arrays, k_scan_one). The length is *n_dev when given (bounded by n_host, which also sizes
-// the grid), else n_host. The grand total goes to *total (if given).
+// scan; one launch for short arrays, k_scan_one). The length is *n_dev when
+// given (bounded by n_host, which also sizes the grid), else n_host. The
+// grand total goes to *total (if given).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t scan_len(uint32_t n_host, const uint32_t* n_dev) {
     if (n_dev == nullptr) return n_host;
@@ -561,11 +559,8 @@ constexpr uint32_t SCAN_ONE_PASS = 1024u * SCAN_ITEMS;
 constexpr uint32_t SCAN_ONE_MAX = 4u * SCAN_ONE_PASS;
 
 template <class T>
-__global__ __launch_bounds__(1024) void k_scan_one(const T* __restrict__ in, uint32_t n_host,
-                                                   const uint32_t* __restrict__ n_dev, T* __restrict__ out,
-                                                   T* __restrict__ total) {
-    __shared__ T lds[16];
-    const uint32_t n = scan_len(n_host, n_dev);
+__device__ __forceinline__ void scan_one(const T* __restrict__ in, uint32_t n, T* __restrict__ out,
+                                         T* __restrict__ total, T* lds) {
     T run = 0;
     for (uint32_t base = 0; base < n; base += SCAN_ONE_PASS) {
         T v[SCAN_ITEMS];
@@ -585,6 +580,14 @@ __global__ __launch_bounds__(1024) void k_scan_one(const T* __restrict__ in, uin
         run += tot;
     }
     if (threadIdx.x == 0 && total) *total = run;
+}
+
+template <class T>
+__global__ __launch_bounds__(1024) void k_scan_one(const T* __restrict__ in, uint32_t n_host,
+                                                   const uint32_t* __restrict__ n_dev, T* __restrict__ out,
+                                                   T* __restrict__ total) {
+    __shared__ T lds[16];
+    scan_one<T>(in, scan_len(n_host, n_dev), out, total, lds);
 }
 
 template <class T>
@@ -1029,7 +1032,7 @@ __device__ __forceinline__ uint32_t coarse_ofs(const JobBufs& b, uint32_t i, uin
     return i < mat_len ? b.bin_offsets[i] : tot;
 }
 
-__global__ __launch_bounds__(256) void k_fine_prep(JobBufs b) {
+__device__ __forceinline__ void fine_prep(const JobBufs& b) {
     const uint32_t NB = b.head->nbins, ns = b.head->nscenes;
     const bool bad = b.cnt->overflow != 0;
     const uint32_t mat_len = b.cnt->mat_len, tot = b.cnt->pairs_binned;
@@ -1051,6 +1054,7 @@ __global__ __launch_bounds__(256) void k_fine_prep(JobBufs b) {
         b.fpack[g] = ((unsigned long long)(T * n) << 32) | n;
     }
 }
+__global__ __launch_bounds__(256) void k_fine_prep(JobBufs b) { fine_prep(b); }
 
 // A fine-list entry is 8 bytes: {prim, key}, key = the prim's depth lower
 // bound zlo rounded toward -inf to a multiple of 2^QB ulps, with the low QB
@@ -1095,7 +1099,7 @@ __device__ __forceinline__ uint32_t fine_quad_bits(uint32_t fts, uint32_t x0, ui
 // k_fchunk_map, one thread per global coarse bin: writes {bin, scene} for
 // each of the bin's fine-binning chunks, so a chunk's workgroup finds its bin
 // with one load instead of two binary searches.
-__global__ __launch_bounds__(256) void k_fchunk_map(JobBufs b) {
+__device__ __forceinline__ void fchunk_map(const JobBufs& b) {
     if (b.cnt->overflow != 0) return;
     const uint32_t NB = b.head->nbins, ns = b.head->nscenes, cap = b.fchunk_cap;
     for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < NB; g += gridDim.x * blockDim.x) {
@@ -1104,6 +1108,23 @@ __global__ __launch_bounds__(256) void k_fchunk_map(JobBufs b) {
         const uint32_t s = find_scene_bin(b.scenes, ns, g);
         for (uint32_t c = 0; c < n && first + c < cap; ++c) b.fchunk_rec[first + c] = make_uint2(g, s);
     }
+}
+__global__ __launch_bounds__(256) void k_fchunk_map(JobBufs b) { fchunk_map(b); }
+
+// k_fine_prep, the u64 scan of fpack and k_fchunk_map by one workgroup, for
+// jobs of at most FINE_PREP_ONE_MAX coarse bins, one per thread (one launch
+// instead of three; the workgroup barriers order each step's global writes
+// before the next step's reads). With more bins the three parallel launches
+// win: a 4096-bin batch lost 0.03 ms in one workgroup.
+constexpr uint32_t FINE_PREP_ONE_MAX = 1024;
+__global__ __launch_bounds__(1024) void k_fine_prep_one(JobBufs b) {
+    __shared__ unsigned long long lds[16];
+    fine_prep(b);
+    __syncthreads();
+    scan_one<unsigned long long>(b.fpack, scan_len(b.bins_cap, &b.head->nbins), b.fpre,
+                                 reinterpret_cast<unsigned long long*>(&b.cnt->fine_chunks), lds);
+    __syncthreads();
+    fchunk_map(b);
 }
 
 template <bool SCATTER>
@@ -2338,7 +2359,6 @@ void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_s
     k_job_init<<<at_least1(std::max({scene_blocks, g.draws, tchunk_blocks})), 256, 0, s>>>(b);
     k_classify<<<at_least1(std::min<uint32_t>(g.tris, 2048u)), 256, 0, s>>>(b, b.tchunk_draw, b.draws);
     k_clip<<<at_least1((g.clip + CLIP_GROUPS - 1) / CLIP_GROUPS), 64, 0, s>>>(b);
-    k_clip_redo<<<64, 64, 0, s>>>(b);
     launch_scan<uint32_t>(s, b.nsub, b.suboff, b.tri_cap, &b.head->ntri, b.parts, &b.cnt->nprims_raw);
     k_scene_prep<<<1, 1024, 0, s>>>(b);
     k_pchunk_map<<<at_least1(std::min<uint32_t>(g.bins / 256 + 1, 1024u)), 256, 0, s>>>(b);
@@ -2349,11 +2369,15 @@ void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_s
     launch_scan<uint32_t>(s, b.bin_counts, b.bin_offsets, b.mat_cap, &b.cnt->mat_len, b.parts,
                           &b.cnt->pairs_binned);
     k_bin<true><<<at_least1(g.bins), 256, 0, s>>>(b, b.pchunk_scene, b.scenes, b.runs, b.pbin, b.bin_offsets);
-    k_fine_prep<<<at_least1(std::min<uint32_t>((b.bins_cap + 255) / 256, 1024u)), 256, 0, s>>>(b);
     // fine_chunks and fmat_len are adjacent: the u64 total fills both
-    launch_scan<unsigned long long>(s, b.fpack, b.fpre, b.bins_cap, &b.head->nbins, b.parts64,
-                                    reinterpret_cast<unsigned long long*>(&b.cnt->fine_chunks));
-    k_fchunk_map<<<at_least1(std::min<uint32_t>((b.bins_cap + 255) / 256, 1024u)), 256, 0, s>>>(b);
+    if (b.bins_cap <= FINE_PREP_ONE_MAX) {
+        k_fine_prep_one<<<1, 1024, 0, s>>>(b);
+    } else {
+        k_fine_prep<<<at_least1(std::min<uint32_t>((b.bins_cap + 255) / 256, 1024u)), 256, 0, s>>>(b);
+        launch_scan<unsigned long long>(s, b.fpack, b.fpre, b.bins_cap, &b.head->nbins, b.parts64,
+                                        reinterpret_cast<unsigned long long*>(&b.cnt->fine_chunks));
+        k_fchunk_map<<<at_least1(std::min<uint32_t>((b.bins_cap + 255) / 256, 1024u)), 256, 0, s>>>(b);
+    }
     k_fine_bin<false><<<at_least1(g.fine), 256, 0, s>>>(b, b.fchunk_rec, b.scenes, b.fpre, b.bin_rng, b.list,
                                                  b.fine_off);
     launch_scan<uint32_t>(s, b.fine_counts, b.fine_off, b.fmat_cap, &b.cnt->fmat_len, b.parts, &b.cnt->fine_total);
