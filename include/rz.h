@@ -208,7 +208,8 @@ int rz_batch_scene_flags(const rz_ctx *ctx, uint32_t scene, uint32_t *flags);
  * bin side automatically (1..256); raster_per_cu = persistent raster
  * workgroups per CU (0: one single-wave workgroup per 8x8 quadrant);
  * use_graph = replay the captured HIP graph (1) or launch the kernels
- * directly (0). Between jobs. */
+ * directly (0), for batches and for frames of at most 65536 triangles
+ * (while profiling is on, frames launch directly). Between jobs. */
 int rz_batch_tuning(rz_ctx *ctx, uint32_t bin_target, uint32_t raster_per_cu, int use_graph);
 /* The current tuning: [bin_target, raster_per_cu, use_graph]. */
 int rz_batch_get_tuning(const rz_ctx *ctx, uint32_t out[3]);

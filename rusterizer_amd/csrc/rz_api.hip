@@ -881,14 +881,23 @@ int complete_job(rz_ctx* c) {
     return 1;
 }
 
-int start_job(rz_ctx* c, bool graph) {
+// graph: replay the captured graph for jobs of at most graph_max_tris input
+// triangles, launch directly otherwise
+int start_job(rz_ctx* c, bool graph, uint64_t graph_max_tris = ~0ull) {
     int rc = build_job(c, c->plan);
     if (rc) return rc;
     c->replays = 0;
     c->ovf_seen = 0;
     c->in_flight = true;
-    return enqueue(c, graph);
+    return enqueue(c, graph && c->plan.tris <= graph_max_tris);
 }
+
+// Frames of at most this many input triangles replay the captured graph: one
+// submission instead of a launch per kernel, for frames whose time is the
+// host's launches (a 12-triangle cube frame: mean 0.115 -> 0.103 ms, p99
+// 0.19 -> 0.11 ms). Larger frames launch directly: the graph's grids, sized
+// to the buffer capacities, made a 1M-triangle frame about 1% slower.
+constexpr uint64_t FRAME_GRAPH_TRIS = 65536;
 
 int wait_job(rz_ctx* c) {
     for (;;) {
@@ -922,7 +931,9 @@ int frame_impl(rz_ctx* c, uint32_t* host_out, uint32_t* dev_out, uint32_t* host_
     c->scenes.clear();
     c->scenes.push_back(PendingScene{c->W, c->H, c->clear_depth, target, want_msaa ? c->msaa_color.p : nullptr,
                                      want_msaa ? c->msaa_depth.p : nullptr});
-    rc = start_job(c, false);
+    // small frames replay the captured graph; direct launches while
+    // profiling, whose stage events sit between the kernels
+    rc = start_job(c, c->use_graph && !c->prof, FRAME_GRAPH_TRIS);
     if (!rc) rc = wait_job(c);
     release_job(c);
     c->clear_depth = c->next_clear_depth;
