@@ -124,11 +124,14 @@ __device__ __forceinline__ void tchunk_map(const JobBufs& b) {
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
         b.tchunk_draw[c] = find_draw(b.draws, nd, c * TRI_ALIGN);
 }
-__global__ __launch_bounds__(256) void k_pchunk_map(JobBufs b) {
-    if (b.cnt->overflow != 0) return;
-    const uint32_t nc = min(b.cnt->nchunks, b.pchunk_cap), ns = b.head->nscenes;
+__device__ __forceinline__ void pchunk_map(const JobBufs& b, uint32_t nchunks) {
+    const uint32_t nc = min(nchunks, b.pchunk_cap), ns = b.head->nscenes;
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
         b.pchunk_scene[c] = find_scene_chunk(b.runs, ns, c);
+}
+__global__ __launch_bounds__(256) void k_pchunk_map(JobBufs b) {
+    if (b.cnt->overflow != 0) return;
+    pchunk_map(b, b.cnt->nchunks);
 }
 
 // main.rs:147-152: projection * view * world, evaluated left to right
@@ -558,9 +561,10 @@ __global__ __launch_bounds__(256) void k_scan_down(const T* __restrict__ in, uin
 constexpr uint32_t SCAN_ONE_PASS = 1024u * SCAN_ITEMS;
 constexpr uint32_t SCAN_ONE_MAX = 4u * SCAN_ONE_PASS;
 
+// (returns the total in every thread)
 template <class T>
-__device__ __forceinline__ void scan_one(const T* __restrict__ in, uint32_t n, T* __restrict__ out,
-                                         T* __restrict__ total, T* lds) {
+__device__ __forceinline__ T scan_one(const T* __restrict__ in, uint32_t n, T* __restrict__ out,
+                                      T* __restrict__ total, T* lds) {
     T run = 0;
     for (uint32_t base = 0; base < n; base += SCAN_ONE_PASS) {
         T v[SCAN_ITEMS];
@@ -580,6 +584,7 @@ __device__ __forceinline__ void scan_one(const T* __restrict__ in, uint32_t n, T
         run += tot;
     }
     if (threadIdx.x == 0 && total) *total = run;
+    return run;
 }
 
 template <class T>
@@ -587,7 +592,7 @@ __global__ __launch_bounds__(1024) void k_scan_one(const T* __restrict__ in, uin
                                                    const uint32_t* __restrict__ n_dev, T* __restrict__ out,
                                                    T* __restrict__ total) {
     __shared__ T lds[16];
-    scan_one<T>(in, scan_len(n_host, n_dev), out, total, lds);
+    (void)scan_one<T>(in, scan_len(n_host, n_dev), out, total, lds);
 }
 
 template <class T>
@@ -607,10 +612,10 @@ void launch_scan(hipStream_t s, const T* in, T* out, uint32_t n_host, const uint
 // rasterizer triangle range, its CHUNK-aligned prim slots, its binning
 // chunks and the base of its coarse count matrix (bins x chunks).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_scene_prep(JobBufs b) {
-    __shared__ uint32_t l1[16], l2[16];
+// (blockDim.x threads; total = the job's rasterizer triangles; returns the
+// job's binning chunks and coarse matrix length in every thread)
+__device__ __forceinline__ uint2 scene_prep(const JobBufs& b, uint32_t total, uint32_t* l1, uint32_t* l2) {
     const uint32_t ns = b.head->nscenes, NT = b.head->ntri;
-    const uint32_t total = b.cnt->nprims_raw;
     uint32_t run_ch = 0, run_mat = 0;
     for (uint32_t off = 0; off < ns; off += blockDim.x) {
         const uint32_t s = off + threadIdx.x;
@@ -648,6 +653,29 @@ __global__ __launch_bounds__(1024) void k_scene_prep(JobBufs b) {
         if (run_mat > b.mat_cap) o |= OVF_MATRIX;
         if (o) atomicOr(&c->overflow, o);
     }
+    return make_uint2(run_ch, run_mat);
+}
+__global__ __launch_bounds__(1024) void k_scene_prep(JobBufs b) {
+    __shared__ uint32_t l1[16], l2[16];
+    (void)scene_prep(b, b.cnt->nprims_raw, l1, l2);
+}
+
+// The scan of the prims per triangle, k_scene_prep and k_pchunk_map by one
+// workgroup, for jobs of at most SCAN_ONE_MAX (padded) triangles: one launch
+// instead of three. The totals each step needs pass in registers (every
+// thread holds them); the barriers order each step's global writes before
+// the next step's reads.
+__global__ __launch_bounds__(1024) void k_scene_prep_one(JobBufs b) {
+    __shared__ uint32_t l1[16], l2[16];
+    const bool bad = b.cnt->overflow != 0;  // from the earlier kernels
+    const uint32_t total = scan_one<uint32_t>(b.nsub, scan_len(b.tri_cap, &b.head->ntri), b.suboff,
+                                              &b.cnt->nprims_raw, l1);
+    __syncthreads();
+    const uint2 cm = scene_prep(b, total, l1, l2);
+    // the prim slots and the coarse matrix fit (the overflow bits scene_prep sets)
+    const bool fits = (uint64_t)cm.x * CHUNK <= b.prim_cap && cm.y <= b.mat_cap;
+    __syncthreads();
+    if (!bad && fits) pchunk_map(b, cm.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1121,8 +1149,8 @@ __global__ __launch_bounds__(1024) void k_fine_prep_one(JobBufs b) {
     __shared__ unsigned long long lds[16];
     fine_prep(b);
     __syncthreads();
-    scan_one<unsigned long long>(b.fpack, scan_len(b.bins_cap, &b.head->nbins), b.fpre,
-                                 reinterpret_cast<unsigned long long*>(&b.cnt->fine_chunks), lds);
+    (void)scan_one<unsigned long long>(b.fpack, scan_len(b.bins_cap, &b.head->nbins), b.fpre,
+                                       reinterpret_cast<unsigned long long*>(&b.cnt->fine_chunks), lds);
     __syncthreads();
     fchunk_map(b);
 }
@@ -2359,9 +2387,13 @@ void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_s
     k_job_init<<<at_least1(std::max({scene_blocks, g.draws, tchunk_blocks})), 256, 0, s>>>(b);
     k_classify<<<at_least1(std::min<uint32_t>(g.tris, 2048u)), 256, 0, s>>>(b, b.tchunk_draw, b.draws);
     k_clip<<<at_least1((g.clip + CLIP_GROUPS - 1) / CLIP_GROUPS), 64, 0, s>>>(b);
-    launch_scan<uint32_t>(s, b.nsub, b.suboff, b.tri_cap, &b.head->ntri, b.parts, &b.cnt->nprims_raw);
-    k_scene_prep<<<1, 1024, 0, s>>>(b);
-    k_pchunk_map<<<at_least1(std::min<uint32_t>(g.bins / 256 + 1, 1024u)), 256, 0, s>>>(b);
+    if (b.tri_cap <= SCAN_ONE_MAX) {
+        k_scene_prep_one<<<1, 1024, 0, s>>>(b);
+    } else {
+        launch_scan<uint32_t>(s, b.nsub, b.suboff, b.tri_cap, &b.head->ntri, b.parts, &b.cnt->nprims_raw);
+        k_scene_prep<<<1, 1024, 0, s>>>(b);
+        k_pchunk_map<<<at_least1(std::min<uint32_t>(g.bins / 256 + 1, 1024u)), 256, 0, s>>>(b);
+    }
     k_setup<<<at_least1(g.tris), 256, 0, s>>>(b, b.draws, b.scenes, b.runs, b.tchunk_draw, b.suboff, b.nsub, b.cls,
                                               b.clip_out);
     if (ev_bin) (void)hipEventRecord(ev_bin, s);
