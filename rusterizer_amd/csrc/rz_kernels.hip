@@ -976,9 +976,12 @@ struct ChunkBand {
 // work follows the bins a prim covers, not the bins of the scene. A bin's
 // count is the popcount of its column; a slot's rank in a bin is the popcount
 // of the column's bits below it, which keeps each bin's list in submission
-// order.
+// order. A prim covering more than WIDE_BINS bins has its bins set and
+// written by its whole wave, 64 at a time (a few screen-sized triangles would
+// otherwise leave single threads looping over hundreds of bins).
 // ---------------------------------------------------------------------------
 constexpr uint32_t BIN_WORDS = CHUNK / 32;
+constexpr uint32_t WIDE_BINS = 16;
 static_assert(MAX_BINS == 256 && CHUNK == 256, "one thread per bin and per chunk slot");
 
 template <bool SCATTER>
@@ -1014,11 +1017,24 @@ __global__ __launch_bounds__(256) void k_bin(JobBufs b, const uint32_t* __restri
         const bool ne = pv && (x0 < x1) && (y0 < y1);
         const uint32_t cx0 = x0 / cb, cx1 = ne ? (x1 - 1) / cb : 0u;
         const uint32_t cy0 = y0 / cb, cy1 = ne ? (y1 - 1) / cb : 0u;
+        const uint32_t bw = cx1 - cx0 + 1u, nbin = ne ? bw * (cy1 - cy0 + 1u) : 0u;
+        const bool wide = nbin > WIDE_BINS;
+        const uint64_t wides = __ballot(wide);
+        const uint32_t wave0 = t & ~63u, lane = t & 63u;
         uint32_t* row = bits + w * MAX_BINS;
         __syncthreads();
-        if (ne)
+        if (ne && !wide)
             for (uint32_t by = cy0; by <= cy1; ++by)
                 for (uint32_t bx = cx0; bx <= cx1; ++bx) atomicOr(&row[by * nbx + bx], 1u << (t & 31));
+        for (uint64_t wm = wides; wm != 0; wm &= wm - 1) {
+            // the wide prim of lane j: its bins i = lane, lane + 64, ... (row-major in its rectangle)
+            const uint32_t j = (uint32_t)__ffsll((unsigned long long)wm) - 1u, jt = wave0 + j;
+            const uint32_t jx0 = __builtin_amdgcn_readlane(cx0, j), jy0 = __builtin_amdgcn_readlane(cy0, j);
+            const uint32_t jw = __builtin_amdgcn_readlane(bw, j), jn = __builtin_amdgcn_readlane(nbin, j);
+            uint32_t* jrow = bits + (jt >> 5) * MAX_BINS;
+            for (uint32_t i = lane; i < jn; i += 64u)
+                atomicOr(&jrow[(jy0 + i / jw) * nbx + jx0 + i % jw], 1u << (jt & 31));
+        }
         __syncthreads();
         if (t < nb) {
             for (uint32_t k = 0; k < BIN_WORDS; ++k) {
@@ -1032,7 +1048,7 @@ __global__ __launch_bounds__(256) void k_bin(JobBufs b, const uint32_t* __restri
             __syncthreads();
             const uint4 en = make_uint4(p, pb.y, pb.z, pb.w);
             const uint32_t* wb = wbase + w * MAX_BINS;
-            if (ne)
+            if (ne && !wide)
                 for (uint32_t by = cy0; by <= cy1; ++by)
                     for (uint32_t bx = cx0; bx <= cx1; ++bx) {
                         const uint32_t bi = by * nbx + bx;
@@ -1040,6 +1056,22 @@ __global__ __launch_bounds__(256) void k_bin(JobBufs b, const uint32_t* __restri
                         if (pos < b.pair_cap) b.list[pos] = en;
                         else ovf = true;
                     }
+            for (uint64_t wm = wides; wm != 0; wm &= wm - 1) {
+                const uint32_t j = (uint32_t)__ffsll((unsigned long long)wm) - 1u, jt = wave0 + j;
+                const uint32_t jx0 = __builtin_amdgcn_readlane(cx0, j), jy0 = __builtin_amdgcn_readlane(cy0, j);
+                const uint32_t jw = __builtin_amdgcn_readlane(bw, j), jn = __builtin_amdgcn_readlane(nbin, j);
+                const uint4 jen = make_uint4(__builtin_amdgcn_readlane(en.x, j), __builtin_amdgcn_readlane(en.y, j),
+                                             __builtin_amdgcn_readlane(en.z, j), __builtin_amdgcn_readlane(en.w, j));
+                const uint32_t* jrow = bits + (jt >> 5) * MAX_BINS;
+                const uint32_t* jwb = wbase + (jt >> 5) * MAX_BINS;
+                const uint32_t jbelow = (1u << (jt & 31)) - 1u;
+                for (uint32_t i = lane; i < jn; i += 64u) {
+                    const uint32_t bi = (jy0 + i / jw) * nbx + jx0 + i % jw;
+                    const uint32_t pos = jwb[bi] + (uint32_t)__popc(jrow[bi] & jbelow);
+                    if (pos < b.pair_cap) b.list[pos] = jen;
+                    else ovf = true;
+                }
+            }
         }
         __syncthreads();  // bits and wbase are reused by the next chunk
     }
