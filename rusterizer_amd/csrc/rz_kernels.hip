@@ -976,9 +976,11 @@ struct ChunkBand {
 // work follows the bins a prim covers, not the bins of the scene. A bin's
 // count is the popcount of its column; a slot's rank in a bin is the popcount
 // of the column's bits below it, which keeps each bin's list in submission
-// order. A prim covering more than WIDE_BINS bins has its bins set and
-// written by its whole wave, 64 at a time (a few screen-sized triangles would
-// otherwise leave single threads looping over hundreds of bins).
+// order. The bins of prims covering more than WIDE_BINS bins are instead
+// spread over the whole workgroup: their (prim, bin) pairs are numbered
+// slot by slot (an exclusive scan of their bin counts) and thread t takes
+// pairs t, t + 256, ... (a few screen-sized triangles would otherwise leave
+// single threads looping over hundreds of bins).
 // ---------------------------------------------------------------------------
 constexpr uint32_t BIN_WORDS = CHUNK / 32;
 constexpr uint32_t WIDE_BINS = 16;
@@ -990,6 +992,8 @@ __global__ __launch_bounds__(256) void k_bin(JobBufs b, const uint32_t* __restri
                                              const uint4* __restrict__ pbin, const uint32_t* __restrict__ bin_offsets) {
     __shared__ uint32_t bits[BIN_WORDS * MAX_BINS];
     __shared__ uint32_t wbase[SCATTER ? BIN_WORDS * MAX_BINS : 1];
+    __shared__ uint32_t wpre[CHUNK];  // first wide (prim, bin) pair of each slot
+    __shared__ uint32_t scan_lds[CHUNK / 64];
     if (uniform(b.cnt->overflow) != 0) return;
     const uint32_t NC = uniform(b.cnt->nchunks);
     const uint32_t t = threadIdx.x, w = t >> 5, below = (1u << (t & 31)) - 1u;
@@ -1017,23 +1021,35 @@ __global__ __launch_bounds__(256) void k_bin(JobBufs b, const uint32_t* __restri
         const bool ne = pv && (x0 < x1) && (y0 < y1);
         const uint32_t cx0 = x0 / cb, cx1 = ne ? (x1 - 1) / cb : 0u;
         const uint32_t cy0 = y0 / cb, cy1 = ne ? (y1 - 1) / cb : 0u;
-        const uint32_t bw = cx1 - cx0 + 1u, nbin = ne ? bw * (cy1 - cy0 + 1u) : 0u;
+        const uint32_t nbin = ne ? (cx1 - cx0 + 1u) * (cy1 - cy0 + 1u) : 0u;
         const bool wide = nbin > WIDE_BINS;
-        const uint64_t wides = __ballot(wide);
-        const uint32_t wave0 = t & ~63u, lane = t & 63u;
         uint32_t* row = bits + w * MAX_BINS;
-        __syncthreads();
+        const bool any_wide = __syncthreads_or(wide) != 0;
         if (ne && !wide)
             for (uint32_t by = cy0; by <= cy1; ++by)
                 for (uint32_t bx = cx0; bx <= cx1; ++bx) atomicOr(&row[by * nbx + bx], 1u << (t & 31));
-        for (uint64_t wm = wides; wm != 0; wm &= wm - 1) {
-            // the wide prim of lane j: its bins i = lane, lane + 64, ... (row-major in its rectangle)
-            const uint32_t j = (uint32_t)__ffsll((unsigned long long)wm) - 1u, jt = wave0 + j;
-            const uint32_t jx0 = __builtin_amdgcn_readlane(cx0, j), jy0 = __builtin_amdgcn_readlane(cy0, j);
-            const uint32_t jw = __builtin_amdgcn_readlane(bw, j), jn = __builtin_amdgcn_readlane(nbin, j);
-            uint32_t* jrow = bits + (jt >> 5) * MAX_BINS;
-            for (uint32_t i = lane; i < jn; i += 64u)
-                atomicOr(&jrow[(jy0 + i / jw) * nbx + jx0 + i % jw], 1u << (jt & 31));
+        // wide pair q: slot s = the last with wpre[s] <= q (slots of no wide
+        // pairs have empty ranges), bin q - wpre[s] of its rectangle, row-major
+        uint32_t npairs = 0;
+        auto wide_pair = [&](uint32_t q, uint32_t& ws, uint4& wpb, uint32_t& wbi) {
+            uint32_t k = 0;
+            for (uint32_t step = CHUNK / 2; step >= 1; step >>= 1)
+                if (wpre[k + step] <= q) k += step;
+            ws = k;
+            wpb = pbin[chunk * CHUNK + k];
+            const uint32_t wx0 = (wpb.y & 0xFFFFu) / cb, wx1 = ((wpb.y >> 16) - 1u) / cb;
+            const uint32_t wy0 = (wpb.z & 0xFFFFu) / cb, wbw = wx1 - wx0 + 1u, i = q - wpre[k];
+            wbi = (wy0 + i / wbw) * nbx + wx0 + i % wbw;
+        };
+        if (any_wide) {
+            wpre[t] = block_exclusive_scan<uint32_t>(wide ? nbin : 0u, scan_lds, &npairs);
+            __syncthreads();
+            for (uint32_t q = t; q < npairs; q += CHUNK) {
+                uint32_t ws, wbi;
+                uint4 wpb;
+                wide_pair(q, ws, wpb, wbi);
+                atomicOr(&bits[(ws >> 5) * MAX_BINS + wbi], 1u << (ws & 31));
+            }
         }
         __syncthreads();
         if (t < nb) {
@@ -1056,21 +1072,14 @@ __global__ __launch_bounds__(256) void k_bin(JobBufs b, const uint32_t* __restri
                         if (pos < b.pair_cap) b.list[pos] = en;
                         else ovf = true;
                     }
-            for (uint64_t wm = wides; wm != 0; wm &= wm - 1) {
-                const uint32_t j = (uint32_t)__ffsll((unsigned long long)wm) - 1u, jt = wave0 + j;
-                const uint32_t jx0 = __builtin_amdgcn_readlane(cx0, j), jy0 = __builtin_amdgcn_readlane(cy0, j);
-                const uint32_t jw = __builtin_amdgcn_readlane(bw, j), jn = __builtin_amdgcn_readlane(nbin, j);
-                const uint4 jen = make_uint4(__builtin_amdgcn_readlane(en.x, j), __builtin_amdgcn_readlane(en.y, j),
-                                             __builtin_amdgcn_readlane(en.z, j), __builtin_amdgcn_readlane(en.w, j));
-                const uint32_t* jrow = bits + (jt >> 5) * MAX_BINS;
-                const uint32_t* jwb = wbase + (jt >> 5) * MAX_BINS;
-                const uint32_t jbelow = (1u << (jt & 31)) - 1u;
-                for (uint32_t i = lane; i < jn; i += 64u) {
-                    const uint32_t bi = (jy0 + i / jw) * nbx + jx0 + i % jw;
-                    const uint32_t pos = jwb[bi] + (uint32_t)__popc(jrow[bi] & jbelow);
-                    if (pos < b.pair_cap) b.list[pos] = jen;
-                    else ovf = true;
-                }
+            for (uint32_t q = t; q < npairs; q += CHUNK) {
+                uint32_t ws, wbi;
+                uint4 wpb;
+                wide_pair(q, ws, wpb, wbi);
+                const uint32_t wi = (ws >> 5) * MAX_BINS + wbi;
+                const uint32_t pos = wbase[wi] + (uint32_t)__popc(bits[wi] & ((1u << (ws & 31)) - 1u));
+                if (pos < b.pair_cap) b.list[pos] = make_uint4(chunk * CHUNK + ws, wpb.y, wpb.z, wpb.w);
+                else ovf = true;
             }
         }
         __syncthreads();  // bits and wbase are reused by the next chunk
