@@ -1181,11 +1181,12 @@ __device__ __forceinline__ void fchunk_map(const JobBufs& b) {
 __global__ __launch_bounds__(256) void k_fchunk_map(JobBufs b) { fchunk_map(b); }
 
 // k_fine_prep, the u64 scan of fpack and k_fchunk_map by one workgroup, for
-// jobs of at most FINE_PREP_ONE_MAX coarse bins, one per thread (one launch
+// jobs of at most FINE_PREP_ONE_MAX coarse bins, two per thread (one launch
 // instead of three; the workgroup barriers order each step's global writes
-// before the next step's reads). With more bins the three parallel launches
-// win: a 4096-bin batch lost 0.03 ms in one workgroup.
-constexpr uint32_t FINE_PREP_ONE_MAX = 1024;
+// before the next step's reads). A frame's bin buffers hold at least 1280
+// bins; with many more the three parallel launches win: a 4096-bin batch
+// lost 0.03 ms in one workgroup.
+constexpr uint32_t FINE_PREP_ONE_MAX = 2048;
 __global__ __launch_bounds__(1024) void k_fine_prep_one(JobBufs b) {
     __shared__ unsigned long long lds[16];
     fine_prep(b);
