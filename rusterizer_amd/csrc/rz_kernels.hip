@@ -554,19 +554,18 @@ __global__ __launch_bounds__(256) void k_scan_down(const T* __restrict__ in, uin
     }
 }
 
-// The same scan by one workgroup, SCAN_ONE_PASS elements per pass with the
-// running total carried between passes: for short arrays, whose scan would
-// otherwise be three launches of a few microseconds each (a small frame's
-// job is mostly such launches).
-constexpr uint32_t SCAN_ONE_PASS = 1024u * SCAN_ITEMS;
-constexpr uint32_t SCAN_ONE_MAX = 4u * SCAN_ONE_PASS;
+// The same scan by one workgroup, blockDim.x * SCAN_ITEMS elements per pass
+// with the running total carried between passes: for short arrays, whose
+// scan would otherwise be three launches of a few microseconds each (a small
+// frame's job is mostly such launches).
+constexpr uint32_t SCAN_ONE_MAX = 4u * 1024u * SCAN_ITEMS;
 
 // (returns the total in every thread)
 template <class T>
 __device__ __forceinline__ T scan_one(const T* __restrict__ in, uint32_t n, T* __restrict__ out,
                                       T* __restrict__ total, T* lds) {
     T run = 0;
-    for (uint32_t base = 0; base < n; base += SCAN_ONE_PASS) {
+    for (uint32_t base = 0; base < n; base += blockDim.x * SCAN_ITEMS) {
         T v[SCAN_ITEMS];
         T s = 0;
         for (int i = 0; i < SCAN_ITEMS; ++i) {
@@ -1180,14 +1179,17 @@ __device__ __forceinline__ void fchunk_map(const JobBufs& b) {
 }
 __global__ __launch_bounds__(256) void k_fchunk_map(JobBufs b) { fchunk_map(b); }
 
-// k_fine_prep, the u64 scan of fpack and k_fchunk_map by one workgroup, for
-// jobs of at most FINE_PREP_ONE_MAX coarse bins, two per thread (one launch
+// k_fine_prep, the u64 scan of fpack and k_fchunk_map by one workgroup of
+// 256 threads, for jobs of at most FINE_PREP_ONE_MAX coarse bins (one launch
 // instead of three; the workgroup barriers order each step's global writes
 // before the next step's reads). A frame's bin buffers hold at least 1280
 // bins; with many more the three parallel launches win: a 4096-bin batch
-// lost 0.03 ms in one workgroup.
+// lost 0.03 ms in one workgroup. 256 threads, not 1024: beside a resident
+// raster service, whose waves hold part of every SIMD's registers, a
+// 1024-thread workgroup of this kernel found no CU with room for its 16
+// waves until the service exited.
 constexpr uint32_t FINE_PREP_ONE_MAX = 2048;
-__global__ __launch_bounds__(1024) void k_fine_prep_one(JobBufs b) {
+__global__ __launch_bounds__(256) void k_fine_prep_one(JobBufs b) {
     __shared__ unsigned long long lds[16];
     fine_prep(b);
     __syncthreads();
@@ -2445,7 +2447,7 @@ void launch_job(hipStream_t s, const JobBufs& b, const JobGrid& g, bool raster_s
     k_bin<true><<<at_least1(g.bins), 256, 0, s>>>(b, b.pchunk_scene, b.scenes, b.runs, b.pbin, b.bin_offsets);
     // fine_chunks and fmat_len are adjacent: the u64 total fills both
     if (b.bins_cap <= FINE_PREP_ONE_MAX) {
-        k_fine_prep_one<<<1, 1024, 0, s>>>(b);
+        k_fine_prep_one<<<1, 256, 0, s>>>(b);
     } else {
         k_fine_prep<<<at_least1(std::min<uint32_t>((b.bins_cap + 255) / 256, 1024u)), 256, 0, s>>>(b);
         launch_scan<unsigned long long>(s, b.fpack, b.fpre, b.bins_cap, &b.head->nbins, b.parts64,
