@@ -225,6 +225,47 @@ def terrain_scene(quads: int = 1415, width: int = 3840, height: int = 2160, seed
     return Scene("terrain", width, height, pos, None, uv, idx, hm.identity(), view, proj, FS_TEXTURE, tex)
 
 
+def nine_vertex_scene(n_tris: int = 200, width: int = 160, height: int = 96, seed: int = 1005) -> Scene:
+    """Triangles that the frustum clips to 9-vertex polygons (clipping.rs:114-177
+    with identity matrices, so clip space is the cube |x|, |y|, |z| <= w = 1).
+    Each lies in a plane through the centre with normal near (1, 1, 1), which
+    cuts the cube in a (near-)regular hexagon of circumradius sqrt(2); the
+    triangle is equilateral about the centre with inradius r in (0.8, 1.3),
+    its edge normals pointing at alternate hexagon corners, so each edge cuts
+    one corner off (the neighbouring corners project to sqrt(2)/2 < r on the
+    edge normal) and the 6 hexagon edges plus 3 triangle edges remain. The
+    polygons outgrow the 8 lanes the clipper gives a polygon at first.
+    Colour FS, random colours; the planes differ per triangle (rotation about
+    the normal, inradius, a small tilt and offset), so the triangles overlap
+    at distinct depths."""
+    rng = np.random.default_rng(seed)
+    tris = []
+    for _ in range(n_tris):
+        nrm = np.array([1.0, 1.0, 1.0]) + rng.uniform(-0.08, 0.08, 3)
+        nrm /= np.linalg.norm(nrm)
+        # corner directions of the hexagon, projected into the plane
+        c0 = np.array([1.0, -1.0, 0.0])
+        c0 -= nrm * (c0 @ nrm)
+        c0 /= np.linalg.norm(c0)
+        c1 = np.cross(nrm, c0)
+        phi = rng.uniform(-0.15, 0.15)  # small turn: the edges still cut the same corners
+        r = rng.uniform(0.8, 1.3)
+        centre = nrm * rng.uniform(-0.05, 0.05)
+        pts = []
+        for k in range(3):
+            a = phi + 2.0 * math.pi * k / 3.0  # edge normal k at angle a; its opposite vertex at a + pi
+            d = -(math.cos(a) * c0 + math.sin(a) * c1)
+            pts.append(centre + 2.0 * r * d)
+        tris.append(pts)
+    pos = np.array(tris, np.float64).reshape(-1, 3).astype(np.float32)
+    n = pos.shape[0]
+    color = rng.random((n, 4)).astype(np.float32)
+    uv = np.zeros((n, 2), np.float32)
+    idx = np.arange(n, dtype=np.uint32)
+    return Scene(f"nine{n_tris}_{width}x{height}", width, height, pos, color, uv, idx,
+                 hm.identity(), hm.identity(), hm.identity(), FS_COLOR)
+
+
 def classify_fractions(sc: Scene) -> dict:
     """Shares of the scene's triangles by the reference's pre-clip
     classification (clipping.rs:62-112), from numpy f32 clip coordinates:

@@ -264,6 +264,20 @@ def test_wide_clipper_same_results(monkeypatch):
     assert_parity(wide, run_oracle(sc), where="wide clipper")
 
 
+def test_nine_vertex_polygons(monkeypatch):
+    """Polygons of 9 vertices (more than the 8 lanes the clipper gives a
+    polygon at first, so each is clipped again by the whole wave) render the
+    same frame as the all-wide clipper, equal to the oracle."""
+    sc = scenes.nine_vertex_scene()
+    narrow = run_gpu(sc)
+    monkeypatch.setenv("RZ_CLIP_WIDE", "1")
+    wide = run_gpu(sc)
+    assert narrow["stats"]["clipped"] == sc.ntri
+    assert np.array_equal(narrow["img"], wide["img"])
+    assert np.array_equal(narrow["dep"].view(np.uint32), wide["dep"].view(np.uint32))
+    assert_parity(narrow, run_oracle(sc), where="nine-vertex polygons")
+
+
 def test_exact_paths_same_results(monkeypatch):
     """With RZ_EXACT_PATHS=1 setup proves nothing, so every prim takes the
     raster's exact paths (f32::clamp of each barycentric, bounds-checked

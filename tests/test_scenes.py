@@ -40,3 +40,19 @@ def test_batch_scenes_distribution():
     assert all(s.texture.shape == (512, 512, 4) for s in scs)
     pal = {s.texture[0, 0].tobytes() for s in scs}
     assert len(pal) > 200  # per-scene palettes
+
+
+def test_nine_vertex_scene_clips_to_nine():
+    """The scene meant to outgrow the clipper's 8 lanes per polygon does: the
+    oracle's Sutherland-Hodgman (clipping.rs:114-190) turns most of its
+    triangles into 9-vertex polygons (7 fan triangles)."""
+    from oracle import oracle as O
+
+    sc = scenes.nine_vertex_scene()
+    m = O.mvp(sc.world, sc.view, sc.proj)
+    fans = []
+    for tri in sc.pos[sc.idx].reshape(-1, 3, 3):
+        v = np.array([O.mat_vec(m, np.array([p[0], p[1], p[2], 1.0], np.float32)) for p in tri], np.float32)
+        _, out, _ = O.try_clip(v)
+        fans.append(len(out))
+    assert max(fans) == 7 and fans.count(7) >= 150, sorted(set(fans))
